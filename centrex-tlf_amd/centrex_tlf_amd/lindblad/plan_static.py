@@ -1,0 +1,139 @@
+"""``prepare_lindblad_problem``: lower an OBE system once, upload it to the GPU once.
+
+Follows ``centrex_tlf/lindblad/plan_static.py:137-185``: parameters are adapted,
+the parameter graph and the decomposed Hamiltonian are lowered, the collapse
+operators are reduced to single jumps, and the expanded sparse RHS program is
+built.  Instead of a PyO3 plan the result carries a device plan created through
+the C ABI (``lb_plan_create``).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from .ir import flatten_plan, lower_expanded_sparse_rhs, lower_hamiltonian_decomposed, lower_parameter_graph
+from .parameters import LindbladParameters, adapt_lindblad_parameters
+from .state_layout import PackedHermitianLayout
+
+__all__ = ["PreparedLindbladProblem", "prepare_lindblad_problem", "lower_structured_jumps"]
+
+# Backends whose prepared problem owns a device plan.  "rust" is accepted so
+# code written for the reference runs unchanged and lands on the GPU.
+_DEVICE_BACKENDS = {"gpu", "rust"}
+_HOST_BACKENDS = {"host"}
+
+
+@dataclass
+class PreparedLindbladProblem:
+    obe_system: Any
+    parameters: LindbladParameters
+    layout: PackedHermitianLayout
+    parameter_graph: dict[str, Any]
+    hamiltonian_plan: dict[str, Any]
+    dense_c_array: np.ndarray
+    structured_jumps: list[dict[str, Any]]
+    source_decay_rates: np.ndarray
+    incoming_transfers_by_target: list[list[dict[str, Any]]]
+    expanded_rhs_plan: dict[str, Any] | None
+    flat_plan: dict[str, Any]
+    backend: str
+    gpu_plan: Any | None = field(default=None, repr=False)
+
+    def check_execution_mode(self, execution_mode: str) -> None:
+        """Fail fast when the mode needs the expanded sparse plan (``plan_static.py:37-56``)."""
+        if self.expanded_rhs_plan is None:
+            raise ValueError(
+                f"execution_mode={execution_mode!r} requires a decomposed Hamiltonian plan; "
+                "pass hamiltonian_representation='decomposed' to prepare_lindblad_problem"
+            )
+
+    def device_plan(self):
+        """The device plan, created on first use."""
+        if self.gpu_plan is None:
+            from .native import DevicePlan
+
+            self.gpu_plan = DevicePlan(self.flat_plan)
+        return self.gpu_plan
+
+    @property
+    def slot_names(self) -> list[str]:
+        return list(self.parameter_graph["slot_names"])
+
+    @property
+    def base_count(self) -> int:
+        return len(self.parameter_graph["base_values"])
+
+
+def lower_structured_jumps(c_array: np.ndarray):
+    """Single-jump collapse operators -> rates (``plan_static.py:107-134``).
+
+    Each ``C_k`` must have exactly one nonzero ``C_k[target, source]``; its rate
+    ``|C|^2`` leaves ``source`` and arrives at ``target``.
+    """
+    c_array = np.asarray(c_array, dtype=np.complex128)
+    n = int(c_array.shape[1])
+    jumps: list[dict[str, Any]] = []
+    source_decay = np.zeros(n, dtype=np.float64)
+    incoming: list[list[dict[str, Any]]] = [[] for _ in range(n)]
+    for idx, collapse in enumerate(c_array):
+        nz = np.argwhere(np.abs(collapse) > 0)
+        if nz.shape[0] != 1:
+            raise AssertionError(
+                f"collapse operator {idx} is not a single-jump operator; found {nz.shape[0]} nonzero elements"
+            )
+        target, source = int(nz[0, 0]), int(nz[0, 1])
+        amplitude = complex(collapse[target, source])
+        rate = float((amplitude * np.conjugate(amplitude)).real)
+        jumps.append({"target": target, "source": source, "amp_re": amplitude.real,
+                      "amp_im": amplitude.imag, "rate": rate})
+        incoming[target].append({"source": source, "rate": rate})
+        source_decay[source] += rate
+    return jumps, source_decay, incoming
+
+
+def prepare_lindblad_problem(obe_system: Any, parameters: Any, backend: str = "gpu",
+                             hamiltonian_representation: str = "decomposed") -> PreparedLindbladProblem:
+    """Lower ``obe_system`` with ``parameters`` and (for a device backend) upload it.
+
+    ``backend="gpu"`` (alias ``"rust"``) creates the device plan immediately;
+    ``backend="host"`` only lowers, which is enough to inspect the tables.
+    """
+    if backend not in _DEVICE_BACKENDS | _HOST_BACKENDS:
+        raise ValueError(f"unsupported backend {backend!r}; use 'gpu' (or 'rust') or 'host'")
+    if hamiltonian_representation not in {"decomposed", "auto"}:
+        raise NotImplementedError(
+            "the GPU solver runs the expanded sparse RHS, which needs "
+            "hamiltonian_representation='decomposed'"
+        )
+    params = adapt_lindblad_parameters(parameters)
+    params.check_hamiltonian_symbols(obe_system.H_symbolic)
+    n = int(obe_system.H_symbolic.shape[0])
+    layout = PackedHermitianLayout(n)
+    graph = lower_parameter_graph(params)
+    hamiltonian_plan = lower_hamiltonian_decomposed(
+        obe_system.H_symbolic, params.slot_index_by_name, graph["tuple_value_names"]
+    )
+    dense_c = np.asarray(obe_system.C_array, dtype=np.complex128)
+    jumps, source_decay, incoming = lower_structured_jumps(dense_c)
+    rhs_plan = lower_expanded_sparse_rhs(hamiltonian_plan, source_decay, incoming)
+    flat = flatten_plan(n, graph, hamiltonian_plan, rhs_plan)
+    prepared = PreparedLindbladProblem(
+        obe_system=obe_system,
+        parameters=params,
+        layout=layout,
+        parameter_graph=graph,
+        hamiltonian_plan=hamiltonian_plan,
+        dense_c_array=dense_c,
+        structured_jumps=jumps,
+        source_decay_rates=source_decay,
+        incoming_transfers_by_target=incoming,
+        expanded_rhs_plan=rhs_plan,
+        flat_plan=flat,
+        backend=backend,
+    )
+    if backend in _DEVICE_BACKENDS:
+        prepared.device_plan()
+    return prepared

@@ -1,0 +1,191 @@
+/*
+ * C ABI of the MI355X Lindblad / optical-Bloch solver.
+ *
+ * This header replaces the PyO3 module `centrex_tlf.centrex_tlf_rust`
+ * (registered at rust/src/lindblad/mod.rs:18-30).  Every entry point takes
+ * plain pointers and sizes, returns an int status (LB_OK on success) and
+ * leaves a message for lb_last_error() on failure; no C++ exception crosses
+ * it.  Array arguments marked "device" are HIP device pointers (for example
+ * torch tensors on a ROCm device); "host" arrays are ordinary memory.
+ *
+ * Packed state layout (n states, dim = n*n doubles): the n real diagonal
+ * entries, then (re, im) of the strict upper triangle in row-major order
+ * (centrex_tlf/lindblad/state_layout.py:11-79).
+ */
+#ifndef CENTREX_LINDBLAD_H
+#define CENTREX_LINDBLAD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. */
+#define LB_OK 0
+#define LB_ERR_INVALID 1     /* bad argument or plan description            */
+#define LB_ERR_DEVICE 2      /* HIP runtime failure                          */
+#define LB_ERR_UNSUPPORTED 3 /* valid request this build does not implement */
+
+/* Per-trajectory status written to lb_solve_args.out_status. */
+#define LB_TRAJ_OK 0
+#define LB_TRAJ_MAXITERS 1  /* "Stopped at x=... Need more than N steps."  rust/src/ode/dopri5.rs:226-228 */
+#define LB_TRAJ_UNDERFLOW 2 /* "Stopped at x=... Step size underflow."     dopri5.rs:229-231 */
+#define LB_TRAJ_STIFF 3     /* "Stiff at x=..."                             dopri5.rs:255-277 */
+#define LB_TRAJ_BAD_SPAN 4  /* t1 < t0 ("only forward integration")         dopri5.rs:175-177 */
+
+/* Solvers (rust/src/ode/batch.rs:7-27). */
+#define LB_SOLVER_DOPRI5 0
+#define LB_SOLVER_TSIT5 1
+
+/* Output kinds (rust/src/ode/output.rs). */
+#define LB_OUT_POPULATIONS 0 /* final diagonal, width n                          */
+#define LB_OUT_SELECTED 1    /* final selected entries, complex, width = count   */
+#define LB_OUT_INTEGRAL 2    /* trapezoid of sum_k w_k rho_kk over pushed points */
+#define LB_OUT_FULL 3        /* final packed state, width dim                     */
+
+/* Launch modes. */
+#define LB_MODE_PERSISTENT 0  /* one launch, persistent workgroups + work queue */
+#define LB_MODE_PLAIN 1       /* one launch per trajectory                      */
+
+/*
+ * Plan description: the lowered OBE system.  Mirrors the payload of
+ * prepare_lindblad_problem_py (rust/src/lindblad/python_api.rs:271-274,
+ * parsed by rust/src/lindblad/plan.rs:1155-1308):
+ *   - parameter graph: base slot values, compounds in evaluation order;
+ *   - decomposed Hamiltonian coefficients as bytecode (ops 1-19, builtins
+ *     1-7, helpers 1-16 of rust/src/lindblad/eval.rs:11-153);
+ *   - the expanded sparse RHS program over the n(n+1)/2 upper outputs
+ *     (centrex_tlf/lindblad/ir.py:447-613, rust/src/lindblad/plan.rs:649-662).
+ * Expressions are concatenated: first the n_compounds compound expressions,
+ * then the n_coefficients Hamiltonian coefficients; expr_ptrs has
+ * n_compounds + n_coefficients + 1 entries into the instr_* arrays.
+ * All arrays are host memory and are copied by lb_plan_create.
+ */
+typedef struct lb_plan_desc {
+  int32_t n_states;
+  int32_t n_slots;
+  int32_t n_base;
+  const double* base_re; /* n_base */
+  const double* base_im;
+  int32_t n_compounds;
+  const int32_t* compound_slots; /* n_compounds */
+  int32_t n_coefficients;
+  const int32_t* expr_ptrs;
+  const int32_t* instr_op;
+  const int32_t* instr_index;
+  const int32_t* instr_argc;
+  const int32_t* instr_function;
+  const double* instr_re;
+  const double* instr_im;
+  int64_t n_terms;
+  const int64_t* output_ptrs;   /* upper_len + 1 */
+  const int64_t* input_indices; /* n_terms, upper-triangle input index */
+  const int64_t* coeff_indices; /* n_terms, -1 = static (coefficient 1) */
+  const uint8_t* coeff_conj;
+  const uint8_t* input_conj;
+  const double* factors_re;
+  const double* factors_im;
+} lb_plan_desc;
+
+typedef struct lb_plan lb_plan;
+
+/*
+ * Build a device plan: validates the description (same checks as
+ * rust/src/lindblad/plan.rs:1069-1153), classifies coefficients as
+ * time-dependent (plan.rs:1235-1291), assigns upper outputs to lanes and
+ * uploads the tables to the current HIP device.
+ * Replaces prepare_lindblad_problem_py (python_api.rs:271-274).
+ */
+int lb_plan_create(const lb_plan_desc* desc, lb_plan** out_plan);
+void lb_plan_destroy(lb_plan* plan);
+
+/* Thread-local message for the last failing call on this thread. */
+const char* lb_last_error(void);
+
+/*
+ * Plan facts, written to info[0..15]:
+ *   0 n_states, 1 dim, 2 upper_len, 3 n_terms, 4 n_coefficients,
+ *   5 time_dependent (0/1), 6 n_published (elements shared through LDS),
+ *   7 n_gather_terms, 8 n_self_static_terms, 9 slot_count (padded),
+ *   10 gather_rows (sum over 64-lane blocks of the block's max gather count),
+ *   11 flops_per_rhs (algorithmic, see lb_solve_args.out_stats),
+ *   12 dynamic coefficient count, 13 lds bytes per trajectory.
+ */
+int lb_plan_info(const lb_plan* plan, int64_t* info);
+
+/*
+ * Batched RHS evaluation: out[b] = d(rho)/dt at (y[b], t[b]) with per-batch
+ * overrides of base parameter slots.  Replaces lindblad_rhs_py /
+ * rhs_packed_py (python_api.rs:276-352) for a batch; used to check the RHS
+ * kernel against the oracle.
+ *   y, out: device, batch x dim;  t: device, batch;
+ *   override_slots: host, n_override base-slot indices;
+ *   override_values: device, batch x n_override complex (re, im interleaved).
+ */
+int lb_rhs_eval(const lb_plan* plan, int64_t batch, const double* y, const double* t,
+                int32_t n_override, const int32_t* override_slots,
+                const double* override_values, double* out, void* stream);
+
+/*
+ * Arguments of lb_solve.  Trajectory b integrates y0 (row b of y0, or the
+ * shared row when y0_shared) from t0 to t1 (t1_per_traj[b] when given) with
+ * base-slot overrides taken either from override_values (row b) or, for a
+ * grid, from the Cartesian product of the axes with the last axis fastest
+ * (rust/src/lindblad/ode_batch.rs:353-391).
+ *
+ * Integral output follows the reference's trapezoid over pushed points
+ * (rust/src/ode/output.rs:315-330): t0 (when save_start) and every accepted
+ * step.  Populations / selected / full are the state at t1.
+ *
+ * out_stats (device, n_traj x 4 int64): accepted, rejected, rhs_calls,
+ * status (LB_TRAJ_*); out_fail_t (device, n_traj): x at failure.
+ */
+typedef struct lb_solve_args {
+  int64_t n_traj;
+  int32_t solver; /* LB_SOLVER_* */
+  double t0;
+  double t1;
+  const double* t1_per_traj; /* device or NULL */
+  double abstol;
+  double reltol;
+  double dt;
+  int64_t maxiters;
+  int32_t save_start;
+  const double* y0; /* device */
+  int32_t y0_shared;
+  int32_t n_override;
+  const int32_t* override_slots;  /* host */
+  const double* override_values;  /* device, n_traj x n_override complex; NULL for grid */
+  int32_t n_axes;                 /* grid form when > 0 (n_axes == n_override) */
+  const int64_t* axis_lengths;    /* host, n_axes */
+  const double* axis_values;      /* device, sum(axis_lengths) complex */
+  int32_t output_kind;            /* LB_OUT_* */
+  int32_t n_selected;
+  const int32_t* selected_rows;   /* host, n_selected (full-matrix indices) */
+  const int32_t* selected_cols;
+  int32_t n_weights;
+  const int32_t* weight_indices;  /* host, n_weights diagonal indices */
+  const double* weight_values;    /* host */
+  double* out_values;             /* device, n_traj x width (selected: complex pairs) */
+  int64_t* out_stats;             /* device, n_traj x 4 */
+  double* out_fail_t;             /* device, n_traj */
+  int32_t mode;                   /* LB_MODE_* */
+  int32_t threads_per_traj;       /* 0 = plan default; 64..1024, multiple of 64 */
+  int32_t workgroups;             /* persistent grid size, 0 = fill the device */
+} lb_solve_args;
+
+/*
+ * Integrate a batch of trajectories.  Replaces solve_lindblad_batch_ode_py
+ * (python_api.rs:530-669) and solve_lindblad_grid_ode_py (python_api.rs:671-799)
+ * for output_when="final" and the accepted-step integral; the single-trajectory
+ * solve_lindblad_ode_py (python_api.rs:422-528) is the n_traj = 1 case.
+ * Stream-ordered on `stream` (a hipStream_t, NULL = default stream).
+ */
+int lb_solve(const lb_plan* plan, const lb_solve_args* args, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CENTREX_LINDBLAD_H */

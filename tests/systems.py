@@ -1,0 +1,77 @@
+"""Benchmark systems of the workload configurations, built from the committed fixtures."""
+
+from __future__ import annotations
+
+import math
+from pathlib import Path
+
+import numpy as np
+
+from centrex_tlf_amd.lindblad import LindbladParameters, Time, gaussian, load_obe_system, square_wave
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+GAMMA = 2.0 * math.pi * 1.56e6
+
+
+def tlf_n39():
+    """X(J=0,1,2) + B(J=1,F1'=1/2,F'=1), one z laser with a Gaussian-in-time Rabi envelope.
+
+    Omega(t) = Omega0 * exp(-(t - t0)^2 / (2 sigma^2)), Omega0 = Gamma, sigma = 3/Gamma,
+    t0 = 10/Gamma, delta0 = 0; rho0 uniform over the four J=0 levels; t in [0, 20/Gamma].
+    """
+    system = load_obe_system(GOLDEN / "tlf_r0_f1_1o2_f1_n39.json")
+    params = LindbladParameters()
+    omega0 = params.real("Omega0", GAMMA)
+    delta0 = params.real("delta0", 0.0)
+    sigma = params.real("sigma", 3.0 / GAMMA)
+    tc = params.real("t_center", 10.0 / GAMMA)
+    params.bind("Ω0", gaussian(Time(), center=tc, sigma=sigma, amplitude=omega0), finalize=False)
+    params.bind("δ0", delta0, finalize=False)
+    params.bind("PZ0", 1.0, finalize=False)
+    params._finalize()
+    n = system.n_states
+    rho0 = np.zeros((n, n), dtype=np.complex128)
+    ground = system.indices(electronic="X", J=0)
+    for i in ground:
+        rho0[i, i] = 1.0 / len(ground)
+    excited = system.indices(electronic="B")
+    weights = [(int(i), GAMMA) for i in excited]
+    return system, params, rho0, (0.0, 20.0 / GAMMA), weights
+
+
+def tlf_n72(polarization_frequency: float = 2.0 * math.pi * 1e6):
+    """X(J=0..3) + B(J=1,F1'=3/2,F'=1,2); two lasers with x<->y polarisation switching, microwave J1-J2."""
+    system = load_obe_system(GOLDEN / "tlf_r0_f1_3o2_n72.json")
+    params = LindbladParameters()
+    omega0 = params.real("Omega0", GAMMA)
+    delta0 = params.real("delta0", 0.0)
+    sigma = params.real("sigma", 5.0 / GAMMA)
+    tc = params.real("t_center", 5.0 / GAMMA)
+    wpol = params.real("omega_pol", polarization_frequency)
+    envelope = gaussian(Time(), center=tc, sigma=sigma, amplitude=omega0)
+    names = set(str(s) for s in system.H_symbolic.free_symbols)
+    for k in (0, 1):
+        if f"Ω{k}" in names:
+            params.bind(f"Ω{k}", envelope, finalize=False)
+        if f"δ{k}" in names:
+            params.bind(f"δ{k}", delta0, finalize=False)
+        px = square_wave(Time(), wpol, 0.0)
+        if f"PX{k}" in names:
+            params.bind(f"PX{k}", px, finalize=False)
+        if f"PY{k}" in names:
+            params.bind(f"PY{k}", 1 - px, finalize=False)
+    if "Ω2" in names:
+        params.bind("Ω2", 0.5 * GAMMA, finalize=False)
+    if "δ2" in names:
+        params.bind("δ2", 0.0, finalize=False)
+    if "PZ2" in names:
+        params.bind("PZ2", 1.0, finalize=False)
+    params._finalize()
+    n = system.n_states
+    rho0 = np.zeros((n, n), dtype=np.complex128)
+    ground = system.indices(electronic="X", J=0)
+    for i in ground:
+        rho0[i, i] = 1.0 / len(ground)
+    excited = system.indices(electronic="B")
+    weights = [(int(i), GAMMA) for i in excited]
+    return system, params, rho0, (0.0, 2.0 * 5.0 / GAMMA), weights
