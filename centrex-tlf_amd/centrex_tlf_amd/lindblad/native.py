@@ -1,0 +1,236 @@
+"""ctypes binding of the HIP library (``liblindblad_gpu.so``, C ABI in ``include/centrex_lindblad.h``).
+
+Device memory and streams come from PyTorch-ROCm; the solver itself is the
+hand-written HIP code.  There is no CPU fallback: if the library or a HIP
+device is missing, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Any, Sequence
+
+import numpy as np
+
+from ._abi import LB_MODE_PERSISTENT, LB_MODE_PLAIN, LB_OUT, LB_SOLVER, LbSolveArgs, make_plan_desc, ptr
+
+__all__ = ["DevicePlan", "NativeError", "library", "library_path", "torch_device"]
+
+LIB_NAME = "liblindblad_gpu.so"
+_PKG_DIR = Path(__file__).resolve().parents[1]
+_INFO_KEYS = [
+    "n_states", "dim", "upper_len", "n_terms", "n_coefficients", "time_dependent", "n_published",
+    "n_gather_terms", "n_self_terms", "slot_count", "gather_rows", "flops_per_rhs",
+    "n_dynamic_coefficients", "lds_bytes", "slots_per_thread", "threads_per_traj",
+]
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def library_path() -> Path:
+    return _PKG_DIR / LIB_NAME
+
+
+_lib: C.CDLL | None = None
+
+
+def library() -> C.CDLL:
+    """Load the HIP library (torch first, so both share one HIP runtime)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401  (loads libamdhip64 before the library resolves it)
+
+    path = library_path()
+    if not path.exists():
+        raise NativeError(
+            f"{path} is not built; run __graft_entry__.build() (or make -C centrex-tlf_amd/csrc)"
+        )
+    L = C.CDLL(str(path))
+    L.lb_plan_create.restype = C.c_int
+    L.lb_plan_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    L.lb_plan_destroy.restype = None
+    L.lb_plan_destroy.argtypes = [C.c_void_p]
+    L.lb_last_error.restype = C.c_char_p
+    L.lb_plan_info.restype = C.c_int
+    L.lb_plan_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.lb_rhs_eval.restype = C.c_int
+    L.lb_rhs_eval.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32,
+                              C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]
+    L.lb_solve.restype = C.c_int
+    L.lb_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise NativeError(library().lb_last_error().decode())
+
+
+def torch_device():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise NativeError("no HIP device is available; the GPU solver needs an MI355X")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream_ptr() -> C.c_void_p:
+    import torch
+
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dptr(t) -> C.c_void_p | None:
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class DevicePlan:
+    """A lowered OBE system resident on the current HIP device."""
+
+    def __init__(self, flat: dict[str, Any]):
+        L = library()
+        torch_device()
+        desc, keep = make_plan_desc(flat)
+        handle = C.c_void_p()
+        rc = L.lb_plan_create(C.byref(desc), C.byref(handle))
+        del keep
+        _check(rc)
+        self._handle = handle
+        info = (C.c_int64 * 16)()
+        _check(L.lb_plan_info(handle, info))
+        self.info = dict(zip(_INFO_KEYS, [int(v) for v in info]))
+        self.n = self.info["n_states"]
+        self.dim = self.info["dim"]
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.lb_plan_destroy(h)
+            self._handle = None
+
+    # -- batched RHS ---------------------------------------------------------
+    def rhs(self, y, t, override_slots: Sequence[int] = (), override_values=None):
+        """d(rho)/dt for a batch of packed states (torch float64 tensors on the device)."""
+        import torch
+
+        dev = torch_device()
+        y = torch.as_tensor(y, dtype=torch.float64, device=dev).contiguous()
+        if y.ndim == 1:
+            y = y[None]
+        B = y.shape[0]
+        t = torch.as_tensor(t, dtype=torch.float64, device=dev).reshape(-1).expand(B).contiguous()
+        slots = np.ascontiguousarray(override_slots, dtype=np.int32)
+        ov = None
+        if slots.size:
+            ov = torch.view_as_real(torch.as_tensor(override_values, dtype=torch.complex128,
+                                                    device=dev).reshape(B, slots.size)).contiguous()
+        out = torch.zeros_like(y)
+        _check(library().lb_rhs_eval(self._handle, B, _dptr(y), _dptr(t), slots.size,
+                                     ptr(slots if slots.size else np.zeros(1, np.int32), C.c_int32),
+                                     _dptr(ov), _dptr(out), _stream_ptr()))
+        return out
+
+    # -- integration ---------------------------------------------------------
+    def solve(self, y0, t0: float, t1: float | Sequence[float], *, solver: str = "dopri5",
+              abstol: float = 1e-7, reltol: float = 1e-4, dt: float = 1e-8, maxiters: int = 100_000,
+              save_start: bool = True, output_kind: str = "populations",
+              selected: Sequence[tuple[int, int]] | None = None,
+              weights: Sequence[tuple[int, float]] | None = None,
+              override_slots: Sequence[int] = (), override_values=None,
+              grid_axes: Sequence[np.ndarray] | None = None, n_traj: int | None = None,
+              mode: str = "persistent", threads_per_traj: int = 0, workgroups: int = 0,
+              stream=None):
+        """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
+
+        ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
+        trajectory (``override_values`` of shape (B, n_override)) or a grid
+        (``grid_axes``, one array per overridden slot, last axis fastest).
+        """
+        import torch
+
+        dev = torch_device()
+        y0 = torch.as_tensor(y0, dtype=torch.float64, device=dev).contiguous()
+        shared = y0.ndim == 1
+        slots = np.ascontiguousarray(override_slots, dtype=np.int32)
+        axis_lengths = None
+        axis_values = ov = None
+        if grid_axes is not None:
+            axes = [np.asarray(a, dtype=np.complex128).reshape(-1) for a in grid_axes]
+            axis_lengths = np.asarray([a.size for a in axes], dtype=np.int64)
+            B = int(np.prod(axis_lengths))
+            axis_values = torch.view_as_real(torch.as_tensor(np.concatenate(axes), device=dev)).contiguous()
+        elif slots.size:
+            ovt = torch.as_tensor(override_values, dtype=torch.complex128, device=dev)
+            B = ovt.shape[0]
+            ov = torch.view_as_real(ovt.reshape(B, slots.size)).contiguous()
+        else:
+            B = int(n_traj) if n_traj is not None else (1 if shared else y0.shape[0])
+        if not shared and y0.shape[0] != B:
+            raise ValueError(f"y0 has {y0.shape[0]} rows for {B} trajectories")
+        t1_arr = None
+        if np.ndim(t1) > 0:
+            t1_arr = torch.as_tensor(np.asarray(t1, dtype=np.float64), device=dev).contiguous()
+            if t1_arr.numel() != B:
+                raise ValueError("per-trajectory t1 must have one entry per trajectory")
+            t1_scalar = float("nan")
+        else:
+            t1_scalar = float(t1)
+        kind = LB_OUT[output_kind]
+        if kind == LB_OUT["populations"]:
+            width = self.n
+        elif kind == LB_OUT["full"]:
+            width = self.dim
+        elif kind == LB_OUT["selected"]:
+            width = 2 * len(selected)
+        else:
+            width = 1
+        values = torch.full((B, width), float("nan"), dtype=torch.float64, device=dev)
+        stats = torch.zeros((B, 4), dtype=torch.int64, device=dev)
+        fail_t = torch.zeros(B, dtype=torch.float64, device=dev)
+        keep: list[np.ndarray] = [slots]
+        a = LbSolveArgs()
+        a.n_traj = B
+        a.solver = LB_SOLVER[solver]
+        a.t0 = float(t0)
+        a.t1 = t1_scalar
+        a.t1_per_traj = _dptr(t1_arr)
+        a.abstol, a.reltol, a.dt = float(abstol), float(reltol), float(dt)
+        a.maxiters = int(maxiters)
+        a.save_start = int(bool(save_start))
+        a.y0 = _dptr(y0)
+        a.y0_shared = int(shared)
+        a.n_override = int(slots.size)
+        a.override_slots = ptr(slots if slots.size else np.zeros(1, np.int32), C.c_int32)
+        a.override_values = _dptr(ov)
+        if axis_lengths is not None:
+            keep.append(axis_lengths)
+            a.n_axes = int(axis_lengths.size)
+            a.axis_lengths = ptr(axis_lengths, C.c_int64)
+            a.axis_values = _dptr(axis_values)
+        a.output_kind = kind
+        if selected is not None:
+            rows = np.asarray([i for i, _ in selected], dtype=np.int32)
+            cols = np.asarray([j for _, j in selected], dtype=np.int32)
+            keep += [rows, cols]
+            a.n_selected = rows.size
+            a.selected_rows, a.selected_cols = ptr(rows, C.c_int32), ptr(cols, C.c_int32)
+        if weights is not None:
+            wi = np.asarray([int(i) for i, _ in weights], dtype=np.int32)
+            wv = np.asarray([float(w) for _, w in weights], dtype=np.float64)
+            keep += [wi, wv]
+            a.n_weights = wi.size
+            a.weight_indices, a.weight_values = ptr(wi, C.c_int32), ptr(wv, C.c_double)
+        a.out_values = _dptr(values)
+        a.out_stats = _dptr(stats)
+        a.out_fail_t = _dptr(fail_t)
+        a.mode = LB_MODE_PLAIN if mode == "plain" else LB_MODE_PERSISTENT
+        a.threads_per_traj = int(threads_per_traj)
+        a.workgroups = int(workgroups)
+        sp = _stream_ptr() if stream is None else C.c_void_p(stream.cuda_stream)
+        _check(library().lb_solve(self._handle, C.byref(a), sp))
+        return values, stats, fail_t

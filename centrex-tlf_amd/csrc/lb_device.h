@@ -1,0 +1,116 @@
+// Device-side data layout of a Lindblad plan and of one solve call.
+//
+// Lane ownership.  The n(n+1)/2 upper-triangle outputs of the RHS program
+// (centrex_tlf/lindblad/ir.py:447-613) are permuted into S "slots", S a
+// multiple of 64.  A workgroup of W threads (W a multiple of 64) owns one
+// trajectory; thread t holds slots t, t+W, t+2W, ... in registers, so slot
+// s = lane + 64*(wave + (W/64)*r).  Each group of 64 consecutive slots (a
+// "block") is owned by one wave for one r; reductions sum a block with a
+// fixed butterfly and then the blocks in index order, so every result is
+// independent of W.
+//
+// Each slot carries one complex element of rho (the diagonal slots only use
+// the real part).  Per RHS evaluation an output is
+//   k_s = a_s * x_s + sum_g f_g * c_g(t) * x_{in(g)}
+// where a_s folds the time-independent terms that read the slot's own element
+// (computed once per trajectory), and the "gather" terms read other elements
+// that the owning threads publish to LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/centrex_lindblad.h"
+
+#define LB_MAX_OVERRIDES 16
+#define LB_MAX_AXES 8
+#define LB_MAX_WEIGHTS 64
+#define LB_MAX_SELECTED 128
+#define LB_MAX_STACK 24
+#define LB_STATIC_COEF 255u
+
+// gather word: bits 0..19 LDS publication index, bit 20 real input (diagonal),
+// bit 21 conjugated input, bit 22 conjugated coefficient, bits 24..31 coefficient id
+// (LB_STATIC_COEF = constant 1).
+__host__ __device__ inline uint32_t lb_gword(uint32_t pub, bool real_in, bool conj_in, bool conj_c,
+                                             uint32_t coef) {
+  return (pub & 0xFFFFFu) | (real_in ? (1u << 20) : 0u) | (conj_in ? (1u << 21) : 0u) |
+         (conj_c ? (1u << 22) : 0u) | (coef << 24);
+}
+
+struct DevInstr {
+  int32_t op, index, argc, function;
+  double re, im;
+};
+
+struct DevPlan {
+  int32_t n, dim, upper_len;
+  int32_t S;        // padded slot count (multiple of 64)
+  int32_t nblk;     // S / 64
+  int32_t n_pub;    // published elements
+  int32_t K;        // Hamiltonian coefficients
+  int32_t n_slots, n_base, n_compounds;
+  int32_t max_self; // self terms per slot (ELL width)
+  int32_t time_dependent;
+  // per slot
+  const int32_t* slot_up;    // upper index or -1 (padding)
+  const int32_t* slot_re;    // packed real index or -1
+  const int32_t* slot_im;    // packed imag index or -1 (diagonal / padding)
+  const int32_t* slot_pub;   // publication index or -1
+  const int32_t* self_coef;  // S*max_self, coefficient id or -1 (constant) ; -2 unused
+  const uint8_t* self_conj;  // S*max_self
+  const double* self_f;      // S*max_self*2
+  // gather ELL, rows grouped per block
+  const int32_t* blk_row0;   // nblk
+  const int32_t* blk_rows;   // nblk
+  const uint32_t* g_word;    // rows*64
+  const double* g_f;         // rows*64*2
+  // expression machine
+  const DevInstr* instr;
+  const int32_t* expr_start;    // n_compounds + K + 1
+  const int32_t* compound_slot; // n_compounds
+  const uint8_t* compound_dyn;  // n_compounds
+  const uint8_t* coef_dyn;      // K
+  const double* base;           // n_base*2
+};
+
+struct DevCall {
+  int64_t n_traj;
+  double t0, t1, abstol, reltol, dt;
+  int64_t maxiters;
+  int32_t save_start;
+  int32_t solver;
+  const double* t1_per_traj;
+  const double* y0;
+  int32_t y0_shared;
+  int32_t n_override;
+  int32_t override_slot[LB_MAX_OVERRIDES];
+  const double* override_values;  // n_traj x n_override x 2, or null for a grid
+  int32_t n_axes;
+  int64_t axis_len[LB_MAX_AXES];
+  int64_t axis_stride[LB_MAX_AXES];
+  int64_t axis_offset[LB_MAX_AXES];
+  const double* axis_values;
+  int32_t output_kind;
+  int32_t n_weights;
+  int32_t weight_index[LB_MAX_WEIGHTS];
+  double weight_value[LB_MAX_WEIGHTS];
+  int32_t n_selected;
+  int32_t sel_upper[LB_MAX_SELECTED];  // upper index of the stored element
+  int32_t sel_flags[LB_MAX_SELECTED];  // bit0 diagonal, bit1 conjugate
+  double* out_values;
+  int64_t* out_stats;
+  double* out_fail_t;
+  int32_t* queue;          // work-queue counter
+  int64_t traj_offset;     // plain mode: first trajectory of this launch
+};
+
+struct DevRhsCall {
+  int64_t batch;
+  const double* y;
+  const double* t;
+  int32_t n_override;
+  int32_t override_slot[LB_MAX_OVERRIDES];
+  const double* override_values;
+  double* out;
+};

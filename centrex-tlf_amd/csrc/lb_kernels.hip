@@ -1,0 +1,773 @@
+// HIP kernels of the MI355X Lindblad solver (gfx950, FP64).
+//
+//   lb_solve_kernel<R, SOLVER>  persistent fused Dopri5/Tsit5 integrator: one
+//                               trajectory per workgroup at a time, taken from a
+//                               device work queue; stage evaluation, RHS, error
+//                               norm, PI controller, FSAL and the photon-count
+//                               trapezoid all stay on the device.
+//   lb_rhs_kernel<R>            one RHS evaluation per workgroup (plan checks).
+//   lb_queue_reset_kernel       clears the work-queue counter (graph-safe, no memset).
+//
+// Method (reference files): expanded sparse RHS, rust/src/lindblad/rhs.rs:633-695,
+// 815-876; controller rust/src/ode/common.rs:32-70; Dopri5 rust/src/ode/dopri5.rs;
+// Tsit5 rust/src/ode/tsit5.rs; trapezoid collector rust/src/ode/output.rs:315-330.
+// Data layout: lb_device.h.
+
+#include "lb_device.h"
+
+#include <float.h>
+
+namespace {
+
+struct dcpx {
+  double re, im;
+};
+
+__device__ __forceinline__ dcpx mk(double r, double i) { return dcpx{r, i}; }
+__device__ __forceinline__ dcpx cadd(dcpx a, dcpx b) { return mk(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ dcpx csub(dcpx a, dcpx b) { return mk(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ dcpx cmul(dcpx a, dcpx b) {
+  return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+__device__ __forceinline__ dcpx cdiv(dcpx a, dcpx b) {
+  double ns = b.re * b.re + b.im * b.im;
+  return mk((a.re * b.re + a.im * b.im) / ns, (a.im * b.re - a.re * b.im) / ns);
+}
+__device__ __forceinline__ dcpx from_polar(double r, double th) {
+  double s, c;
+  sincos(th, &s, &c);
+  return mk(r * c, r * s);
+}
+// exp(z) = e^re (cos im, sin im); exact for a real argument.
+__device__ dcpx dev_cexp(dcpx z) {
+  if (z.im == 0.0) return mk(exp(z.re), 0.0);
+  return from_polar(exp(z.re), z.im);
+}
+// x^y = exp(y ln x) in polar form (x^0 = 1, 0^y = 0).  A real exponent skips the
+// exp/log factors, which are exactly 1 and 0 in that case; a positive real base
+// skips the angle, which is exactly 0.
+__device__ dcpx dev_cpow(dcpx a, dcpx b) {
+  if (b.re == 0.0 && b.im == 0.0) return mk(1.0, 0.0);
+  double r = (a.im == 0.0) ? fabs(a.re) : hypot(a.re, a.im);
+  if (r == 0.0) return mk(0.0, 0.0);
+  if (b.im == 0.0) {
+    double m = (b.re == 2.0) ? r * r : pow(r, b.re);
+    if (a.im == 0.0 && a.re > 0.0) return mk(m, 0.0);
+    return from_polar(m, b.re * atan2(a.im, a.re));
+  }
+  double th = atan2(a.im, a.re);
+  return from_polar(pow(r, b.re) * exp(-b.im * th), b.re * th + b.im * log(r));
+}
+
+__device__ double rem_euclid(double x, double m) {
+  double r = fmod(x, m);
+  return r < 0.0 ? r + fabs(m) : r;
+}
+
+__device__ double gaussian_2d(double x, double y, double a, double mx, double my, double sx,
+                              double sy) {
+  double dx = x - mx, dy = y - my;
+  return a * exp(-((dx * dx) / (2.0 * sx * sx) + (dy * dy) / (2.0 * sy * sy)));
+}
+
+__device__ double rabi_from_intensity(double intensity, double coupling, double dipole) {
+  const double hbar = 1.0545718176461565e-34, c = 299792458.0, eps0 = 8.8541878128e-12;
+  return sqrt(intensity * 2.0 / (c * eps0)) * coupling * dipole / hbar;
+}
+
+// Scalar helpers of rust/src/lindblad/eval.rs:155-451, 664-789 (ids 1-7, 9, 11, 12, 13, 15).
+__device__ dcpx dev_helper(int id, const dcpx* a, int argc) {
+  const double ED_XTB = 0.315 * 1.602176634e-19 * 5.29177210544e-11;
+#define A(i) (a[i].re)
+  switch (id) {
+    case 1: return mk(gaussian_2d(A(0), A(1), A(2), A(3), A(4), A(5), A(6)), 0.0);
+    case 2: {
+      double th = A(7), ct = cos(th), st = sin(th), sx = A(5), sy = A(6);
+      double p = ct * ct / (2.0 * sx * sx) + st * st / (2.0 * sy * sy);
+      double q = sin(2.0 * th) / (2.0 * sx * sx) - sin(2.0 * th) / (2.0 * sy * sy);
+      double u = st * st / (2.0 * sx * sx) + ct * ct / (2.0 * sy * sy);
+      double dx = A(0) - A(3), dy = A(1) - A(4);
+      return mk(A(2) * exp(-(p * dx * dx + q * dx * dy + u * dy * dy)), 0.0);
+    }
+    case 3: {
+      double phi = A(1) * sin(A(2) * A(0));
+      return mk(cos(phi), sin(phi));
+    }
+    case 4: return mk(0.5 * (1.0 + (sin(A(1) * A(0) + A(2)) >= 0.0 ? 1.0 : -1.0)), 0.0);
+    case 5: {
+      double th = 0.5 * A(1) * sin(A(2) * A(0));
+      double v = 0.5 * (cos(th) + sin(th));
+      return mk(v, v);
+    }
+    case 6: return mk(rem_euclid((A(1) * A(0) + A(2)) / (2.0 * M_PI), 1.0), 0.0);
+    case 7: {
+      double period = A(1) + A(2), duty = A(1) / period;
+      double frac = rem_euclid(2.0 * M_PI * A(0) / period + A(3), 2.0 * M_PI) / (2.0 * M_PI);
+      return mk(frac < duty ? 1.0 : 0.0, 0.0);
+    }
+    case 9: return mk(rabi_from_intensity(A(0), A(1), argc > 2 ? A(2) : ED_XTB), 0.0);
+    case 11: {
+      double inten = gaussian_2d(A(0), A(1), A(2), A(3), A(4), A(5), A(6));
+      return mk(rabi_from_intensity(inten, A(7), argc > 8 ? A(8) : ED_XTB), 0.0);
+    }
+    case 12: {
+      double frac = rem_euclid(A(0) * A(2) + A(3) / (2.0 * M_PI), 1.0);
+      if (frac <= 0.0) frac += 1.0;
+      return mk(frac < A(1) ? 1.0 : 0.0, 0.0);
+    }
+    case 13: {
+      long long k = (long long)floor((A(0) - A(1)) / A(2));
+      return mk(k % 2 == 0 ? 1.0 : -1.0, 0.0);
+    }
+    case 15: {
+      double dx = A(0) - A(1);
+      return mk(exp(-dx * dx / (2.0 * A(2) * A(2))), 0.0);
+    }
+    default: return mk(__builtin_nan(""), 0.0);  // rejected at plan creation
+  }
+#undef A
+}
+
+// Stack machine of rust/src/lindblad/eval.rs:791-911 (scalar form).  The plan
+// builder has checked ops, helper ids and the stack depth.
+__device__ dcpx vm_eval(const DevInstr* ins, int start, int end, const dcpx* slots, double t) {
+  dcpx st[LB_MAX_STACK];
+  int sp = 0;
+  for (int p = start; p < end; ++p) {
+    const DevInstr q = ins[p];
+    switch (q.op) {
+      case 1: st[sp++] = mk(q.re, q.im); break;
+      case 2: st[sp++] = slots[q.index]; break;
+      case 4: st[sp++] = mk(t, 0.0); break;
+      case 5: --sp; st[sp - 1] = cadd(st[sp - 1], st[sp]); break;
+      case 6: --sp; st[sp - 1] = csub(st[sp - 1], st[sp]); break;
+      case 7: --sp; st[sp - 1] = cmul(st[sp - 1], st[sp]); break;
+      case 8: --sp; st[sp - 1] = cdiv(st[sp - 1], st[sp]); break;
+      case 9: --sp; st[sp - 1] = dev_cpow(st[sp - 1], st[sp]); break;
+      case 10: st[sp - 1] = mk(-st[sp - 1].re, -st[sp - 1].im); break;
+      case 11: st[sp - 1].im = -st[sp - 1].im; break;
+      case 12: {
+        int s0 = sp - q.argc;
+        dcpx v = st[s0], r;
+        switch (q.function) {
+          case 1: r = mk(sin(v.re) * cosh(v.im), cos(v.re) * sinh(v.im)); break;
+          case 2: r = mk(cos(v.re) * cosh(v.im), -(sin(v.re) * sinh(v.im))); break;
+          case 3: {
+            double tr = v.re + v.re, ti = v.im + v.im, d = cos(tr) + cosh(ti);
+            r = mk(sin(tr) / d, sinh(ti) / d);
+            break;
+          }
+          case 4: r = dev_cexp(v); break;
+          case 5: r = mk(hypot(v.re, v.im), 0.0); break;
+          case 6: r = mk(v.re, 0.0); break;
+          default: r = mk(v.im, 0.0); break;
+        }
+        sp = s0;
+        st[sp++] = r;
+        break;
+      }
+      case 13: {
+        int s0 = sp - q.argc;
+        dcpx r = dev_helper(q.function, &st[s0], q.argc);
+        sp = s0;
+        st[sp++] = r;
+        break;
+      }
+      default: {
+        double rv = st[sp - 1].re, lv = st[sp - 2].re;
+        sp -= 2;
+        bool res = q.op == 14 ? lv > rv : q.op == 15 ? lv >= rv : q.op == 16 ? lv < rv
+                 : q.op == 17 ? lv <= rv : q.op == 18 ? lv == rv : lv != rv;
+        st[sp++] = mk(res ? 1.0 : 0.0, 0.0);
+        break;
+      }
+    }
+  }
+  return st[0];
+}
+
+// Butcher tableaus (rust/src/ode/dopri5.rs:5-37, rust/src/ode/tsit5.rs:5-38).
+template <int SOLVER>
+struct Tab;
+
+template <>
+struct Tab<0> {
+  __device__ static constexpr double c(int s) {
+    return s == 2 ? 1.0 / 5.0 : s == 3 ? 3.0 / 10.0 : s == 4 ? 4.0 / 5.0 : s == 5 ? 8.0 / 9.0 : 1.0;
+  }
+  __device__ static constexpr double a(int s, int j) {
+    return s == 2 ? (j == 1 ? 1.0 / 5.0 : 0.0)
+         : s == 3 ? (j == 1 ? 3.0 / 40.0 : j == 2 ? 9.0 / 40.0 : 0.0)
+         : s == 4 ? (j == 1 ? 44.0 / 45.0 : j == 2 ? -56.0 / 15.0 : j == 3 ? 32.0 / 9.0 : 0.0)
+         : s == 5 ? (j == 1 ? 19372.0 / 6561.0 : j == 2 ? -25360.0 / 2187.0 : j == 3 ? 64448.0 / 6561.0
+                     : j == 4 ? -212.0 / 729.0 : 0.0)
+         : s == 6 ? (j == 1 ? 9017.0 / 3168.0 : j == 2 ? -355.0 / 33.0 : j == 3 ? 46732.0 / 5247.0
+                     : j == 4 ? 49.0 / 176.0 : j == 5 ? -5103.0 / 18656.0 : 0.0)
+                  : (j == 1 ? 35.0 / 384.0 : j == 2 ? 0.0 : j == 3 ? 500.0 / 1113.0 : j == 4 ? 125.0 / 192.0
+                     : j == 5 ? -2187.0 / 6784.0 : j == 6 ? 11.0 / 84.0 : 0.0);
+  }
+  __device__ static constexpr double e(int j) {
+    return j == 1 ? 71.0 / 57600.0 : j == 2 ? 0.0 : j == 3 ? -71.0 / 16695.0 : j == 4 ? 71.0 / 1920.0
+         : j == 5 ? -17253.0 / 339200.0 : j == 6 ? 22.0 / 525.0 : -1.0 / 40.0;
+  }
+};
+
+template <>
+struct Tab<1> {
+  __device__ static constexpr double c(int s) {
+    return s == 2 ? 0.161 : s == 3 ? 0.327 : s == 4 ? 0.9 : s == 5 ? 0.9800255409045097 : 1.0;
+  }
+  __device__ static constexpr double a(int s, int j) {
+    return s == 2 ? (j == 1 ? 0.161 : 0.0)
+         : s == 3 ? (j == 1 ? -0.008480655492356989 : j == 2 ? 0.335480655492357 : 0.0)
+         : s == 4 ? (j == 1 ? 2.8971530571054935 : j == 2 ? -6.359448489975075 : j == 3 ? 4.3622954328695815 : 0.0)
+         : s == 5 ? (j == 1 ? 5.325864828439257 : j == 2 ? -11.748883564062828 : j == 3 ? 7.4955393428898365
+                     : j == 4 ? -0.09249506636175525 : 0.0)
+         : s == 6 ? (j == 1 ? 5.86145544294642 : j == 2 ? -12.92096931784711 : j == 3 ? 8.159367898576159
+                     : j == 4 ? -0.071584973281401 : j == 5 ? -0.028269050394068383 : 0.0)
+                  : (j == 1 ? 0.09646076681806523 : j == 2 ? 0.01 : j == 3 ? 0.4798896504144996
+                     : j == 4 ? 1.379008574103742 : j == 5 ? -3.290069515436081 : j == 6 ? 2.324710524099774 : 0.0);
+  }
+  __device__ static constexpr double e(int j) {
+    return j == 1 ? -0.0017800110522257771 : j == 2 ? -0.0008164344596567469 : j == 3 ? 0.007880878010261995
+         : j == 4 ? -0.1447110071732629 : j == 5 ? 0.5823571654525552 : j == 6 ? -0.458082105929187
+         : 1.0 / 66.0;
+  }
+};
+
+// Per-thread view of the slots it owns: LDS publication index and flags
+// (bit 0 valid, bit 1 diagonal, bits 2.. diagonal state index).
+template <int R>
+struct Slots {
+  int pub[R];
+  int flags[R];
+  __device__ __forceinline__ bool valid(int r) const { return flags[r] & 1; }
+  __device__ __forceinline__ bool diag(int r) const { return flags[r] & 2; }
+  __device__ __forceinline__ int state(int r) const { return flags[r] >> 2; }
+};
+
+// LDS carve-up of one workgroup (sizes from the plan, see lds_bytes_for).
+struct Lds {
+  double* pub;    // 2 buffers x 2*n_pub
+  double* selfc;  // 2*S folded self coefficients
+  double* wts;    // n integral weights per state
+  dcpx* coef;     // 8 x (K+1): [0] t0 and hinit, [1] hinit second point, [2..7] stages
+  dcpx* slots;    // 8 x n_slots
+  double* red;    // nblk x 4
+  int* misc;      // [0] queue value
+};
+
+__device__ __forceinline__ Lds carve(const DevPlan& P) {
+  extern __shared__ double smem[];
+  Lds L;
+  L.pub = smem;
+  L.selfc = L.pub + 4 * (size_t)P.n_pub;
+  L.wts = L.selfc + 2 * (size_t)P.S;
+  L.coef = reinterpret_cast<dcpx*>(L.wts + ((P.n + 1) & ~1));
+  L.slots = L.coef + 8 * (P.K + 1);
+  L.red = reinterpret_cast<double*>(L.slots + 8 * P.n_slots);
+  L.misc = reinterpret_cast<int*>(L.red + 4 * P.nblk);
+  return L;
+}
+
+// d(rho)/dt for the slots of this thread: publish the cross-coupled elements,
+// barrier, then self term + gathers.  `pubbuf` alternates between two buffers
+// so one barrier per evaluation suffices.
+template <int R>
+__device__ __forceinline__ void rhs_eval(const DevPlan& P, const Slots<R>& S, const double (&xr)[R],
+                                         const double (&xi)[R], double (&kr)[R], double (&ki)[R],
+                                         double* pubbuf, const double* selfc, const dcpx* coef,
+                                         int tid, int W) {
+  const int lane = tid & 63;
+  // wave-uniform block index: lets the row counts live in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (S.pub[r] >= 0) {
+      pubbuf[2 * S.pub[r]] = xr[r];
+      pubbuf[2 * S.pub[r] + 1] = xi[r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int blk = wave + (W >> 6) * r;
+    if (blk >= P.nblk) { kr[r] = 0.0; ki[r] = 0.0; continue; }
+    const int s = tid + W * r;
+    const double2 a = reinterpret_cast<const double2*>(selfc)[s];
+    double accr = a.x * xr[r] - a.y * xi[r];
+    double acci = a.x * xi[r] + a.y * xr[r];
+    const int rows = P.blk_rows[blk];
+    const int row0 = P.blk_row0[blk];
+    for (int j = 0; j < rows; ++j) {
+      const size_t idx = (size_t)(row0 + j) * 64 + lane;
+      const uint32_t w = P.g_word[idx];
+      const double2 f = reinterpret_cast<const double2*>(P.g_f)[idx];
+      const uint32_t cid = w >> 24;
+      double vr = f.x, vi = f.y;
+      if (cid != LB_STATIC_COEF) {
+        dcpx c = coef[cid];
+        if (w & (1u << 22)) c.im = -c.im;
+        vr = f.x * c.re - f.y * c.im;
+        vi = f.x * c.im + f.y * c.re;
+      }
+      const uint32_t p = w & 0xFFFFFu;
+      const double x_r = pubbuf[2 * p];
+      double x_i = (w & (1u << 20)) ? 0.0 : pubbuf[2 * p + 1];
+      if (w & (1u << 21)) x_i = -x_i;
+      accr += vr * x_r - vi * x_i;
+      acci += vr * x_i + vi * x_r;
+    }
+    kr[r] = accr;
+    ki[r] = S.diag(r) ? 0.0 : acci;
+  }
+}
+
+// Butterfly sum of one value over the 64 lanes of a wave; lane 0's result has
+// a fixed association order.
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// Store the block sums of slot row r into red[4*blk + q0 ...].
+__device__ __forceinline__ void block_store(const DevPlan& P, int tid, int W, int r, double* red, int q,
+                                            double value) {
+  const int s = tid + W * r;
+  const double sum = wave_sum(value);
+  if ((tid & 63) == 0 && s < P.S) red[4 * (s >> 6) + q] = sum;
+}
+
+// Sum the stored block values in block order (call after a barrier); the same
+// in every thread and independent of the workgroup size.
+__device__ __forceinline__ double block_total(const DevPlan& P, const double* red, int q) {
+  double s = 0.0;
+  for (int b = 0; b < P.nblk; ++b) s += red[4 * b + q];
+  return s;
+}
+
+__device__ __forceinline__ double time_tol(double t) { return 10.0 * DBL_EPSILON * fmax(fabs(t), 1.0); }
+
+// Evaluate the parameter graph + coefficients for time t into slot copy `sl`
+// and coefficient row `cf`; only_dynamic skips the time-independent parts.
+__device__ __noinline__ void eval_coefficients(const DevPlan& P, dcpx* sl, dcpx* cf, double t,
+                                               bool only_dynamic) {
+  for (int c = 0; c < P.n_compounds; ++c) {
+    if (only_dynamic && !P.compound_dyn[c]) continue;
+    sl[P.compound_slot[c]] = vm_eval(P.instr, P.expr_start[c], P.expr_start[c + 1], sl, t);
+  }
+  for (int k = 0; k < P.K; ++k) {
+    if (only_dynamic && !P.coef_dyn[k]) continue;
+    cf[k] = vm_eval(P.instr, P.expr_start[P.n_compounds + k], P.expr_start[P.n_compounds + k + 1], sl, t);
+  }
+  cf[P.K] = mk(1.0, 0.0);
+}
+
+template <int R>
+__device__ __forceinline__ void load_slots(const DevPlan& P, Slots<R>& S, int tid, int W) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int s = tid + W * r;
+    const bool in = s < P.S;
+    const int up = in ? P.slot_up[s] : -1;
+    const int im = in ? P.slot_im[s] : -1;
+    const int re = in ? P.slot_re[s] : 0;
+    S.flags[r] = (up >= 0 ? 1 : 0) | (im < 0 ? 2 : 0) | ((im < 0 && up >= 0 ? re : 0) << 2);
+    S.pub[r] = in ? P.slot_pub[s] : -1;
+  }
+}
+
+// Per-trajectory parameter slots: base values, then overrides (row of a batch
+// or a grid point, last axis fastest: rust/src/lindblad/ode_batch.rs:353-391).
+__device__ void init_slots(const DevPlan& P, const int32_t* ov_slot, int n_ov, const double* ov_row,
+                           const DevCall* C, int64_t traj, dcpx* sl, int tid, int W) {
+  for (int i = tid; i < P.n_slots; i += W)
+    sl[i] = i < P.n_base ? mk(P.base[2 * i], P.base[2 * i + 1]) : mk(0.0, 0.0);
+  __syncthreads();
+  if (tid == 0) {
+    for (int j = 0; j < n_ov; ++j) {
+      double vr, vi;
+      if (C != nullptr && C->n_axes > 0) {
+        const int64_t ai = (traj / C->axis_stride[j]) % C->axis_len[j];
+        vr = C->axis_values[2 * (C->axis_offset[j] + ai)];
+        vi = C->axis_values[2 * (C->axis_offset[j] + ai) + 1];
+      } else {
+        vr = ov_row[2 * j];
+        vi = ov_row[2 * j + 1];
+      }
+      sl[ov_slot[j]] = mk(vr, vi);
+    }
+  }
+  __syncthreads();
+}
+
+// Fold the time-independent self terms of every slot with this trajectory's
+// coefficient values into LDS (rhs.rs:901-919 folds static terms the same way).
+__device__ __forceinline__ void fold_self(const DevPlan& P, int tid, int W, const dcpx* coef,
+                                          double* selfc) {
+  for (int s = tid; s < P.S; s += W) {
+    dcpx acc = mk(0.0, 0.0);
+    for (int j = 0; j < P.max_self; ++j) {
+      const size_t e = (size_t)s * P.max_self + j;
+      const int id = P.self_coef[e];
+      if (id == -2) break;
+      const dcpx f = mk(P.self_f[2 * e], P.self_f[2 * e + 1]);
+      dcpx c = id < 0 ? mk(1.0, 0.0) : coef[id];
+      if (P.self_conj[e]) c.im = -c.im;
+      acc = cadd(acc, cmul(f, c));
+    }
+    selfc[2 * s] = acc.re;
+    selfc[2 * s + 1] = acc.im;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Persistent integrator.
+// ---------------------------------------------------------------------------
+template <int R, int SOLVER, int MAXW, int WAVES>
+__global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, const DevCall C) {
+  using T = Tab<SOLVER>;
+  const int tid = threadIdx.x, W = blockDim.x;
+  const Lds L = carve(P);
+  const int KK = P.K + 1;
+  Slots<R> S;
+  load_slots<R>(P, S, tid, W);
+
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) L.misc[0] = atomicAdd(C.queue, 1);
+    __syncthreads();
+    const int64_t local = L.misc[0];
+    if (local >= C.n_traj) break;
+    const int64_t traj = local + C.traj_offset;
+    const double t0 = C.t0;
+    const double t1 = C.t1_per_traj ? C.t1_per_traj[traj] : C.t1;
+
+    // ---- parameters and coefficients at t0, self-term folds, weights
+    init_slots(P, C.override_slot, C.n_override,
+               C.override_values ? C.override_values + 2 * traj * C.n_override : nullptr, &C, traj,
+               L.slots, tid, W);
+    if (tid == 0) eval_coefficients(P, L.slots, L.coef, t0, false);
+    for (int i = tid; i < P.n; i += W) {
+      double w = 0.0;
+      for (int q = 0; q < C.n_weights; ++q)
+        if (C.weight_index[q] == i) w += C.weight_value[q];
+      L.wts[i] = w;
+    }
+    __syncthreads();
+    for (int i = tid; i < 7 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
+    for (int i = tid; i < 7 * KK; i += W) L.coef[KK + i] = L.coef[i % KK];
+    fold_self(P, tid, W, L.coef, L.selfc);
+
+    // ---- state
+    double y_r[R], y_i[R];
+    const double* y0 = C.y0 + (C.y0_shared ? 0 : traj * (int64_t)P.dim);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int s = tid + W * r;
+      y_r[r] = S.valid(r) ? y0[P.slot_re[s]] : 0.0;
+      y_i[r] = (S.valid(r) && !S.diag(r)) ? y0[P.slot_im[s]] : 0.0;
+    }
+
+    int64_t accepted = 0, rejected = 0, rhs_calls = 0;
+    int status = 0;
+    double fail_x = 0.0, integral = 0.0, last_t = __builtin_nan(""), last_v = 0.0;
+    double k_r[8][R], k_i[8][R];
+    int buf = 0;
+#define PUB (L.pub + buf * 2 * P.n_pub)
+
+    if (t1 < t0) {
+      status = LB_TRAJ_BAD_SPAN;
+      fail_x = t0;
+    } else if (t1 > t0) {
+      double x = t0, h;
+      // ---- k1 = f(t0, y0) (also f0 of hinit); the barrier inside orders the LDS fills above
+      rhs_eval<R>(P, S, y_r, y_i, k_r[1], k_i[1], PUB, L.selfc, L.coef, tid, W);
+      buf ^= 1;
+      if (isfinite(C.dt) && C.dt > 0.0) {
+        h = fmin(C.dt, t1 - t0);
+      } else {
+        // hinit (rust/src/ode/common.rs:72-118)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double sr = C.abstol + fabs(y_r[r]) * C.reltol, si = C.abstol + fabs(y_i[r]) * C.reltol;
+          const double a0 = y_r[r] / sr, a1 = y_i[r] / si, b0 = k_r[1][r] / sr, b1 = k_i[1][r] / si;
+          const bool ok = S.valid(r), cpx_ = !S.diag(r);
+          block_store(P, tid, W, r, L.red, 0, ok ? a0 * a0 + (cpx_ ? a1 * a1 : 0.0) : 0.0);
+          block_store(P, tid, W, r, L.red, 1, ok ? b0 * b0 + (cpx_ ? b1 * b1 : 0.0) : 0.0);
+        }
+        __syncthreads();
+        const double d0 = block_total(P, L.red, 0), d1 = block_total(P, L.red, 1), hmax = fabs(t1 - x);
+        double h0 = (d0 < 1.0e-10 || d1 < 1.0e-10) ? 1.0e-6 : 0.01 * sqrt(d0 / d1);
+        h0 = fmin(h0, hmax);
+        if (tid == 0) eval_coefficients(P, L.slots + P.n_slots, L.coef + KK, x + h0, true);
+        double yt_r[R], yt_i[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          yt_r[r] = y_r[r] + k_r[1][r] * h0;
+          yt_i[r] = y_i[r] + k_i[1][r] * h0;
+        }
+        rhs_eval<R>(P, S, yt_r, yt_i, k_r[2], k_i[2], PUB, L.selfc, L.coef + KK, tid, W);
+        buf ^= 1;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double sr = C.abstol + fabs(y_r[r]) * C.reltol, si = C.abstol + fabs(y_i[r]) * C.reltol;
+          const double a0 = (k_r[2][r] - k_r[1][r]) / sr, a1 = (k_i[2][r] - k_i[1][r]) / si;
+          block_store(P, tid, W, r, L.red, 2, S.valid(r) ? a0 * a0 + (S.diag(r) ? 0.0 : a1 * a1) : 0.0);
+        }
+        __syncthreads();
+        const double d2 = sqrt(block_total(P, L.red, 2)) / h0;
+        const double h1 = fmax(sqrt(d1), fabs(d2)) <= 1.0e-15 ? fmax(1.0e-6, fabs(h0) * 1.0e-3)
+                                                              : pow(0.01 / fmax(sqrt(d1), d2), 1.0 / 5.0);
+        h = fmin(100.0 * fabs(h0), fmin(h1, hmax));
+      }
+      rhs_calls = 1;
+      // ---- first trapezoid point (output.rs:315-330)
+      if (C.output_kind == LB_OUT_INTEGRAL && C.save_start) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          block_store(P, tid, W, r, L.red, 1, S.diag(r) && S.valid(r) ? L.wts[S.state(r)] * y_r[r] : 0.0);
+        __syncthreads();
+        last_t = t0;
+        last_v = block_total(P, L.red, 1);
+      }
+      // ---- controller state (common.rs:32-70)
+      const double beta = 0.04, alpha = 0.2 - beta * 0.75, facc1 = 1.0 / 0.2, facc2 = 1.0 / 10.0;
+      const double safety = 0.9, hmax = fabs(t1 - t0);
+      double fac_old = 1.0e-4;
+      bool reject = false, last = false;
+      int64_t ns = 0, nst = 0, ia = 0;
+
+      while (!last) {
+        if (ns > C.maxiters) { status = LB_TRAJ_MAXITERS; fail_x = x; break; }
+        if (0.1 * fabs(h) <= DBL_EPSILON * fabs(x)) { status = LB_TRAJ_UNDERFLOW; fail_x = x; break; }
+        if (x + 1.01 * h - t1 > 0.0) { h = t1 - x; last = true; }
+        ++ns;
+        // time-dependent coefficients at the six stage times, one lane per stage
+        if (P.time_dependent && tid < 6) {
+          const int s = tid + 2;
+          eval_coefficients(P, L.slots + s * P.n_slots, L.coef + s * KK, x + h * T::c(s), true);
+        }
+        const bool need_stiff = ((accepted + 1) % 1000 == 0) || ia > 0;
+        double yt_r[R], yt_i[R], yn_r[R], yn_i[R];
+#pragma unroll
+        for (int s = 2; s <= 7; ++s) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            double sr = T::a(s, 1) * k_r[1][r], si = T::a(s, 1) * k_i[1][r];
+#pragma unroll
+            for (int j = 2; j < s; ++j) {
+              if (T::a(s, j) != 0.0) {
+                sr = fma(T::a(s, j), k_r[j][r], sr);
+                si = fma(T::a(s, j), k_i[j][r], si);
+              }
+            }
+            if (s < 7) {
+              yt_r[r] = fma(h, sr, y_r[r]);
+              yt_i[r] = fma(h, si, y_i[r]);
+            } else {
+              yn_r[r] = fma(h, sr, y_r[r]);
+              yn_i[r] = fma(h, si, y_i[r]);
+            }
+          }
+          if (s == 7) {
+            if (need_stiff) {
+              // |yn - y_stage6|^2 per block (dopri5.rs:255-264)
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const double a0 = yn_r[r] - yt_r[r], a1 = yn_i[r] - yt_i[r];
+                block_store(P, tid, W, r, L.red, 3, a0 * a0 + a1 * a1);
+              }
+            }
+            rhs_eval<R>(P, S, yn_r, yn_i, k_r[7], k_i[7], PUB, L.selfc, L.coef + 7 * KK, tid, W);
+          } else {
+            rhs_eval<R>(P, S, yt_r, yt_i, k_r[s], k_i[s], PUB, L.selfc, L.coef + s * KK, tid, W);
+          }
+          buf ^= 1;
+        }
+        rhs_calls += 6;
+        // ---- error norm (dopri5.rs:88-102, tsit5.rs:104-119), integrand, stiffness
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          double er = T::e(1) * k_r[1][r], ei = T::e(1) * k_i[1][r];
+#pragma unroll
+          for (int j = 2; j <= 7; ++j) {
+            if (T::e(j) != 0.0) {
+              er = fma(T::e(j), k_r[j][r], er);
+              ei = fma(T::e(j), k_i[j][r], ei);
+            }
+          }
+          er *= h;
+          ei *= h;
+          const double sr = C.abstol + fmax(fabs(y_r[r]), fabs(yn_r[r])) * C.reltol;
+          const double si = C.abstol + fmax(fabs(y_i[r]), fabs(yn_i[r])) * C.reltol;
+          const double qr = er / sr, qi = ei / si;
+          block_store(P, tid, W, r, L.red, 0, S.valid(r) ? qr * qr + (S.diag(r) ? 0.0 : qi * qi) : 0.0);
+          if (C.output_kind == LB_OUT_INTEGRAL)
+            block_store(P, tid, W, r, L.red, 1, S.diag(r) && S.valid(r) ? L.wts[S.state(r)] * yn_r[r] : 0.0);
+          if (need_stiff) {
+            const double a0 = k_r[7][r] - k_r[6][r], a1 = k_i[7][r] - k_i[6][r];
+            block_store(P, tid, W, r, L.red, 2, a0 * a0 + a1 * a1);
+          }
+        }
+        __syncthreads();
+        const double err = sqrt(block_total(P, L.red, 0) / (double)P.dim);
+        // ---- controller (common.rs:48-69)
+        const double fac11 = pow(err, alpha);
+        double fac = fac11 * pow(fac_old, -beta);
+        fac = fmax(facc2, fmin(facc1, fac / safety));
+        double hn = h / fac;
+        if (err <= 1.0) {
+          fac_old = fmax(err, 1.0e-4);
+          if (fabs(hn) > hmax) hn = hmax;
+          if (reject) hn = fmin(fabs(hn), fabs(h));
+          reject = false;
+          ++accepted;
+          if (accepted % 1000 == 0 || ia > 0) {
+            const double n2 = block_total(P, L.red, 2), dd2 = block_total(P, L.red, 3);
+            const double hl = dd2 > 0.0 ? h * sqrt(n2 / dd2) : 0.0;
+            if (hl > 3.25) {
+              ++ia;
+              nst = 0;
+              if (ia == 15) { status = LB_TRAJ_STIFF; fail_x = x; break; }
+            } else {
+              ++nst;
+              if (nst == 6) ia = 0;
+            }
+          }
+          x += h;
+          if (C.output_kind == LB_OUT_INTEGRAL) {
+            const double v = block_total(P, L.red, 1);
+            if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (x - last_t);
+            last_t = x;
+            last_v = v;
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            y_r[r] = yn_r[r];
+            y_i[r] = yn_i[r];
+            k_r[1][r] = k_r[7][r];
+            k_i[1][r] = k_i[7][r];
+          }
+          if (last) break;
+        } else {
+          hn = h / fmin(facc1, fac11 / safety);
+          reject = true;
+          last = false;
+          if (accepted >= 1) ++rejected;
+        }
+        h = hn;
+      }
+    }
+#undef PUB
+
+    // ---- outputs
+    const bool ok = status == 0;
+    const double nanv = __builtin_nan("");
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!S.valid(r)) continue;
+      const int s = tid + W * r;
+      if (C.output_kind == LB_OUT_POPULATIONS) {
+        if (S.diag(r)) C.out_values[traj * P.n + S.state(r)] = ok ? y_r[r] : nanv;
+      } else if (C.output_kind == LB_OUT_FULL) {
+        C.out_values[traj * P.dim + P.slot_re[s]] = ok ? y_r[r] : nanv;
+        if (!S.diag(r)) C.out_values[traj * P.dim + P.slot_im[s]] = ok ? y_i[r] : nanv;
+      } else if (C.output_kind == LB_OUT_SELECTED) {
+        const int up = P.slot_up[s];
+        for (int q = 0; q < C.n_selected; ++q) {
+          if (C.sel_upper[q] != up) continue;
+          const double im = (C.sel_flags[q] & 1) ? 0.0 : ((C.sel_flags[q] & 2) ? -y_i[r] : y_i[r]);
+          C.out_values[2 * (traj * C.n_selected + q)] = ok ? y_r[r] : nanv;
+          C.out_values[2 * (traj * C.n_selected + q) + 1] = ok ? im : nanv;
+        }
+      }
+    }
+    if (tid == 0) {
+      if (C.output_kind == LB_OUT_INTEGRAL) C.out_values[traj] = ok ? integral : nanv;
+      C.out_stats[4 * traj + 0] = accepted;
+      C.out_stats[4 * traj + 1] = rejected;
+      C.out_stats[4 * traj + 2] = rhs_calls;
+      C.out_stats[4 * traj + 3] = status;
+      C.out_fail_t[traj] = fail_x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched RHS (one evaluation per workgroup).
+// ---------------------------------------------------------------------------
+template <int R, int MAXW, int WAVES>
+__global__ __launch_bounds__(MAXW, WAVES) void lb_rhs_kernel(const DevPlan P, const DevRhsCall C) {
+  const int tid = threadIdx.x, W = blockDim.x;
+  const Lds L = carve(P);
+  Slots<R> S;
+  load_slots<R>(P, S, tid, W);
+  for (int64_t b = blockIdx.x; b < C.batch; b += gridDim.x) {
+    __syncthreads();
+    init_slots(P, C.override_slot, C.n_override,
+               C.override_values ? C.override_values + 2 * b * C.n_override : nullptr, nullptr, b,
+               L.slots, tid, W);
+    if (tid == 0) eval_coefficients(P, L.slots, L.coef, C.t[b], false);
+    __syncthreads();
+    fold_self(P, tid, W, L.coef, L.selfc);
+    double xr[R], xi[R], kr[R], ki[R];
+    const double* y = C.y + b * (int64_t)P.dim;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int s = tid + W * r;
+      xr[r] = S.valid(r) ? y[P.slot_re[s]] : 0.0;
+      xi[r] = (S.valid(r) && !S.diag(r)) ? y[P.slot_im[s]] : 0.0;
+    }
+    rhs_eval<R>(P, S, xr, xi, kr, ki, L.pub, L.selfc, L.coef, tid, W);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!S.valid(r)) continue;
+      const int s = tid + W * r;
+      C.out[b * (int64_t)P.dim + P.slot_re[s]] = kr[r];
+      if (!S.diag(r)) C.out[b * (int64_t)P.dim + P.slot_im[s]] = ki[r];
+    }
+  }
+}
+
+__global__ void lb_queue_reset_kernel(int32_t* q) { *q = 0; }
+
+// ---------------------------------------------------------------------------
+// Kernel table: (slots per thread R, largest workgroup, waves per SIMD hint).
+// The host picks a row with R * W >= plan slots; see lb_host.cpp.
+// ---------------------------------------------------------------------------
+#define LB_CONFIGS(X) X(1, 256, 4) X(2, 512, 4) X(3, 1024, 4) X(4, 256, 2) X(6, 512, 2) X(8, 128, 2)
+
+extern "C" void* lb_dev_solve_kernel(int R, int solver);
+extern "C" void* lb_dev_rhs_kernel(int R);
+extern "C" int lb_dev_max_threads(int R);
+extern "C" void* lb_dev_queue_reset_kernel(void);
+
+extern "C" void* lb_dev_solve_kernel(int R, int solver) {
+#define X(r, w, v)                                                                                 \
+  if (R == r) return solver == 1 ? (void*)&lb_solve_kernel<r, 1, w, v> : (void*)&lb_solve_kernel<r, 0, w, v>;
+  LB_CONFIGS(X)
+#undef X
+  return nullptr;
+}
+
+extern "C" void* lb_dev_rhs_kernel(int R) {
+#define X(r, w, v) \
+  if (R == r) return (void*)&lb_rhs_kernel<r, w, v>;
+  LB_CONFIGS(X)
+#undef X
+  return nullptr;
+}
+
+extern "C" int lb_dev_max_threads(int R) {
+#define X(r, w, v) \
+  if (R == r) return w;
+  LB_CONFIGS(X)
+#undef X
+  return 0;
+}
+
+extern "C" void* lb_dev_queue_reset_kernel(void) { return (void*)&lb_queue_reset_kernel; }
