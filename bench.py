@@ -1,0 +1,256 @@
+#!/usr/bin/env python3
+"""Benchmark: batched persistent GPU solve of the n=39 TlF 64x64 grid scan.
+
+Workload (one "step"): X(J=0,1,2) + B(J=1,F1'=1/2,F'=1), n = 39, one z laser
+with a Gaussian Rabi envelope Omega(t) = Omega0 exp(-(t-t0)^2/(2 sigma^2)),
+sigma = 3/Gamma, t0 = 10/Gamma, t in [0, 20/Gamma], Gamma = 2 pi 1.56 MHz;
+grid delta0 = linspace(-5, 5, 64) Gamma (slowest axis) x Omega0 =
+geomspace(0.1, 10, 64) Gamma -> 4096 trajectories, Dopri5, rtol 1e-8,
+atol 1e-10, dt 1e-8.  Each trajectory returns its final populations and the
+photon count (trapezoid over accepted steps) from one fused launch.  The
+initial state, the plan tables and the scan axes are resident on the GPU
+before the timed region.
+
+Also reported: the per-trajectory ("plain") GPU path on a seeded sample, the
+CPU oracle (this project's C restatement of the reference's Rust/rayon path)
+timed on the host cores on a bounded seeded sample, a parity check of the
+sample against the oracle, and the FP64 roofline of the solver kernel.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (str(ROOT / "centrex-tlf_amd"), str(ROOT)):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+F_RK = {"dopri5": 65, "tsit5": 67}  # algorithmic FLOPs per packed element per attempted step
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_lb_solve_kernel_n39_grid4096.json"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops(stats, flops_per_rhs: int, dim: int, solver: str) -> float:
+    """Sum over trajectories of rhs_calls*F_rhs + attempted_steps*dim*F_rk.
+
+    attempted steps = (rhs_calls - 1) / 6 (one FSAL call, then six per attempt).
+    """
+    import numpy as np
+
+    rhs = stats[:, 2].astype(np.float64)
+    attempted = (rhs - 1.0) / 6.0
+    return float(np.sum(rhs * flops_per_rhs + attempted * dim * F_RK[solver]))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--solver", default="dopri5", choices=["dopri5", "tsit5"])
+    ap.add_argument("--threads-per-traj", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0, help="kernel variant (0 = plan default)")
+    ap.add_argument("--workgroups", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
+    ap.add_argument("--plain-sample", type=int, default=128)
+    ap.add_argument("--profile", action="store_true", help="GPU steps only (for rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards "
+                                   "by trajectory range, see DESIGN.md)"}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from centrex_tlf_amd.lindblad import prepare_lindblad_problem
+    from centrex_tlf_amd.lindblad.native import GridAxes
+    from tests.systems import GAMMA, tlf_n39
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        return 1
+    dev = torch.device("cuda", 0)
+    system, params, rho0, t_span, weights = tlf_n39()
+    prepared = prepare_lindblad_problem(system, params, backend="gpu")
+    plan = prepared.device_plan()
+    info = plan.info
+    slot_names = prepared.slot_names
+    slots = [slot_names.index("delta0"), slot_names.index("Omega0")]
+    deltas = np.linspace(-5.0 * GAMMA, 5.0 * GAMMA, 64)
+    omegas = np.geomspace(0.1 * GAMMA, 10.0 * GAMMA, 64)
+    axes = GridAxes([deltas, omegas])
+    y0 = torch.as_tensor(prepared.layout.pack(rho0), device=dev)
+    tol = dict(abstol=1e-10, reltol=1e-8, dt=1e-8)
+    n_traj = axes.count
+
+    def step(**kw):
+        return plan.solve(y0, t_span[0], t_span[1], solver=args.solver, output_kind="populations",
+                          weights=weights, with_integral=True, override_slots=slots, grid_axes=axes,
+                          threads_per_traj=args.threads_per_traj, workgroups=args.workgroups,
+                          variant=args.variant, **tol, **kw)
+
+    log(f"bench: plan {info}")
+    for i in range(args.warmup):
+        out = step()
+        torch.cuda.synchronize()
+        log(f"bench: warmup {i + 1}/{args.warmup} done")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_begin = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record()
+        out = step()
+        ends[i].record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_begin
+    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    pops, stats, fail_t, photons = out
+    stats_np = stats.cpu().numpy()
+    if np.any(stats_np[:, 3] != 0):
+        bad = int(np.argmax(stats_np[:, 3] != 0))
+        log(f"bench: trajectory {bad} failed with status {stats_np[bad, 3]}")
+        return 1
+    log(f"bench: {args.steps} steps in {elapsed:.3f} s")
+    flops = algorithmic_flops(stats_np, info["flops_per_rhs"], info["dim"], args.solver)
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+    achieved_tflops = flops / avg_kernel_s / 1e12
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
+    result = {
+        "metric": "Lindblad trajectories/s (batched Dopri5, n=39, rtol 1e-8); adjoint grads/s",
+        "value": n_traj * args.steps / elapsed,
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (TlF n=39 system from the committed fixture; 64x64 detuning x Rabi grid)",
+        "config": {
+            "workload": "tlf_n39_grid_64x64",
+            "n_states": 39,
+            "trajectories": n_traj,
+            "solver": args.solver,
+            "rtol": 1e-8,
+            "atol": 1e-10,
+            "outputs": "final populations + photon count (one launch)",
+            "parallelism": "single_gpu",
+            "threads_per_traj": args.threads_per_traj or info["threads_per_traj"],
+        },
+        "roofline": {
+            "bound": "mfma",
+            "compute": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s",
+            "achieved": achieved_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel_ms": float(np.mean(kernel_ms)),
+            "flops_per_launch": flops,
+        },
+        "totals": {
+            "accepted_steps": int(stats_np[:, 0].sum()),
+            "rejected_steps": int(stats_np[:, 1].sum()),
+            "rhs_calls": int(stats_np[:, 2].sum()),
+            "steps_min_max": [int(stats_np[:, 0].min()), int(stats_np[:, 0].max())],
+        },
+    }
+    if args.profile:
+        print(json.dumps(result))
+        return 0
+
+    rng = np.random.default_rng(20261202 + 1)
+    # ---- plain GPU path: one launch per trajectory on a seeded sample
+    sample = np.sort(rng.choice(n_traj, size=min(args.plain_sample, n_traj), replace=False))
+    pb = np.stack([deltas[sample // 64], omegas[sample % 64]], axis=1).astype(np.complex128)
+    pb_dev = torch.as_tensor(pb, device=dev)
+    plan.solve(y0, *t_span, solver=args.solver, output_kind="populations", weights=weights,
+               with_integral=True, override_slots=slots, override_values=pb_dev[:2], mode="plain", **tol)
+    torch.cuda.synchronize()
+    t_p = time.perf_counter()
+    p_pops, p_stats, _, p_phot = plan.solve(y0, *t_span, solver=args.solver, output_kind="populations",
+                                            weights=weights, with_integral=True, override_slots=slots,
+                                            override_values=pb_dev, mode="plain", **tol)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t_p
+    plain_rate = len(sample) / plain_s
+    # the plain path must reproduce the batched results bit for bit
+    plain_equal = bool(torch.equal(p_pops, pops[torch.as_tensor(sample, device=dev)]) and
+                       torch.equal(p_phot, photons[torch.as_tensor(sample, device=dev)]))
+    log(f"bench: plain path {plain_rate:.1f} traj/s on {len(sample)} trajectories")
+    result["plain_gpu"] = {"value": plain_rate, "unit": "trajectories/s", "sample": len(sample),
+                           "bitwise_equal_to_batched": plain_equal}
+    result["speedup_vs_plain"] = result["value"] / plain_rate
+
+    # ---- CPU baseline: the oracle on all host cores, bounded seeded sample
+    from oracle.lb_oracle import Oracle, host_threads
+
+    oracle = Oracle(prepared)
+    cores = host_threads()
+    y0_np = prepared.layout.pack(rho0)
+    probe = np.sort(rng.choice(n_traj, size=cores, replace=False))
+    pbp = np.stack([deltas[probe // 64], omegas[probe % 64]], axis=1).astype(np.complex128)
+    t_c = time.perf_counter()
+    oracle.solve_batch(y0_np, t_span, solver=args.solver, output="populations", output_when="final",
+                       parameter_slots=["delta0", "Omega0"], parameter_batch=pbp, threads=cores, **tol)
+    probe_s = time.perf_counter() - t_c
+    n_cpu = int(max(cores, min(n_traj, cores * math.floor(args.cpu_seconds / max(probe_s, 1e-3)))))
+    cpu_idx = np.sort(rng.choice(n_traj, size=n_cpu, replace=False))
+    pbc = np.stack([deltas[cpu_idx // 64], omegas[cpu_idx % 64]], axis=1).astype(np.complex128)
+    log(f"bench: cpu baseline on {n_cpu} trajectories x {cores} threads")
+    t_c = time.perf_counter()
+    ref = oracle.solve_batch(y0_np, t_span, solver=args.solver, output="populations", output_when="final",
+                             parameter_slots=["delta0", "Omega0"], parameter_batch=pbc, threads=cores, **tol)
+    cpu_s = time.perf_counter() - t_c
+    cpu_rate = n_cpu / cpu_s
+    ref_int = oracle.solve_batch(y0_np, t_span, solver=args.solver, output="photon_integral",
+                                 output_when="final", integral_weights=weights,
+                                 parameter_slots=["delta0", "Omega0"], parameter_batch=pbc,
+                                 threads=cores, **tol)
+    result["cpu_baseline"] = {
+        "value": cpu_rate,
+        "unit": "trajectories/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{n_cpu} seeded trajectories of the same 64x64 grid (final populations), "
+                  f"oracle C restatement of the reference Rust path, {cores} threads, {cpu_s:.1f} s",
+    }
+    result["speedup_vs_cpu"] = result["value"] / cpu_rate
+    # ---- parity of the same sample: populations, photon counts, accepted steps
+    idx_dev = torch.as_tensor(cpu_idx, device=dev)
+    gpu_pops = pops[idx_dev].cpu().numpy()
+    gpu_phot = photons[idx_dev].cpu().numpy()
+    gpu_steps = stats_np[cpu_idx, 0]
+    result["parity"] = {
+        "trajectories": n_cpu,
+        "max_abs_population_error": float(np.max(np.abs(gpu_pops - ref["final"]))),
+        "max_abs_photon_error": float(np.max(np.abs(gpu_phot - ref_int["final"][:, 0]))),
+        "max_trace_error": float(np.max(np.abs(pops.sum(dim=1).cpu().numpy() - 1.0))),
+        "accepted_steps_equal": bool(np.array_equal(gpu_steps, ref["accepted"])),
+        "bar": "1e-7 absolute, trace 1e-10",
+    }
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

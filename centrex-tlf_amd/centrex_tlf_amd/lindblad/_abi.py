@@ -98,6 +98,8 @@ class LbSolveArgs(C.Structure):
         ("mode", C.c_int32),
         ("threads_per_traj", C.c_int32),
         ("workgroups", C.c_int32),
+        ("out_integral", C.c_void_p),
+        ("variant", C.c_int32),
     ]
 
 

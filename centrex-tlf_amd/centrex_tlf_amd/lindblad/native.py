@@ -22,7 +22,7 @@ _PKG_DIR = Path(__file__).resolve().parents[1]
 _INFO_KEYS = [
     "n_states", "dim", "upper_len", "n_terms", "n_coefficients", "time_dependent", "n_published",
     "n_gather_terms", "n_self_terms", "slot_count", "gather_rows", "flops_per_rhs",
-    "n_dynamic_coefficients", "lds_bytes", "slots_per_thread", "threads_per_traj",
+    "n_dynamic_coefficients", "lds_bytes", "kernel_variant", "threads_per_traj",
 ]
 
 
@@ -62,8 +62,25 @@ def library() -> C.CDLL:
                               C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]
     L.lb_solve.restype = C.c_int
     L.lb_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.lb_variant_info.restype = C.c_int
+    L.lb_variant_info.argtypes = [C.c_int32] + [C.POINTER(C.c_int32)] * 3
     _lib = L
     return L
+
+
+def kernel_variants() -> list[dict[str, int]]:
+    """The compiled solver variants: slots per thread, largest workgroup, occupancy hint."""
+    L = library()
+    out = []
+    i = 0
+    while True:
+        r, w, v = C.c_int32(), C.c_int32(), C.c_int32()
+        if L.lb_variant_info(i, C.byref(r), C.byref(w), C.byref(v)) != 0:
+            break
+        out.append({"variant": i + 1, "slots_per_thread": r.value, "max_threads": w.value,
+                    "waves_per_simd": v.value})
+        i += 1
+    return out
 
 
 def _check(rc: int) -> None:
@@ -87,6 +104,21 @@ def _stream_ptr() -> C.c_void_p:
 
 def _dptr(t) -> C.c_void_p | None:
     return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class GridAxes:
+    """Scan axes uploaded once: complex values on the device plus their lengths."""
+
+    def __init__(self, axes: Sequence[np.ndarray]):
+        import torch
+
+        arrays = [np.asarray(a, dtype=np.complex128).reshape(-1) for a in axes]
+        if any(a.size == 0 for a in arrays):
+            raise ValueError("scan axes must be non-empty")
+        self.lengths = np.asarray([a.size for a in arrays], dtype=np.int64)
+        self.count = int(np.prod(self.lengths))
+        self.values = torch.view_as_real(
+            torch.as_tensor(np.concatenate(arrays), device=torch_device())).contiguous()
 
 
 class DevicePlan:
@@ -144,12 +176,14 @@ class DevicePlan:
               override_slots: Sequence[int] = (), override_values=None,
               grid_axes: Sequence[np.ndarray] | None = None, n_traj: int | None = None,
               mode: str = "persistent", threads_per_traj: int = 0, workgroups: int = 0,
-              stream=None):
+              stream=None, with_integral: bool = False, variant: int = 0):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
         trajectory (``override_values`` of shape (B, n_override)) or a grid
         (``grid_axes``, one array per overridden slot, last axis fastest).
+        With ``with_integral`` the weighted trapezoid is returned as a fourth
+        tensor alongside a non-integral output, from the same pass.
         """
         import torch
 
@@ -159,11 +193,12 @@ class DevicePlan:
         slots = np.ascontiguousarray(override_slots, dtype=np.int32)
         axis_lengths = None
         axis_values = ov = None
-        if grid_axes is not None:
-            axes = [np.asarray(a, dtype=np.complex128).reshape(-1) for a in grid_axes]
-            axis_lengths = np.asarray([a.size for a in axes], dtype=np.int64)
-            B = int(np.prod(axis_lengths))
-            axis_values = torch.view_as_real(torch.as_tensor(np.concatenate(axes), device=dev)).contiguous()
+        if isinstance(grid_axes, GridAxes):
+            axis_lengths, axis_values = grid_axes.lengths, grid_axes.values
+            B = grid_axes.count
+        elif grid_axes is not None:
+            g = GridAxes(grid_axes)
+            axis_lengths, axis_values, B = g.lengths, g.values, g.count
         elif slots.size:
             ovt = torch.as_tensor(override_values, dtype=torch.complex128, device=dev)
             B = ovt.shape[0]
@@ -228,9 +263,16 @@ class DevicePlan:
         a.out_values = _dptr(values)
         a.out_stats = _dptr(stats)
         a.out_fail_t = _dptr(fail_t)
+        integral = None
+        if with_integral:
+            integral = torch.full((B,), float("nan"), dtype=torch.float64, device=dev)
+            a.out_integral = _dptr(integral)
         a.mode = LB_MODE_PLAIN if mode == "plain" else LB_MODE_PERSISTENT
         a.threads_per_traj = int(threads_per_traj)
         a.workgroups = int(workgroups)
+        a.variant = int(variant)
         sp = _stream_ptr() if stream is None else C.c_void_p(stream.cuda_stream)
         _check(library().lb_solve(self._handle, C.byref(a), sp))
+        if with_integral:
+            return values, stats, fail_t, integral
         return values, stats, fail_t

@@ -101,6 +101,8 @@ struct DevCall {
   double* out_values;
   int64_t* out_stats;
   double* out_fail_t;
+  double* out_integral;    // optional trapezoid next to a non-integral output
+  int32_t want_integral;   // accumulate the trapezoid
   int32_t* queue;          // work-queue counter
   int64_t traj_offset;     // plain mode: first trajectory of this launch
 };

@@ -17,9 +17,10 @@
 
 #include "lb_device.h"
 
-extern "C" void* lb_dev_solve_kernel(int R, int solver);
-extern "C" void* lb_dev_rhs_kernel(int R);
-extern "C" int lb_dev_max_threads(int R);
+extern "C" int lb_dev_variant_count(void);
+extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
+extern "C" void* lb_dev_solve_kernel(int id, int solver);
+extern "C" void* lb_dev_rhs_kernel(int id);
 extern "C" void* lb_dev_queue_reset_kernel(void);
 
 namespace {
@@ -38,8 +39,6 @@ int fail(int code, const std::string& msg) {
       return fail(LB_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
   } while (0)
 
-const int kConfigR[] = {1, 2, 3, 4, 6, 8};
-
 }  // namespace
 
 struct lb_plan {
@@ -47,7 +46,7 @@ struct lb_plan {
   std::vector<void*> allocations;
   int device = 0;
   int32_t* queue = nullptr;
-  int default_R = 0, default_W = 0;
+  int default_variant = -1, default_W = 0;
   int64_t info[16] = {0};
   int64_t lds_bytes = 0;
   std::vector<int32_t> slot_up_host;  // slot -> upper index
@@ -109,21 +108,31 @@ int check_expression(const lb_plan_desc* d, int start, int end, int n_slots, std
   return 0;
 }
 
-// Slots per thread and workgroup size for a plan with S slots.
-void pick_config(int S, int* R_out, int* W_out) {
-  // Prefer four slots per thread (good register/occupancy balance on CDNA),
-  // then the smallest padding.
-  int bestR = 0, bestW = 0, bestScore = 1 << 30;
-  for (int R : kConfigR) {
-    int maxw = lb_dev_max_threads(R);
-    int W = ((S + R - 1) / R + 63) / 64 * 64;
-    if (W > maxw) continue;
-    if (W < 64) W = 64;
-    int waste = R * W - S;
-    int score = waste + (R == 4 ? 0 : 64) + (R == 8 ? 256 : 0);
-    if (score < bestScore) { bestScore = score; bestR = R; bestW = W; }
+// Smallest workgroup (multiple of 64) with which variant `id` covers S slots, or 0.
+int variant_threads(int id, int S) {
+  int R, maxw, waves;
+  if (!lb_dev_variant(id, &R, &maxw, &waves)) return 0;
+  int W = ((S + R - 1) / R + 63) / 64 * 64;
+  if (W < 64) W = 64;
+  return W <= maxw ? W : 0;
+}
+
+// Default variant: the most registers per lane that still keeps two
+// workgroups' worth of waves per SIMD pair, then the least padding.  The
+// autotuner (centrex_tlf_amd.lindblad.autotune) replaces this by measurement.
+void pick_config(int S, int* id_out, int* W_out) {
+  int best = -1, bestW = 0;
+  int64_t bestScore = INT64_MAX;
+  for (int id = 0; id < lb_dev_variant_count(); ++id) {
+    int R, maxw, waves;
+    lb_dev_variant(id, &R, &maxw, &waves);
+    int W = variant_threads(id, S);
+    if (!W) continue;
+    const int64_t waste = (int64_t)R * W - S;
+    const int64_t score = waste + (waves == 2 ? 0 : 256) + 8 * R;
+    if (score < bestScore) { bestScore = score; best = id; bestW = W; }
   }
-  *R_out = bestR;
+  *id_out = best;
   *W_out = bestW;
 }
 
@@ -341,7 +350,7 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
     lb_plan_destroy(p);
     return rc;
   }
-  pick_config(S, &p->default_R, &p->default_W);
+  pick_config(S, &p->default_variant, &p->default_W);
   p->lds_bytes = lds_bytes_for(P);
   p->slot_up_host = slot_up;
   int64_t self_terms = 0, gather_terms = 0;
@@ -350,7 +359,7 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   info[0] = n; info[1] = n * n; info[2] = U; info[3] = T; info[4] = d->n_coefficients;
   info[5] = time_dependent; info[6] = n_pub; info[7] = gather_terms; info[8] = self_terms;
   info[9] = S; info[10] = rows_total; info[11] = flops; info[12] = n_dyn_coef; info[13] = p->lds_bytes;
-  info[14] = p->default_R; info[15] = p->default_W;
+  info[14] = p->default_variant + 1; info[15] = p->default_W;
   *out_plan = p;
   return LB_OK;
 }
@@ -369,21 +378,36 @@ extern "C" int lb_plan_info(const lb_plan* p, int64_t* info) {
 
 namespace {
 
-int resolve_config(const lb_plan* p, int threads, int* R, int* W) {
-  if (threads <= 0) {
-    *R = p->default_R;
+// variant: 0 = plan default, k > 0 = kernel variant k - 1.  threads: 0 = the
+// variant's smallest covering workgroup.
+int resolve_config(const lb_plan* p, int variant, int threads, int* id, int* W) {
+  if (threads < 0 || threads % 64 != 0)
+    return fail(LB_ERR_INVALID, "threads_per_traj must be a multiple of 64");
+  if (variant <= 0 && threads == 0) {
+    *id = p->default_variant;
     *W = p->default_W;
-  } else {
-    if (threads % 64 != 0) return fail(LB_ERR_INVALID, "threads_per_traj must be a multiple of 64");
-    int best = 0;
-    for (int r : kConfigR)
-      if ((int64_t)r * threads >= p->dev.S && threads <= lb_dev_max_threads(r)) { best = r; break; }
-    if (!best) return fail(LB_ERR_UNSUPPORTED, "no kernel variant covers the plan with this workgroup size");
-    *R = best;
-    *W = threads;
+    if (*id < 0) return fail(LB_ERR_UNSUPPORTED, "plan is too large for the available kernel variants");
+    return LB_OK;
   }
-  if (*R == 0) return fail(LB_ERR_UNSUPPORTED, "plan is too large for the available kernel variants");
-  return LB_OK;
+  int R, maxw, waves;
+  if (variant > 0) {
+    *id = variant - 1;
+    if (!lb_dev_variant(*id, &R, &maxw, &waves)) return fail(LB_ERR_INVALID, "unknown kernel variant");
+    *W = threads ? threads : variant_threads(*id, p->dev.S);
+    if (!*W || *W > maxw || (int64_t)R * *W < p->dev.S)
+      return fail(LB_ERR_UNSUPPORTED, "kernel variant does not cover the plan with this workgroup size");
+    return LB_OK;
+  }
+  // threads given: the variant with the fewest slots per thread that covers the plan
+  for (int v = 0; v < lb_dev_variant_count(); ++v) {
+    lb_dev_variant(v, &R, &maxw, &waves);
+    if (threads <= maxw && (int64_t)R * threads >= p->dev.S) {
+      *id = v;
+      *W = threads;
+      return LB_OK;
+    }
+  }
+  return fail(LB_ERR_UNSUPPORTED, "no kernel variant covers the plan with this workgroup size");
 }
 
 int fill_overrides(const lb_plan* p, int32_t n, const int32_t* slots, int32_t* dst) {
@@ -413,9 +437,9 @@ extern "C" int lb_rhs_eval(const lb_plan* p, int64_t batch, const double* y, con
   c.n_override = n_override;
   c.override_values = override_values;
   c.out = out;
-  int R, W;
-  if ((rc = resolve_config(p, 0, &R, &W))) return rc;
-  void* fn = lb_dev_rhs_kernel(R);
+  int id, W;
+  if ((rc = resolve_config(p, 0, 0, &id, &W))) return rc;
+  void* fn = lb_dev_rhs_kernel(id);
   DevPlan P = p->dev;
   void* args[] = {&P, &c};
   if ((rc = set_lds_limit(fn, (size_t)p->lds_bytes))) return rc;
@@ -495,10 +519,14 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   c.out_values = a->out_values;
   c.out_stats = a->out_stats;
   c.out_fail_t = a->out_fail_t;
+  c.out_integral = a->out_integral;
+  c.want_integral = (a->output_kind == LB_OUT_INTEGRAL || a->out_integral != nullptr) ? 1 : 0;
+  if (c.want_integral && a->n_weights <= 0)
+    return fail(LB_ERR_INVALID, "integral outputs require integral weights");
   c.queue = p->queue;
-  int R, W;
-  if ((rc = resolve_config(p, a->threads_per_traj, &R, &W))) return rc;
-  void* fn = lb_dev_solve_kernel(R, a->solver);
+  int id, W;
+  if ((rc = resolve_config(p, a->variant, a->threads_per_traj, &id, &W))) return rc;
+  void* fn = lb_dev_solve_kernel(id, a->solver);
   void* reset = lb_dev_queue_reset_kernel();
   hipStream_t s = (hipStream_t)stream;
   DevPlan P = p->dev;
@@ -534,5 +562,15 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   void* args[] = {&P, &c};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
   HIP_TRY(hipGetLastError());
+  return LB_OK;
+}
+
+extern "C" int lb_variant_info(int32_t id, int32_t* slots_per_thread, int32_t* max_threads,
+                               int32_t* waves_per_simd) {
+  int R, maxw, waves;
+  if (!lb_dev_variant(id, &R, &maxw, &waves)) return fail(LB_ERR_INVALID, "no such kernel variant");
+  if (slots_per_thread) *slots_per_thread = R;
+  if (max_threads) *max_threads = maxw;
+  if (waves_per_simd) *waves_per_simd = waves;
   return LB_OK;
 }

@@ -38,15 +38,19 @@ __device__ __forceinline__ dcpx from_polar(double r, double th) {
   sincos(th, &s, &c);
   return mk(r * c, r * s);
 }
+// The transcendental leaves of the expression machine are kept out of line:
+// inlined, their temporaries would set the register allocation of the whole
+// integrator kernel.  They run a few times per step in at most six lanes.
+//
 // exp(z) = e^re (cos im, sin im); exact for a real argument.
-__device__ dcpx dev_cexp(dcpx z) {
+__device__ __noinline__ dcpx dev_cexp(dcpx z) {
   if (z.im == 0.0) return mk(exp(z.re), 0.0);
   return from_polar(exp(z.re), z.im);
 }
 // x^y = exp(y ln x) in polar form (x^0 = 1, 0^y = 0).  A real exponent skips the
 // exp/log factors, which are exactly 1 and 0 in that case; a positive real base
 // skips the angle, which is exactly 0.
-__device__ dcpx dev_cpow(dcpx a, dcpx b) {
+__device__ __noinline__ dcpx dev_cpow(dcpx a, dcpx b) {
   if (b.re == 0.0 && b.im == 0.0) return mk(1.0, 0.0);
   double r = (a.im == 0.0) ? fabs(a.re) : hypot(a.re, a.im);
   if (r == 0.0) return mk(0.0, 0.0);
@@ -76,7 +80,7 @@ __device__ double rabi_from_intensity(double intensity, double coupling, double 
 }
 
 // Scalar helpers of rust/src/lindblad/eval.rs:155-451, 664-789 (ids 1-7, 9, 11, 12, 13, 15).
-__device__ dcpx dev_helper(int id, const dcpx* a, int argc) {
+__device__ __noinline__ dcpx dev_helper(int id, const dcpx* a, int argc) {
   const double ED_XTB = 0.315 * 1.602176634e-19 * 5.29177210544e-11;
 #define A(i) (a[i].re)
   switch (id) {
@@ -130,7 +134,23 @@ __device__ dcpx dev_helper(int id, const dcpx* a, int argc) {
 
 // Stack machine of rust/src/lindblad/eval.rs:791-911 (scalar form).  The plan
 // builder has checked ops, helper ids and the stack depth.
-__device__ dcpx vm_eval(const DevInstr* ins, int start, int end, const dcpx* slots, double t) {
+// Builtins 1-7 of rust/src/lindblad/eval.rs:135-153 (num-complex formulas).
+__device__ __noinline__ dcpx dev_builtin(int f, dcpx v) {
+  switch (f) {
+    case 1: return mk(sin(v.re) * cosh(v.im), cos(v.re) * sinh(v.im));
+    case 2: return mk(cos(v.re) * cosh(v.im), -(sin(v.re) * sinh(v.im)));
+    case 3: {
+      double tr = v.re + v.re, ti = v.im + v.im, d = cos(tr) + cosh(ti);
+      return mk(sin(tr) / d, sinh(ti) / d);
+    }
+    case 4: return dev_cexp(v);
+    case 5: return mk(hypot(v.re, v.im), 0.0);
+    case 6: return mk(v.re, 0.0);
+    default: return mk(v.im, 0.0);
+  }
+}
+
+__device__ __forceinline__ dcpx vm_eval(const DevInstr* ins, int start, int end, const dcpx* slots, double t) {
   dcpx st[LB_MAX_STACK];
   int sp = 0;
   for (int p = start; p < end; ++p) {
@@ -148,20 +168,7 @@ __device__ dcpx vm_eval(const DevInstr* ins, int start, int end, const dcpx* slo
       case 11: st[sp - 1].im = -st[sp - 1].im; break;
       case 12: {
         int s0 = sp - q.argc;
-        dcpx v = st[s0], r;
-        switch (q.function) {
-          case 1: r = mk(sin(v.re) * cosh(v.im), cos(v.re) * sinh(v.im)); break;
-          case 2: r = mk(cos(v.re) * cosh(v.im), -(sin(v.re) * sinh(v.im))); break;
-          case 3: {
-            double tr = v.re + v.re, ti = v.im + v.im, d = cos(tr) + cosh(ti);
-            r = mk(sin(tr) / d, sinh(ti) / d);
-            break;
-          }
-          case 4: r = dev_cexp(v); break;
-          case 5: r = mk(hypot(v.re, v.im), 0.0); break;
-          case 6: r = mk(v.re, 0.0); break;
-          default: r = mk(v.im, 0.0); break;
-        }
+        dcpx r = dev_builtin(q.function, st[s0]);
         sp = s0;
         st[sp++] = r;
         break;
@@ -234,6 +241,38 @@ struct Tab<1> {
          : 1.0 / 66.0;
   }
 };
+
+// Stage input y + h * sum_j a_sj k_j (dopri5.rs:45-86) for stage S; S = 7 is
+// the solution.  Zero tableau entries are skipped at compile time.
+template <int R, int SOLVER, int S>
+__device__ __forceinline__ void stage_combo(double h, const double (&y_r)[R], const double (&y_i)[R],
+                                            const double (&k_r)[8][R], const double (&k_i)[8][R],
+                                            double (&o_r)[R], double (&o_i)[R]) {
+  using T = Tab<SOLVER>;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double sr = T::a(S, 1) * k_r[1][r], si = T::a(S, 1) * k_i[1][r];
+#pragma unroll
+    for (int j = 2; j < S; ++j) {
+      if (T::a(S, j) != 0.0) {
+        sr = fma(T::a(S, j), k_r[j][r], sr);
+        si = fma(T::a(S, j), k_i[j][r], si);
+      }
+    }
+    o_r[r] = fma(h, sr, y_r[r]);
+    o_i[r] = fma(h, si, y_i[r]);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void copy_k(const double (&a)[R], const double (&b)[R], double (&ka)[R],
+                                       double (&kb)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    ka[r] = a[r];
+    kb[r] = b[r];
+  }
+}
 
 // Per-thread view of the slots it owns: LDS publication index and flags
 // (bit 0 valid, bit 1 diagonal, bits 2.. diagonal state index).
@@ -351,7 +390,7 @@ __device__ __forceinline__ double time_tol(double t) { return 10.0 * DBL_EPSILON
 
 // Evaluate the parameter graph + coefficients for time t into slot copy `sl`
 // and coefficient row `cf`; only_dynamic skips the time-independent parts.
-__device__ __noinline__ void eval_coefficients(const DevPlan& P, dcpx* sl, dcpx* cf, double t,
+__device__ __forceinline__ void eval_coefficients(const DevPlan& P, dcpx* sl, dcpx* cf, double t,
                                                bool only_dynamic) {
   for (int c = 0; c < P.n_compounds; ++c) {
     if (only_dynamic && !P.compound_dyn[c]) continue;
@@ -378,28 +417,29 @@ __device__ __forceinline__ void load_slots(const DevPlan& P, Slots<R>& S, int ti
   }
 }
 
-// Per-trajectory parameter slots: base values, then overrides (row of a batch
-// or a grid point, last axis fastest: rust/src/lindblad/ode_batch.rs:353-391).
-__device__ void init_slots(const DevPlan& P, const int32_t* ov_slot, int n_ov, const double* ov_row,
-                           const DevCall* C, int64_t traj, dcpx* sl, int tid, int W) {
+// Per-trajectory parameter slots, part 1: base values (overrides follow in the
+// kernels, which read them straight from their argument structs).
+__device__ __forceinline__ void init_slots(const DevPlan& P, dcpx* sl, int tid, int W) {
   for (int i = tid; i < P.n_slots; i += W)
     sl[i] = i < P.n_base ? mk(P.base[2 * i], P.base[2 * i + 1]) : mk(0.0, 0.0);
   __syncthreads();
-  if (tid == 0) {
-    for (int j = 0; j < n_ov; ++j) {
-      double vr, vi;
-      if (C != nullptr && C->n_axes > 0) {
-        const int64_t ai = (traj / C->axis_stride[j]) % C->axis_len[j];
-        vr = C->axis_values[2 * (C->axis_offset[j] + ai)];
-        vi = C->axis_values[2 * (C->axis_offset[j] + ai) + 1];
-      } else {
-        vr = ov_row[2 * j];
-        vi = ov_row[2 * j + 1];
-      }
-      sl[ov_slot[j]] = mk(vr, vi);
+}
+
+// Part 2: base-slot overrides of trajectory `traj`, from a row of a batch or a
+// grid point with the last axis fastest (rust/src/lindblad/ode_batch.rs:353-391).
+__device__ __forceinline__ void apply_overrides(const DevCall& C, int64_t traj, dcpx* sl) {
+  for (int j = 0; j < C.n_override; ++j) {
+    double vr, vi;
+    if (C.n_axes > 0) {
+      const int64_t ai = C.axis_offset[j] + (traj / C.axis_stride[j]) % C.axis_len[j];
+      vr = C.axis_values[2 * ai];
+      vi = C.axis_values[2 * ai + 1];
+    } else {
+      vr = C.override_values[2 * (traj * C.n_override + j)];
+      vi = C.override_values[2 * (traj * C.n_override + j) + 1];
     }
+    sl[C.override_slot[j]] = mk(vr, vi);
   }
-  __syncthreads();
 }
 
 // Fold the time-independent self terms of every slot with this trajectory's
@@ -447,10 +487,11 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     const double t1 = C.t1_per_traj ? C.t1_per_traj[traj] : C.t1;
 
     // ---- parameters and coefficients at t0, self-term folds, weights
-    init_slots(P, C.override_slot, C.n_override,
-               C.override_values ? C.override_values + 2 * traj * C.n_override : nullptr, &C, traj,
-               L.slots, tid, W);
-    if (tid == 0) eval_coefficients(P, L.slots, L.coef, t0, false);
+    init_slots(P, L.slots, tid, W);
+    if (tid == 0) {
+      apply_overrides(C, traj, L.slots);
+      eval_coefficients(P, L.slots, L.coef, t0, false);
+    }
     for (int i = tid; i < P.n; i += W) {
       double w = 0.0;
       for (int q = 0; q < C.n_weights; ++q)
@@ -526,7 +567,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
       }
       rhs_calls = 1;
       // ---- first trapezoid point (output.rs:315-330)
-      if (C.output_kind == LB_OUT_INTEGRAL && C.save_start) {
+      if (C.want_integral && C.save_start) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
           block_store(P, tid, W, r, L.red, 1, S.diag(r) && S.valid(r) ? L.wts[S.state(r)] * y_r[r] : 0.0);
@@ -553,41 +594,27 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         }
         const bool need_stiff = ((accepted + 1) % 1000 == 0) || ia > 0;
         double yt_r[R], yt_i[R], yn_r[R], yn_i[R];
-#pragma unroll
-        for (int s = 2; s <= 7; ++s) {
+#define LB_STAGE(s)                                                                          \
+  stage_combo<R, SOLVER, s>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                              \
+  rhs_eval<R>(P, S, yt_r, yt_i, k_r[s], k_i[s], PUB, L.selfc, L.coef + (s) * KK, tid, W); \
+  buf ^= 1;
+        LB_STAGE(2)
+        LB_STAGE(3)
+        LB_STAGE(4)
+        LB_STAGE(5)
+        LB_STAGE(6)
+#undef LB_STAGE
+        stage_combo<R, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yn_r, yn_i);
+        if (need_stiff) {
+          // |yn - y_stage6|^2 per block (dopri5.rs:255-264)
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            double sr = T::a(s, 1) * k_r[1][r], si = T::a(s, 1) * k_i[1][r];
-#pragma unroll
-            for (int j = 2; j < s; ++j) {
-              if (T::a(s, j) != 0.0) {
-                sr = fma(T::a(s, j), k_r[j][r], sr);
-                si = fma(T::a(s, j), k_i[j][r], si);
-              }
-            }
-            if (s < 7) {
-              yt_r[r] = fma(h, sr, y_r[r]);
-              yt_i[r] = fma(h, si, y_i[r]);
-            } else {
-              yn_r[r] = fma(h, sr, y_r[r]);
-              yn_i[r] = fma(h, si, y_i[r]);
-            }
+            const double a0 = yn_r[r] - yt_r[r], a1 = yn_i[r] - yt_i[r];
+            block_store(P, tid, W, r, L.red, 3, a0 * a0 + a1 * a1);
           }
-          if (s == 7) {
-            if (need_stiff) {
-              // |yn - y_stage6|^2 per block (dopri5.rs:255-264)
-#pragma unroll
-              for (int r = 0; r < R; ++r) {
-                const double a0 = yn_r[r] - yt_r[r], a1 = yn_i[r] - yt_i[r];
-                block_store(P, tid, W, r, L.red, 3, a0 * a0 + a1 * a1);
-              }
-            }
-            rhs_eval<R>(P, S, yn_r, yn_i, k_r[7], k_i[7], PUB, L.selfc, L.coef + 7 * KK, tid, W);
-          } else {
-            rhs_eval<R>(P, S, yt_r, yt_i, k_r[s], k_i[s], PUB, L.selfc, L.coef + s * KK, tid, W);
-          }
-          buf ^= 1;
         }
+        rhs_eval<R>(P, S, yn_r, yn_i, k_r[7], k_i[7], PUB, L.selfc, L.coef + 7 * KK, tid, W);
+        buf ^= 1;
         rhs_calls += 6;
         // ---- error norm (dopri5.rs:88-102, tsit5.rs:104-119), integrand, stiffness
 #pragma unroll
@@ -606,7 +633,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
           const double si = C.abstol + fmax(fabs(y_i[r]), fabs(yn_i[r])) * C.reltol;
           const double qr = er / sr, qi = ei / si;
           block_store(P, tid, W, r, L.red, 0, S.valid(r) ? qr * qr + (S.diag(r) ? 0.0 : qi * qi) : 0.0);
-          if (C.output_kind == LB_OUT_INTEGRAL)
+          if (C.want_integral)
             block_store(P, tid, W, r, L.red, 1, S.diag(r) && S.valid(r) ? L.wts[S.state(r)] * yn_r[r] : 0.0);
           if (need_stiff) {
             const double a0 = k_r[7][r] - k_r[6][r], a1 = k_i[7][r] - k_i[6][r];
@@ -639,7 +666,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
             }
           }
           x += h;
-          if (C.output_kind == LB_OUT_INTEGRAL) {
+          if (C.want_integral) {
             const double v = block_total(P, L.red, 1);
             if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (x - last_t);
             last_t = x;
@@ -688,6 +715,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     }
     if (tid == 0) {
       if (C.output_kind == LB_OUT_INTEGRAL) C.out_values[traj] = ok ? integral : nanv;
+      if (C.out_integral) C.out_integral[traj] = ok ? integral : nanv;
       C.out_stats[4 * traj + 0] = accepted;
       C.out_stats[4 * traj + 1] = rejected;
       C.out_stats[4 * traj + 2] = rhs_calls;
@@ -708,10 +736,13 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_rhs_kernel(const DevPlan P, co
   load_slots<R>(P, S, tid, W);
   for (int64_t b = blockIdx.x; b < C.batch; b += gridDim.x) {
     __syncthreads();
-    init_slots(P, C.override_slot, C.n_override,
-               C.override_values ? C.override_values + 2 * b * C.n_override : nullptr, nullptr, b,
-               L.slots, tid, W);
-    if (tid == 0) eval_coefficients(P, L.slots, L.coef, C.t[b], false);
+    init_slots(P, L.slots, tid, W);
+    if (tid == 0) {
+      for (int j = 0; j < C.n_override; ++j)
+        L.slots[C.override_slot[j]] = mk(C.override_values[2 * (b * C.n_override + j)],
+                                         C.override_values[2 * (b * C.n_override + j) + 1]);
+      eval_coefficients(P, L.slots, L.coef, C.t[b], false);
+    }
     __syncthreads();
     fold_self(P, tid, W, L.coef, L.selfc);
     double xr[R], xi[R], kr[R], ki[R];
@@ -736,38 +767,50 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_rhs_kernel(const DevPlan P, co
 __global__ void lb_queue_reset_kernel(int32_t* q) { *q = 0; }
 
 // ---------------------------------------------------------------------------
-// Kernel table: (slots per thread R, largest workgroup, waves per SIMD hint).
-// The host picks a row with R * W >= plan slots; see lb_host.cpp.
+// Kernel variants: (id, slots per thread R, largest workgroup, waves-per-SIMD
+// hint).  The hint caps registers per lane (512 / hint); the host picks a
+// variant with R * W >= plan slots (lb_host.cpp) and the autotuner measures.
 // ---------------------------------------------------------------------------
-#define LB_CONFIGS(X) X(1, 256, 4) X(2, 512, 4) X(3, 1024, 4) X(4, 256, 2) X(6, 512, 2) X(8, 128, 2)
+#define LB_CONFIGS(X)                                                                            \
+  X(0, 1, 1024, 4) X(1, 2, 512, 4) X(2, 2, 512, 2) X(3, 3, 1024, 4) X(4, 4, 256, 4)                \
+  X(5, 4, 256, 2) X(6, 4, 256, 1) X(7, 6, 512, 2) X(8, 6, 512, 1) X(9, 8, 128, 1) X(10, 12, 256, 1)
 
-extern "C" void* lb_dev_solve_kernel(int R, int solver);
-extern "C" void* lb_dev_rhs_kernel(int R);
-extern "C" int lb_dev_max_threads(int R);
+extern "C" int lb_dev_variant_count(void);
+extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
+extern "C" void* lb_dev_solve_kernel(int id, int solver);
+extern "C" void* lb_dev_rhs_kernel(int id);
 extern "C" void* lb_dev_queue_reset_kernel(void);
 
-extern "C" void* lb_dev_solve_kernel(int R, int solver) {
-#define X(r, w, v)                                                                                 \
-  if (R == r) return solver == 1 ? (void*)&lb_solve_kernel<r, 1, w, v> : (void*)&lb_solve_kernel<r, 0, w, v>;
+extern "C" int lb_dev_variant_count(void) {
+  int n = 0;
+#define X(i, r, w, v) ++n;
   LB_CONFIGS(X)
 #undef X
-  return nullptr;
+  return n;
 }
 
-extern "C" void* lb_dev_rhs_kernel(int R) {
-#define X(r, w, v) \
-  if (R == r) return (void*)&lb_rhs_kernel<r, w, v>;
-  LB_CONFIGS(X)
-#undef X
-  return nullptr;
-}
-
-extern "C" int lb_dev_max_threads(int R) {
-#define X(r, w, v) \
-  if (R == r) return w;
+extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves) {
+#define X(i, r, w, v)                                                                           \
+  if (id == i) { *R = r; *maxw = w; *waves = v; return 1; }
   LB_CONFIGS(X)
 #undef X
   return 0;
+}
+
+extern "C" void* lb_dev_solve_kernel(int id, int solver) {
+#define X(i, r, w, v)                                                                               \
+  if (id == i) return solver == 1 ? (void*)&lb_solve_kernel<r, 1, w, v> : (void*)&lb_solve_kernel<r, 0, w, v>;
+  LB_CONFIGS(X)
+#undef X
+  return nullptr;
+}
+
+extern "C" void* lb_dev_rhs_kernel(int id) {
+#define X(i, r, w, v) \
+  if (id == i) return (void*)&lb_rhs_kernel<r, w, v>;
+  LB_CONFIGS(X)
+#undef X
+  return nullptr;
 }
 
 extern "C" void* lb_dev_queue_reset_kernel(void) { return (void*)&lb_queue_reset_kernel; }

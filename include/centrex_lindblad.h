@@ -173,6 +173,12 @@ typedef struct lb_solve_args {
   int32_t mode;                   /* LB_MODE_* */
   int32_t threads_per_traj;       /* 0 = plan default; 64..1024, multiple of 64 */
   int32_t workgroups;             /* persistent grid size, 0 = fill the device */
+  double* out_integral;           /* device, n_traj, or NULL: the weighted trapezoid
+                                     alongside a non-integral output (one pass
+                                     instead of the reference's two solves) */
+  int32_t variant;                /* kernel variant: 0 = plan default, k = variant k-1
+                                     (slots per thread / occupancy; results are
+                                     bitwise independent of it) */
 } lb_solve_args;
 
 /*
@@ -183,6 +189,14 @@ typedef struct lb_solve_args {
  * Stream-ordered on `stream` (a hipStream_t, NULL = default stream).
  */
 int lb_solve(const lb_plan* plan, const lb_solve_args* args, void* stream);
+
+/*
+ * Kernel variant `id` (0-based): slots per thread, largest workgroup and the
+ * waves-per-SIMD occupancy hint it was compiled for.  Returns LB_ERR_INVALID
+ * past the last variant.  No reference counterpart: it feeds the autotuner.
+ */
+int lb_variant_info(int32_t id, int32_t* slots_per_thread, int32_t* max_threads,
+                    int32_t* waves_per_simd);
 
 #ifdef __cplusplus
 }
