@@ -176,7 +176,8 @@ class DevicePlan:
               override_slots: Sequence[int] = (), override_values=None,
               grid_axes: Sequence[np.ndarray] | None = None, n_traj: int | None = None,
               mode: str = "persistent", threads_per_traj: int = 0, workgroups: int = 0,
-              stream=None, with_integral: bool = False, variant: int = 0):
+              stream=None, with_integral: bool = False, variant: int = 0,
+              restrict_support: bool = True, support=None):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
@@ -184,12 +185,22 @@ class DevicePlan:
         (``grid_axes``, one array per overridden slot, last axis fastest).
         With ``with_integral`` the weighted trapezoid is returned as a fourth
         tensor alongside a non-integral output, from the same pass.
+        With ``restrict_support`` the solve runs only the density-matrix
+        elements reachable from the nonzero entries of ``y0`` (the rest stay
+        exactly zero); otherwise every element is integrated.
         """
         import torch
 
         dev = torch_device()
         y0 = torch.as_tensor(y0, dtype=torch.float64, device=dev).contiguous()
         shared = y0.ndim == 1
+        if support is not None:
+            support = np.ascontiguousarray(np.asarray(support, dtype=bool).astype(np.uint8).reshape(-1))
+            if support.size != self.dim:
+                raise ValueError("support must have one entry per packed index")
+        elif restrict_support:
+            nz = (y0 != 0) if shared else (y0 != 0).any(dim=0)
+            support = np.ascontiguousarray(nz.to("cpu").numpy().astype(np.uint8))
         slots = np.ascontiguousarray(override_slots, dtype=np.int32)
         axis_lengths = None
         axis_values = ov = None
@@ -271,6 +282,9 @@ class DevicePlan:
         a.threads_per_traj = int(threads_per_traj)
         a.workgroups = int(workgroups)
         a.variant = int(variant)
+        if support is not None:
+            keep.append(support)
+            a.y0_support = ptr(support, C.c_uint8)
         sp = _stream_ptr() if stream is None else C.c_void_p(stream.cuda_stream)
         _check(library().lb_solve(self._handle, C.byref(a), sp))
         if with_integral:

@@ -38,9 +38,13 @@ __host__ __device__ inline uint32_t lb_gword(uint32_t pub, bool real_in, bool co
          (conj_c ? (1u << 22) : 0u) | (coef << 24);
 }
 
+// One bytecode instruction (rust/src/lindblad/eval.rs:11-60 opcodes).  Op 20
+// (POWC, device only) raises the top of the stack to the real constant `re`,
+// with im = cos(re*pi) and aux = sin(re*pi) for negative real bases.
+#define LB_OP_POWC 20
 struct DevInstr {
   int32_t op, index, argc, function;
-  double re, im;
+  double re, im, aux;
 };
 
 struct DevPlan {
@@ -65,12 +69,19 @@ struct DevPlan {
   const int32_t* blk_rows;   // nblk
   const uint32_t* g_word;    // rows*64
   const double* g_f;         // rows*64*2
+  int32_t g_rows;            // total gather rows
+  int32_t lds_tables;        // 1: kernels copy the gather tables into LDS
+  // restricted plans hold only the elements reachable from an initial support;
+  // the others are written as zeros
+  int32_t restricted;
+  const uint8_t* upper_active;   // upper_len
+  const int32_t* packed_upper;   // dim: packed index -> upper index
   // expression machine
   const DevInstr* instr;
   const int32_t* expr_start;    // n_compounds + K + 1
   const int32_t* compound_slot; // n_compounds
-  const uint8_t* compound_dyn;  // n_compounds
-  const uint8_t* coef_dyn;      // K
+  const uint8_t* expr_dyn;      // n_compounds + K: 1 if time-dependent
+  int32_t vm_depth;             // deepest expression stack
   const double* base;           // n_base*2
 };
 

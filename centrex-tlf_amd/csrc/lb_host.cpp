@@ -11,11 +11,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
 
 #include "lb_device.h"
+#include "lb_program.h"
 
 extern "C" int lb_dev_variant_count(void);
 extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
@@ -41,25 +45,58 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace
 
-struct lb_plan {
+// Validated copy of a plan description plus the lowered expression program.
+struct PlanSource {
+  int n = 0, U = 0, K = 0, n_base = 0;
+  int64_t T = 0;
+  std::vector<int64_t> output_ptrs, input_indices, coeff_indices;
+  std::vector<uint8_t> coeff_conj, input_conj, coef_dyn;
+  std::vector<double> factors_re, factors_im, base;
+  std::vector<int32_t> up_re, up_im;  // upper index -> packed indices (-1 imag on the diagonal)
+  LbProgram prog;
+  std::vector<uint8_t> expr_dyn;
+  std::vector<int32_t> compound_slot;
+  bool time_dependent = false;
+  int n_dyn_coef = 0;
+};
+
+// Device tables for one set of active upper elements (all of them, or the
+// closure of an initial-state support, see active_closure()).
+struct DevImage {
   DevPlan dev{};
   std::vector<void*> allocations;
-  int device = 0;
-  int32_t* queue = nullptr;
   int default_variant = -1, default_W = 0;
-  int64_t info[16] = {0};
   int64_t lds_bytes = 0;
-  std::vector<int32_t> slot_up_host;  // slot -> upper index
+  int64_t n_pub = 0, gather_terms = 0, self_terms = 0, rows_total = 0, flops = 0, active = 0;
+  std::map<std::pair<int, int>, int> per_cu;  // (variant, W) -> resident workgroups per CU
+};
+
+struct lb_plan {
+  PlanSource src;
+  int device = 0, cus = 0;
+  int32_t* queue = nullptr;
+  std::unique_ptr<DevImage> full;
+  std::map<std::vector<uint8_t>, std::unique_ptr<DevImage>> restricted;
+  std::mutex mu;  // guards `restricted` and the occupancy caches
+  int64_t info[16] = {0};
+  ~lb_plan() {
+    auto release = [](DevImage* im) {
+      if (im) for (void* a : im->allocations) (void)hipFree(a);
+    };
+    release(full.get());
+    for (auto& kv : restricted) release(kv.second.get());
+    if (queue) (void)hipFree(queue);
+  }
 };
 
 namespace {
 
 template <typename T>
-int upload(lb_plan* p, const std::vector<T>& host, const T** out) {
+int upload(DevImage* im, const std::vector<T>& host, const T** out) {
   size_t bytes = std::max<size_t>(host.size(), 1) * sizeof(T);
   void* d = nullptr;
   HIP_TRY(hipMalloc(&d, bytes));
-  p->allocations.push_back(d);
+  im->allocations.push_back(d);
   if (!host.empty()) HIP_TRY(hipMemcpy(d, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
   *out = static_cast<const T*>(d);
   return LB_OK;
@@ -136,10 +173,15 @@ void pick_config(int S, int* id_out, int* W_out) {
   *W_out = bestW;
 }
 
+// Gather tables up to this size are copied into LDS by every workgroup.
+constexpr int64_t kLdsTableBytes = 32 * 1024;
+
 // Must match carve() in lb_kernels.hip.
 int64_t lds_bytes_for(const DevPlan& P) {
+  const int64_t cells = P.lds_tables ? (int64_t)P.g_rows * 64 : 0;
   return 8 * (int64_t)(4 * P.n_pub) + 8 * 2 * (int64_t)P.S + 8 * (int64_t)((P.n + 1) & ~1) +
-         16 * 8 * (int64_t)(P.K + 1) + 16 * 8 * (int64_t)P.n_slots + 8 * 4 * (int64_t)P.nblk + 16;
+         16 * 8 * (int64_t)(P.K + 1) + 16 * 8 * (int64_t)P.n_slots + 8 * 4 * (int64_t)P.nblk +
+         16 * 8 * (int64_t)P.vm_depth + 16 * cells + 4 * ((cells + 3) & ~3) + 16;
 }
 
 int set_lds_limit(const void* fn, size_t lds) {
@@ -151,6 +193,168 @@ int set_lds_limit(const void* fn, size_t lds) {
 }  // namespace
 
 extern "C" const char* lb_last_error(void) { return g_error.c_str(); }
+
+namespace {
+
+// Closure of an initial support under the RHS program: an upper element can
+// become nonzero only if it is nonzero at t0 or one of its terms reads an
+// element that can.  Elements outside the closure stay exactly zero for the
+// whole integration (every term that writes them reads zeros), so dropping
+// them changes no value that is kept.
+std::vector<uint8_t> active_closure(const PlanSource& S, std::vector<uint8_t> act) {
+  for (bool grew = true; grew;) {
+    grew = false;
+    for (int o = 0; o < S.U; ++o) {
+      if (act[o]) continue;
+      for (int64_t k = S.output_ptrs[o]; k < S.output_ptrs[o + 1]; ++k)
+        if (act[S.input_indices[k]]) { act[o] = 1; grew = true; break; }
+    }
+  }
+  return act;
+}
+
+// Lane layout and device tables for the active elements `act` (lb_device.h).
+int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage* im) {
+  const int n = src.n, U = src.U;
+  auto is_dyn = [&](int64_t k) { return src.coeff_indices[k] >= 0 && src.coef_dyn[src.coeff_indices[k]]; };
+  // ---- split terms of active outputs into self (own element, time-independent
+  //      coefficient) and gathers
+  std::vector<int> n_gather(U, 0), n_self(U, 0);
+  std::vector<uint8_t> published(U, 0);
+  int64_t flops = 0;
+  int n_act = 0;
+  for (int o = 0; o < U; ++o) {
+    if (!act[o]) continue;
+    ++n_act;
+    for (int64_t k = src.output_ptrs[o]; k < src.output_ptrs[o + 1]; ++k) {
+      const int in = (int)src.input_indices[k];
+      if (!act[in]) return fail(LB_ERR_INVALID, "active set is not closed under the RHS program");
+      flops += (src.up_im[in] < 0) ? 4 : 8;
+      if (is_dyn(k)) flops += 6;
+      if (in == o && !is_dyn(k) && !src.input_conj[k]) {
+        ++n_self[o];
+      } else {
+        ++n_gather[o];
+        published[in] = 1;
+      }
+    }
+  }
+  // slots: active outputs ordered by gather count (descending), then index
+  std::vector<int> order;
+  for (int o = 0; o < U; ++o)
+    if (act[o]) order.push_back(o);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n_gather[a] > n_gather[b]; });
+  const int S = std::max(64, (n_act + 63) / 64 * 64), nblk = S / 64;
+  std::vector<int32_t> slot_up(S, -1), slot_re(S, -1), slot_im(S, -1), slot_pub(S, -1);
+  std::vector<int32_t> pub_of_upper(U, -1);
+  int n_pub = 0;
+  for (int s = 0; s < n_act; ++s) {
+    const int o = order[s];
+    slot_up[s] = o;
+    slot_re[s] = src.up_re[o];
+    slot_im[s] = src.up_im[o];
+    if (published[o]) { pub_of_upper[o] = n_pub; slot_pub[s] = n_pub++; }
+  }
+  if (n_pub > (1 << 20)) return fail(LB_ERR_UNSUPPORTED, "too many published elements");
+  int max_self = 1;
+  for (int o = 0; o < U; ++o) max_self = std::max(max_self, n_self[o]);
+  std::vector<int32_t> self_coef((size_t)S * max_self, -2);
+  std::vector<uint8_t> self_conj((size_t)S * max_self, 0);
+  std::vector<double> self_f((size_t)S * max_self * 2, 0.0);
+  std::vector<int32_t> blk_row0(nblk), blk_rows(nblk);
+  int rows_total = 0;
+  for (int b = 0; b < nblk; ++b) {
+    int rows = 0;
+    for (int l = 0; l < 64; ++l) {
+      int s = 64 * b + l;
+      if (s < n_act) rows = std::max(rows, n_gather[order[s]]);
+    }
+    blk_row0[b] = rows_total;
+    blk_rows[b] = rows;
+    rows_total += rows;
+  }
+  std::vector<uint32_t> g_word((size_t)rows_total * 64, lb_gword(0, true, false, false, LB_STATIC_COEF));
+  std::vector<double> g_f((size_t)rows_total * 64 * 2, 0.0);
+  for (int s = 0; s < n_act; ++s) {
+    const int o = order[s], b = s / 64, l = s % 64;
+    int js = 0, jg = 0;
+    for (int64_t k = src.output_ptrs[o]; k < src.output_ptrs[o + 1]; ++k) {
+      const int in = (int)src.input_indices[k];
+      const int64_t ci = src.coeff_indices[k];
+      if (in == o && !is_dyn(k) && !src.input_conj[k]) {
+        const size_t e = (size_t)s * max_self + js++;
+        self_coef[e] = (int32_t)ci;
+        self_conj[e] = src.coeff_conj[k];
+        self_f[2 * e] = src.factors_re[k];
+        self_f[2 * e + 1] = src.factors_im[k];
+      } else {
+        const size_t e = (size_t)(blk_row0[b] + jg++) * 64 + l;
+        g_word[e] = lb_gword((uint32_t)pub_of_upper[in], src.up_im[in] < 0, src.input_conj[k] != 0,
+                             src.coeff_conj[k] != 0, ci < 0 ? LB_STATIC_COEF : (uint32_t)ci);
+        g_f[2 * e] = src.factors_re[k];
+        g_f[2 * e + 1] = src.factors_im[k];
+      }
+    }
+  }
+  // packed index -> upper index, and the active flags (outputs of dropped
+  // elements are written as zeros)
+  std::vector<int32_t> packed_upper((size_t)n * n, 0);
+  for (int u = 0; u < U; ++u) {
+    packed_upper[src.up_re[u]] = u;
+    if (src.up_im[u] >= 0) packed_upper[src.up_im[u]] = u;
+  }
+
+  DevPlan& P = im->dev;
+  P.n = n;
+  P.dim = n * n;
+  P.upper_len = U;
+  P.S = S;
+  P.nblk = nblk;
+  P.n_pub = n_pub;
+  P.K = src.K;
+  P.n_slots = src.prog.n_slots;
+  P.n_base = src.n_base;
+  P.n_compounds = src.prog.n_compounds;
+  P.vm_depth = std::max(src.prog.max_depth, 1);
+  P.g_rows = rows_total;
+  P.lds_tables = (int64_t)rows_total * 64 * 20 <= kLdsTableBytes ? 1 : 0;
+  P.max_self = max_self;
+  P.time_dependent = src.time_dependent ? 1 : 0;
+  P.restricted = n_act < U ? 1 : 0;
+  int rc = LB_OK;
+  auto up = [&](auto& v, auto** dst) {
+    if (rc == LB_OK) rc = upload(im, v, dst);
+  };
+  up(slot_up, &P.slot_up);
+  up(slot_re, &P.slot_re);
+  up(slot_im, &P.slot_im);
+  up(slot_pub, &P.slot_pub);
+  up(self_coef, &P.self_coef);
+  up(self_conj, &P.self_conj);
+  up(self_f, &P.self_f);
+  up(blk_row0, &P.blk_row0);
+  up(blk_rows, &P.blk_rows);
+  up(g_word, &P.g_word);
+  up(g_f, &P.g_f);
+  up(src.prog.instr, &P.instr);
+  up(src.prog.expr_start, &P.expr_start);
+  up(src.compound_slot, &P.compound_slot);
+  up(src.expr_dyn, &P.expr_dyn);
+  up(src.base, &P.base);
+  up(act, &P.upper_active);
+  up(packed_upper, &P.packed_upper);
+  if (rc != LB_OK) return rc;
+  pick_config(S, &im->default_variant, &im->default_W);
+  im->lds_bytes = lds_bytes_for(P);
+  im->n_pub = n_pub;
+  im->rows_total = rows_total;
+  im->flops = flops;
+  im->active = n_act;
+  for (int o = 0; o < U; ++o) { im->self_terms += n_self[o]; im->gather_terms += n_gather[o]; }
+  return LB_OK;
+}
+
+}  // namespace
 
 extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   if (!out_plan) return fail(LB_ERR_INVALID, "out_plan is null");
@@ -182,15 +386,30 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
       return fail(LB_ERR_INVALID, "expanded RHS coefficient index out of bounds");
   }
 
+  std::unique_ptr<lb_plan> p(new lb_plan());
+  PlanSource& src = p->src;
+  src.n = n;
+  src.U = U;
+  src.K = d->n_coefficients;
+  src.n_base = d->n_base;
+  src.T = T;
+  src.output_ptrs.assign(d->output_ptrs, d->output_ptrs + U + 1);
+  src.input_indices.assign(d->input_indices, d->input_indices + T);
+  src.coeff_indices.assign(d->coeff_indices, d->coeff_indices + T);
+  src.coeff_conj.assign(d->coeff_conj, d->coeff_conj + T);
+  src.input_conj.assign(d->input_conj, d->input_conj + T);
+  src.factors_re.assign(d->factors_re, d->factors_re + T);
+  src.factors_im.assign(d->factors_im, d->factors_im + T);
+
   // ---- time dependence (plan.rs:1235-1291)
   std::vector<uint8_t> dyn_slot(d->n_slots, 0), compound_dyn(d->n_compounds, 0), coef_dyn(d->n_coefficients, 0);
   bool time_dependent = false;
   auto reads_dynamic = [&](int e, bool* uses_time) {
     bool dyn = false;
     *uses_time = false;
-    for (int p = d->expr_ptrs[e]; p < d->expr_ptrs[e + 1]; ++p) {
-      if (d->instr_op[p] == 4) { dyn = true; *uses_time = true; }
-      if (d->instr_op[p] == 2 && dyn_slot[d->instr_index[p]]) dyn = true;
+    for (int q = d->expr_ptrs[e]; q < d->expr_ptrs[e + 1]; ++q) {
+      if (d->instr_op[q] == 4) { dyn = true; *uses_time = true; }
+      if (d->instr_op[q] == 2 && dyn_slot[d->instr_index[q]]) dyn = true;
     }
     return dyn;
   };
@@ -207,168 +426,58 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
     time_dependent |= ut;
     n_dyn_coef += coef_dyn[k];
   }
+  src.coef_dyn = coef_dyn;
+  src.time_dependent = time_dependent;
+  src.n_dyn_coef = n_dyn_coef;
 
   // ---- upper index -> packed indices (rhs.rs:151-176)
-  std::vector<int32_t> up_re(U), up_im(U);
+  src.up_re.assign(U, 0);
+  src.up_im.assign(U, 0);
   {
     int cursor = n, u = 0;
     for (int i = 0; i < n; ++i)
       for (int j = i; j < n; ++j, ++u) {
-        if (i == j) { up_re[u] = i; up_im[u] = -1; }
-        else { up_re[u] = cursor; up_im[u] = cursor + 1; cursor += 2; }
+        if (i == j) { src.up_re[u] = i; src.up_im[u] = -1; }
+        else { src.up_re[u] = cursor; src.up_im[u] = cursor + 1; cursor += 2; }
       }
   }
 
-  // ---- split terms into self (own element, time-independent coefficient) and gathers
-  auto is_dyn = [&](int64_t k) { return d->coeff_indices[k] >= 0 && coef_dyn[d->coeff_indices[k]]; };
-  std::vector<int> n_gather(U, 0), n_self(U, 0);
-  std::vector<uint8_t> published(U, 0);
-  int64_t flops = 0;
-  for (int o = 0; o < U; ++o) {
-    for (int64_t k = d->output_ptrs[o]; k < d->output_ptrs[o + 1]; ++k) {
-      const int in = (int)d->input_indices[k];
-      flops += (up_im[in] < 0) ? 4 : 8;
-      if (is_dyn(k)) flops += 6;
-      if (in == o && !is_dyn(k) && !d->input_conj[k]) {
-        ++n_self[o];
-      } else {
-        ++n_gather[o];
-        published[in] = 1;
-      }
-    }
+  // ---- expression machine (time-independent subexpressions cached per trajectory)
+  {
+    std::string why;
+    if (lb_build_program(d, dyn_slot, compound_dyn, coef_dyn, &src.prog, why) != 0)
+      return fail(LB_ERR_INVALID, why);
   }
-  // slots: outputs ordered by gather count (descending), then index
-  std::vector<int> order(U);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n_gather[a] > n_gather[b]; });
-  const int S = (U + 63) / 64 * 64, nblk = S / 64;
-  std::vector<int32_t> slot_up(S, -1), slot_re(S, -1), slot_im(S, -1), slot_pub(S, -1);
-  std::vector<int32_t> pub_of_upper(U, -1);
-  int n_pub = 0;
-  for (int s = 0; s < U; ++s) {
-    const int o = order[s];
-    slot_up[s] = o;
-    slot_re[s] = up_re[o];
-    slot_im[s] = up_im[o];
-    if (published[o]) { pub_of_upper[o] = n_pub; slot_pub[s] = n_pub++; }
-  }
-  if (n_pub > (1 << 20)) return fail(LB_ERR_UNSUPPORTED, "too many published elements");
-  int max_self = 1;
-  for (int o = 0; o < U; ++o) max_self = std::max(max_self, n_self[o]);
-  std::vector<int32_t> self_coef((size_t)S * max_self, -2);
-  std::vector<uint8_t> self_conj((size_t)S * max_self, 0);
-  std::vector<double> self_f((size_t)S * max_self * 2, 0.0);
-  std::vector<int32_t> blk_row0(nblk), blk_rows(nblk);
-  int rows_total = 0;
-  for (int b = 0; b < nblk; ++b) {
-    int rows = 0;
-    for (int l = 0; l < 64; ++l) {
-      int s = 64 * b + l;
-      if (s < U) rows = std::max(rows, n_gather[order[s]]);
-    }
-    blk_row0[b] = rows_total;
-    blk_rows[b] = rows;
-    rows_total += rows;
-  }
-  std::vector<uint32_t> g_word((size_t)rows_total * 64, lb_gword(0, true, false, false, LB_STATIC_COEF));
-  std::vector<double> g_f((size_t)rows_total * 64 * 2, 0.0);
-  for (int s = 0; s < U; ++s) {
-    const int o = order[s], b = s / 64, l = s % 64;
-    int js = 0, jg = 0;
-    for (int64_t k = d->output_ptrs[o]; k < d->output_ptrs[o + 1]; ++k) {
-      const int in = (int)d->input_indices[k];
-      const int64_t ci = d->coeff_indices[k];
-      if (in == o && !is_dyn(k) && !d->input_conj[k]) {
-        const size_t e = (size_t)s * max_self + js++;
-        self_coef[e] = (int32_t)ci;
-        self_conj[e] = d->coeff_conj[k];
-        self_f[2 * e] = d->factors_re[k];
-        self_f[2 * e + 1] = d->factors_im[k];
-      } else {
-        const size_t e = (size_t)(blk_row0[b] + jg++) * 64 + l;
-        g_word[e] = lb_gword((uint32_t)pub_of_upper[in], up_im[in] < 0, d->input_conj[k] != 0,
-                             d->coeff_conj[k] != 0, ci < 0 ? LB_STATIC_COEF : (uint32_t)ci);
-        g_f[2 * e] = d->factors_re[k];
-        g_f[2 * e + 1] = d->factors_im[k];
-      }
-    }
-  }
+  src.expr_dyn = src.prog.compound_dyn;
+  src.expr_dyn.insert(src.expr_dyn.end(), src.prog.coef_dyn.begin(), src.prog.coef_dyn.end());
+  if (src.expr_dyn.empty()) src.expr_dyn.push_back(0);
+  src.compound_slot = src.prog.compound_slot;
+  if (src.compound_slot.empty()) src.compound_slot.push_back(0);
+  src.base.assign(2 * (size_t)d->n_base, 0.0);
+  for (int i = 0; i < d->n_base; ++i) { src.base[2 * i] = d->base_re[i]; src.base[2 * i + 1] = d->base_im[i]; }
 
-  // ---- expression machine
-  std::vector<DevInstr> instr(std::max(d->expr_ptrs[n_expr], 1));
-  for (int p = 0; p < d->expr_ptrs[n_expr]; ++p)
-    instr[p] = DevInstr{d->instr_op[p], d->instr_index[p], d->instr_argc[p], d->instr_function[p],
-                        d->instr_re[p], d->instr_im[p]};
-  std::vector<int32_t> expr_start(d->expr_ptrs, d->expr_ptrs + n_expr + 1);
-  std::vector<int32_t> compound_slot(d->compound_slots, d->compound_slots + d->n_compounds);
-  std::vector<double> base(2 * (size_t)d->n_base);
-  for (int i = 0; i < d->n_base; ++i) { base[2 * i] = d->base_re[i]; base[2 * i + 1] = d->base_im[i]; }
-
-  lb_plan* p = new lb_plan();
-  (void)hipGetDevice(&p->device);
-  DevPlan& P = p->dev;
-  P.n = n;
-  P.dim = n * n;
-  P.upper_len = U;
-  P.S = S;
-  P.nblk = nblk;
-  P.n_pub = n_pub;
-  P.K = d->n_coefficients;
-  P.n_slots = d->n_slots;
-  P.n_base = d->n_base;
-  P.n_compounds = d->n_compounds;
-  P.max_self = max_self;
-  P.time_dependent = time_dependent ? 1 : 0;
-  int rc = LB_OK;
-  auto up = [&](auto& v, auto** dst) {
-    if (rc == LB_OK) rc = upload(p, v, dst);
-  };
-  up(slot_up, &P.slot_up);
-  up(slot_re, &P.slot_re);
-  up(slot_im, &P.slot_im);
-  up(slot_pub, &P.slot_pub);
-  up(self_coef, &P.self_coef);
-  up(self_conj, &P.self_conj);
-  up(self_f, &P.self_f);
-  up(blk_row0, &P.blk_row0);
-  up(blk_rows, &P.blk_rows);
-  up(g_word, &P.g_word);
-  up(g_f, &P.g_f);
-  up(instr, &P.instr);
-  up(expr_start, &P.expr_start);
-  up(compound_slot, &P.compound_slot);
-  up(compound_dyn, &P.compound_dyn);
-  up(coef_dyn, &P.coef_dyn);
-  up(base, &P.base);
-  if (rc == LB_OK) {
+  // ---- device side
+  HIP_TRY(hipGetDevice(&p->device));
+  HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device));
+  {
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, 64);
-    if (e != hipSuccess) rc = fail(LB_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
-    else { p->allocations.push_back(q); p->queue = static_cast<int32_t*>(q); }
+    HIP_TRY(hipMalloc(&q, 64));
+    p->queue = static_cast<int32_t*>(q);
   }
-  if (rc != LB_OK) {
-    lb_plan_destroy(p);
-    return rc;
-  }
-  pick_config(S, &p->default_variant, &p->default_W);
-  p->lds_bytes = lds_bytes_for(P);
-  p->slot_up_host = slot_up;
-  int64_t self_terms = 0, gather_terms = 0;
-  for (int o = 0; o < U; ++o) { self_terms += n_self[o]; gather_terms += n_gather[o]; }
+  p->full.reset(new DevImage());
+  int rc = build_image(src, std::vector<uint8_t>(U, 1), p->full.get());
+  if (rc != LB_OK) return rc;
+  const DevImage& F = *p->full;
   int64_t* info = p->info;
-  info[0] = n; info[1] = n * n; info[2] = U; info[3] = T; info[4] = d->n_coefficients;
-  info[5] = time_dependent; info[6] = n_pub; info[7] = gather_terms; info[8] = self_terms;
-  info[9] = S; info[10] = rows_total; info[11] = flops; info[12] = n_dyn_coef; info[13] = p->lds_bytes;
-  info[14] = p->default_variant + 1; info[15] = p->default_W;
-  *out_plan = p;
+  info[0] = n; info[1] = (int64_t)n * n; info[2] = U; info[3] = T; info[4] = d->n_coefficients;
+  info[5] = time_dependent; info[6] = F.n_pub; info[7] = F.gather_terms; info[8] = F.self_terms;
+  info[9] = F.dev.S; info[10] = F.rows_total; info[11] = F.flops; info[12] = n_dyn_coef; info[13] = F.lds_bytes;
+  info[14] = F.default_variant + 1; info[15] = F.default_W;
+  *out_plan = p.release();
   return LB_OK;
 }
 
-extern "C" void lb_plan_destroy(lb_plan* p) {
-  if (!p) return;
-  for (void* a : p->allocations) (void)hipFree(a);
-  delete p;
-}
+extern "C" void lb_plan_destroy(lb_plan* p) { delete p; }
 
 extern "C" int lb_plan_info(const lb_plan* p, int64_t* info) {
   if (!p || !info) return fail(LB_ERR_INVALID, "null argument");
@@ -380,7 +489,7 @@ namespace {
 
 // variant: 0 = plan default, k > 0 = kernel variant k - 1.  threads: 0 = the
 // variant's smallest covering workgroup.
-int resolve_config(const lb_plan* p, int variant, int threads, int* id, int* W) {
+int resolve_config(const DevImage* p, int variant, int threads, int* id, int* W) {
   if (threads < 0 || threads % 64 != 0)
     return fail(LB_ERR_INVALID, "threads_per_traj must be a multiple of 64");
   if (variant <= 0 && threads == 0) {
@@ -413,11 +522,40 @@ int resolve_config(const lb_plan* p, int variant, int threads, int* id, int* W) 
 int fill_overrides(const lb_plan* p, int32_t n, const int32_t* slots, int32_t* dst) {
   if (n < 0 || n > LB_MAX_OVERRIDES) return fail(LB_ERR_UNSUPPORTED, "at most 16 overridden parameters");
   for (int i = 0; i < n; ++i) {
-    if (slots[i] < 0 || slots[i] >= p->dev.n_base)
+    if (slots[i] < 0 || slots[i] >= p->src.n_base)
       return fail(LB_ERR_INVALID, "parameter override slot " + std::to_string(slots[i]) +
                                       " is not a base parameter slot");
     dst[i] = slots[i];
   }
+  return LB_OK;
+}
+
+// The device image for a solve: the full plan, or (with an initial-support
+// mask over packed indices) the plan restricted to that support's closure,
+// built once and cached.
+int select_image(lb_plan* p, const uint8_t* y0_support, const DevImage** out) {
+  *out = p->full.get();
+  if (!y0_support) return LB_OK;
+  const PlanSource& S = p->src;
+  std::vector<uint8_t> seed(S.U, 0);
+  for (int u = 0; u < S.U; ++u)
+    seed[u] = (y0_support[S.up_re[u]] || (S.up_im[u] >= 0 && y0_support[S.up_im[u]])) ? 1 : 0;
+  std::vector<uint8_t> act = active_closure(S, seed);
+  if (std::all_of(act.begin(), act.end(), [](uint8_t v) { return v != 0; })) return LB_OK;
+  std::lock_guard<std::mutex> lock(p->mu);
+  auto it = p->restricted.find(act);
+  if (it != p->restricted.end()) {
+    *out = it->second.get();
+    return LB_OK;
+  }
+  std::unique_ptr<DevImage> im(new DevImage());
+  int rc = build_image(S, act, im.get());
+  if (rc != LB_OK) {
+    for (void* a : im->allocations) (void)hipFree(a);
+    return rc;
+  }
+  *out = im.get();
+  p->restricted.emplace(std::move(act), std::move(im));
   return LB_OK;
 }
 
@@ -438,13 +576,14 @@ extern "C" int lb_rhs_eval(const lb_plan* p, int64_t batch, const double* y, con
   c.override_values = override_values;
   c.out = out;
   int id, W;
-  if ((rc = resolve_config(p, 0, 0, &id, &W))) return rc;
+  const DevImage* im = p->full.get();
+  if ((rc = resolve_config(im, 0, 0, &id, &W))) return rc;
   void* fn = lb_dev_rhs_kernel(id);
-  DevPlan P = p->dev;
+  DevPlan P = im->dev;
   void* args[] = {&P, &c};
-  if ((rc = set_lds_limit(fn, (size_t)p->lds_bytes))) return rc;
+  if ((rc = set_lds_limit(fn, (size_t)im->lds_bytes))) return rc;
   int grid = (int)std::min<int64_t>(batch, 4096);
-  HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, (size_t)p->lds_bytes, (hipStream_t)stream));
+  HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, (size_t)im->lds_bytes, (hipStream_t)stream));
   HIP_TRY(hipGetLastError());
   return LB_OK;
 }
@@ -459,6 +598,10 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
     return fail(LB_ERR_INVALID, "solver must be dopri5 or tsit5");
   if (!std::isfinite(a->t0) || (!a->t1_per_traj && !std::isfinite(a->t1)))
     return fail(LB_ERR_INVALID, "t_span must be finite");
+  if (a->n_traj > INT32_MAX) return fail(LB_ERR_UNSUPPORTED, "at most 2^31-1 trajectories per call");
+  const DevImage* im = nullptr;
+  int rc = select_image(const_cast<lb_plan*>(p), a->y0_support, &im);
+  if (rc) return rc;
   DevCall c{};
   c.n_traj = a->n_traj;
   c.t0 = a->t0;
@@ -472,8 +615,7 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   c.solver = a->solver;
   c.y0 = a->y0;
   c.y0_shared = a->y0_shared;
-  int rc = fill_overrides(p, a->n_override, a->override_slots, c.override_slot);
-  if (rc) return rc;
+  if ((rc = fill_overrides(p, a->n_override, a->override_slots, c.override_slot))) return rc;
   c.n_override = a->n_override;
   c.override_values = a->override_values;
   if (a->n_axes > 0) {
@@ -498,14 +640,14 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   if (a->n_weights > LB_MAX_WEIGHTS) return fail(LB_ERR_UNSUPPORTED, "at most 64 integral weights");
   c.n_weights = a->n_weights;
   for (int i = 0; i < a->n_weights; ++i) {
-    if (a->weight_indices[i] < 0 || a->weight_indices[i] >= p->dev.n)
+    if (a->weight_indices[i] < 0 || a->weight_indices[i] >= p->src.n)
       return fail(LB_ERR_INVALID, "integral weight index out of range");
     c.weight_index[i] = a->weight_indices[i];
     c.weight_value[i] = a->weight_values[i];
   }
   if (a->output_kind == LB_OUT_SELECTED) {
     if (a->n_selected > LB_MAX_SELECTED) return fail(LB_ERR_UNSUPPORTED, "at most 128 selected entries");
-    const int n = p->dev.n;
+    const int n = p->src.n;
     for (int q = 0; q < a->n_selected; ++q) {
       int i = a->selected_rows[q], j = a->selected_cols[q];
       if (i < 0 || j < 0 || i >= n || j >= n) return fail(LB_ERR_INVALID, "selected index out of range");
@@ -525,12 +667,12 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
     return fail(LB_ERR_INVALID, "integral outputs require integral weights");
   c.queue = p->queue;
   int id, W;
-  if ((rc = resolve_config(p, a->variant, a->threads_per_traj, &id, &W))) return rc;
+  if ((rc = resolve_config(im, a->variant, a->threads_per_traj, &id, &W))) return rc;
   void* fn = lb_dev_solve_kernel(id, a->solver);
   void* reset = lb_dev_queue_reset_kernel();
   hipStream_t s = (hipStream_t)stream;
-  DevPlan P = p->dev;
-  const size_t lds = (size_t)p->lds_bytes;
+  DevPlan P = im->dev;
+  const size_t lds = (size_t)im->lds_bytes;
   if ((rc = set_lds_limit(fn, lds))) return rc;
   if (a->mode == LB_MODE_PLAIN) {
     // one launch per trajectory: the per-trajectory GPU path
@@ -548,12 +690,19 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   }
   int grid = a->workgroups;
   if (grid <= 0) {
-    int per_cu = 0, cus = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, W, lds));
-    hipDeviceProp_t prop;
-    HIP_TRY(hipGetDeviceProperties(&prop, p->device));
-    cus = prop.multiProcessorCount;
-    grid = std::max(1, per_cu) * std::max(1, cus);
+    int per_cu = 0;
+    {
+      std::lock_guard<std::mutex> lock(const_cast<lb_plan*>(p)->mu);
+      auto& cache = const_cast<DevImage*>(im)->per_cu;
+      auto it = cache.find({id, W});
+      if (it != cache.end()) {
+        per_cu = it->second;
+      } else {
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, W, lds));
+        cache[{id, W}] = per_cu;
+      }
+    }
+    grid = std::max(1, per_cu) * std::max(1, p->cus);
   }
   grid = (int)std::min<int64_t>(grid, a->n_traj);
   c.traj_offset = 0;

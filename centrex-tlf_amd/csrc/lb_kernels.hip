@@ -14,6 +14,7 @@
 // Data layout: lb_device.h.
 
 #include "lb_device.h"
+#include "lb_variants.h"
 
 #include <float.h>
 
@@ -150,47 +151,68 @@ __device__ __noinline__ dcpx dev_builtin(int f, dcpx v) {
   }
 }
 
-__device__ __forceinline__ dcpx vm_eval(const DevInstr* ins, int start, int end, const dcpx* slots, double t) {
-  dcpx st[LB_MAX_STACK];
-  int sp = 0;
+// x^c for a real constant c, with cs = cos(c*pi) and sn = sin(c*pi) from the
+// host: the polar form of dev_cpow with the angle of a negative real base
+// (atan2(+-0, x<0) = +-pi) folded into the constants.
+__device__ __forceinline__ dcpx dev_powc(dcpx a, double c, double cs, double sn) {
+  if (c == 0.0) return mk(1.0, 0.0);
+  if (a.im != 0.0) return dev_cpow(a, mk(c, 0.0));
+  const double r = fabs(a.re);
+  if (r == 0.0) return mk(0.0, 0.0);
+  const double m = (c == 2.0) ? r * r : pow(r, c);
+  if (a.re > 0.0) return mk(m, 0.0);
+  return mk(m * cs, signbit(a.im) ? -(m * sn) : m * sn);
+}
+
+// Stack machine of rust/src/lindblad/eval.rs:791-911 (scalar form).  The plan
+// builder has checked ops, helper ids and the stack depth.  The top of the
+// stack lives in registers and the entries below it in this lane's LDS
+// window `stk` (nothing goes to scratch memory).
+__device__ __forceinline__ dcpx vm_eval(const DevInstr* ins, int start, int end, const dcpx* slots,
+                                        double t, dcpx* stk) {
+  dcpx top = mk(0.0, 0.0);
+  int d = 0;  // entries on the stack; entry d-1 is `top`
   for (int p = start; p < end; ++p) {
     const DevInstr q = ins[p];
-    switch (q.op) {
-      case 1: st[sp++] = mk(q.re, q.im); break;
-      case 2: st[sp++] = slots[q.index]; break;
-      case 4: st[sp++] = mk(t, 0.0); break;
-      case 5: --sp; st[sp - 1] = cadd(st[sp - 1], st[sp]); break;
-      case 6: --sp; st[sp - 1] = csub(st[sp - 1], st[sp]); break;
-      case 7: --sp; st[sp - 1] = cmul(st[sp - 1], st[sp]); break;
-      case 8: --sp; st[sp - 1] = cdiv(st[sp - 1], st[sp]); break;
-      case 9: --sp; st[sp - 1] = dev_cpow(st[sp - 1], st[sp]); break;
-      case 10: st[sp - 1] = mk(-st[sp - 1].re, -st[sp - 1].im); break;
-      case 11: st[sp - 1].im = -st[sp - 1].im; break;
+    const int op = q.op;
+    if (op <= 4) {
+      if (d > 0) stk[d - 1] = top;
+      ++d;
+      top = op == 1 ? mk(q.re, q.im) : op == 2 ? slots[q.index] : mk(t, 0.0);
+      continue;
+    }
+    switch (op) {
+      case 5: top = cadd(stk[d - 2], top); --d; break;
+      case 6: top = csub(stk[d - 2], top); --d; break;
+      case 7: top = cmul(stk[d - 2], top); --d; break;
+      case 8: top = cdiv(stk[d - 2], top); --d; break;
+      case 9: top = dev_cpow(stk[d - 2], top); --d; break;
+      case 10: top = mk(-top.re, -top.im); break;
+      case 11: top.im = -top.im; break;
       case 12: {
-        int s0 = sp - q.argc;
-        dcpx r = dev_builtin(q.function, st[s0]);
-        sp = s0;
-        st[sp++] = r;
+        const dcpx v = q.argc == 1 ? top : stk[d - q.argc];
+        if (q.function == 4 && v.im == 0.0) top = mk(exp(v.re), 0.0);
+        else top = dev_builtin(q.function, v);
+        d -= q.argc - 1;
         break;
       }
-      case 13: {
-        int s0 = sp - q.argc;
-        dcpx r = dev_helper(q.function, &st[s0], q.argc);
-        sp = s0;
-        st[sp++] = r;
+      case 13:
+        stk[d - 1] = top;
+        top = dev_helper(q.function, &stk[d - q.argc], q.argc);
+        d -= q.argc - 1;
         break;
-      }
+      case LB_OP_POWC: top = dev_powc(top, q.re, q.im, q.aux); break;
       default: {
-        double rv = st[sp - 1].re, lv = st[sp - 2].re;
-        sp -= 2;
-        bool res = q.op == 14 ? lv > rv : q.op == 15 ? lv >= rv : q.op == 16 ? lv < rv
-                 : q.op == 17 ? lv <= rv : q.op == 18 ? lv == rv : lv != rv;
-        st[sp++] = mk(res ? 1.0 : 0.0, 0.0);
+        const double rv = top.re, lv = stk[d - 2].re;
+        --d;
+        const bool res = op == 14 ? lv > rv : op == 15 ? lv >= rv : op == 16 ? lv < rv
+                       : op == 17 ? lv <= rv : op == 18 ? lv == rv : lv != rv;
+        top = mk(res ? 1.0 : 0.0, 0.0);
         break;
       }
     }
   }
-  return st[0];
+  return top;
 }
 
 // Butcher tableaus (rust/src/ode/dopri5.rs:5-37, rust/src/ode/tsit5.rs:5-38).
@@ -292,7 +314,10 @@ struct Lds {
   double* wts;    // n integral weights per state
   dcpx* coef;     // 8 x (K+1): [0] t0 and hinit, [1] hinit second point, [2..7] stages
   dcpx* slots;    // 8 x n_slots
-  double* red;    // nblk x 4
+  double* red;    // 4 x nblk block sums
+  dcpx* vm;       // 8 lanes x vm_depth expression stacks
+  double2* gf;    // gather factors (when the plan stages its tables in LDS)
+  uint32_t* gw;   // gather words
   int* misc;      // [0] queue value
 };
 
@@ -305,21 +330,69 @@ __device__ __forceinline__ Lds carve(const DevPlan& P) {
   L.coef = reinterpret_cast<dcpx*>(L.wts + ((P.n + 1) & ~1));
   L.slots = L.coef + 8 * (P.K + 1);
   L.red = reinterpret_cast<double*>(L.slots + 8 * P.n_slots);
-  L.misc = reinterpret_cast<int*>(L.red + 4 * P.nblk);
+  L.vm = reinterpret_cast<dcpx*>(L.red + 4 * P.nblk);
+  L.gf = reinterpret_cast<double2*>(L.vm + 8 * P.vm_depth);
+  const size_t cells = P.lds_tables ? (size_t)P.g_rows * 64 : 0;
+  L.gw = reinterpret_cast<uint32_t*>(L.gf + cells);
+  L.misc = reinterpret_cast<int*>(L.gw + ((cells + 3) & ~(size_t)3));
   return L;
+}
+
+// One gather term: acc += f * c * x_in (rhs.rs:815-876 term semantics).
+__device__ __forceinline__ void gather_term(uint32_t w, double2 f, const dcpx* coef, const double* pubbuf,
+                                            double& accr, double& acci) {
+  const uint32_t cid = w >> 24;
+  double vr = f.x, vi = f.y;
+  if (cid != LB_STATIC_COEF) {
+    dcpx c = coef[cid];
+    if (w & (1u << 22)) c.im = -c.im;
+    vr = fma(f.x, c.re, -(f.y * c.im));
+    vi = fma(f.x, c.im, f.y * c.re);
+  }
+  const uint32_t p = w & 0xFFFFFu;
+  const double x_r = pubbuf[2 * p];
+  double x_i = (w & (1u << 20)) ? 0.0 : pubbuf[2 * p + 1];
+  if (w & (1u << 21)) x_i = -x_i;
+  // explicit fused operations: the library is built without implicit
+  // contraction, so every kernel variant rounds identically
+  accr = fma(-vi, x_i, fma(vr, x_r, accr));
+  acci = fma(vi, x_r, fma(vr, x_i, acci));
+}
+
+// The gather rows of one block, in row order.  The tables are read through
+// generic pointers (LDS copy or global memory, one code path) with the next
+// row's word and factor loaded before the current row is applied.
+__device__ __forceinline__ void gather_block(const uint32_t* gw, const double2* gf, int row0, int rows,
+                                             int lane, const dcpx* coef, const double* pubbuf,
+                                             double& accr, double& acci) {
+  if (rows <= 0) return;
+  size_t idx = (size_t)row0 * 64 + lane;
+  uint32_t w = gw[idx];
+  double2 f = gf[idx];
+  for (int j = 1; j < rows; ++j) {
+    idx += 64;
+    const uint32_t wn = gw[idx];
+    const double2 fn = gf[idx];
+    gather_term(w, f, coef, pubbuf, accr, acci);
+    w = wn;
+    f = fn;
+  }
+  gather_term(w, f, coef, pubbuf, accr, acci);
 }
 
 // d(rho)/dt for the slots of this thread: publish the cross-coupled elements,
 // barrier, then self term + gathers.  `pubbuf` alternates between two buffers
 // so one barrier per evaluation suffices.
 template <int R>
-__device__ __forceinline__ void rhs_eval(const DevPlan& P, const Slots<R>& S, const double (&xr)[R],
-                                         const double (&xi)[R], double (&kr)[R], double (&ki)[R],
-                                         double* pubbuf, const double* selfc, const dcpx* coef,
-                                         int tid, int W) {
+__device__ __forceinline__ void rhs_eval(const DevPlan& P, const Lds& L, const Slots<R>& S,
+                                         const double (&xr)[R], const double (&xi)[R], double (&kr)[R],
+                                         double (&ki)[R], double* pubbuf, const dcpx* coef, int tid,
+                                         int W) {
   const int lane = tid & 63;
   // wave-uniform block index: lets the row counts live in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t* gw = P.lds_tables ? static_cast<const uint32_t*>(L.gw) : P.g_word;
+  const double2* gf = P.lds_tables ? static_cast<const double2*>(L.gf) : reinterpret_cast<const double2*>(P.g_f);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (S.pub[r] >= 0) {
@@ -333,72 +406,79 @@ __device__ __forceinline__ void rhs_eval(const DevPlan& P, const Slots<R>& S, co
     const int blk = wave + (W >> 6) * r;
     if (blk >= P.nblk) { kr[r] = 0.0; ki[r] = 0.0; continue; }
     const int s = tid + W * r;
-    const double2 a = reinterpret_cast<const double2*>(selfc)[s];
-    double accr = a.x * xr[r] - a.y * xi[r];
-    double acci = a.x * xi[r] + a.y * xr[r];
-    const int rows = P.blk_rows[blk];
-    const int row0 = P.blk_row0[blk];
-    for (int j = 0; j < rows; ++j) {
-      const size_t idx = (size_t)(row0 + j) * 64 + lane;
-      const uint32_t w = P.g_word[idx];
-      const double2 f = reinterpret_cast<const double2*>(P.g_f)[idx];
-      const uint32_t cid = w >> 24;
-      double vr = f.x, vi = f.y;
-      if (cid != LB_STATIC_COEF) {
-        dcpx c = coef[cid];
-        if (w & (1u << 22)) c.im = -c.im;
-        vr = f.x * c.re - f.y * c.im;
-        vi = f.x * c.im + f.y * c.re;
-      }
-      const uint32_t p = w & 0xFFFFFu;
-      const double x_r = pubbuf[2 * p];
-      double x_i = (w & (1u << 20)) ? 0.0 : pubbuf[2 * p + 1];
-      if (w & (1u << 21)) x_i = -x_i;
-      accr += vr * x_r - vi * x_i;
-      acci += vr * x_i + vi * x_r;
-    }
+    const double2 a = reinterpret_cast<const double2*>(L.selfc)[s];
+    double accr = fma(a.x, xr[r], -(a.y * xi[r]));
+    double acci = fma(a.x, xi[r], a.y * xr[r]);
+    gather_block(gw, gf, P.blk_row0[blk], P.blk_rows[blk], lane, coef, pubbuf, accr, acci);
     kr[r] = accr;
     ki[r] = S.diag(r) ? 0.0 : acci;
   }
 }
 
-// Butterfly sum of one value over the 64 lanes of a wave; lane 0's result has
-// a fixed association order.
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-  return x;
+// Copy the gather tables into LDS once per workgroup (plans whose tables fit).
+__device__ __forceinline__ void stage_tables(const DevPlan& P, const Lds& L, int tid, int W) {
+  if (!P.lds_tables) return;
+  const int cells = P.g_rows * 64;
+  const double2* gf = reinterpret_cast<const double2*>(P.g_f);
+  for (int i = tid; i < cells; i += W) {
+    L.gf[i] = gf[i];
+    L.gw[i] = P.g_word[i];
+  }
 }
 
-// Store the block sums of slot row r into red[4*blk + q0 ...].
+// Sum over the 64 lanes of a wave with a fixed association order: neighbour
+// pairs (xor 1, xor 2) by DPP quad permutes, then the mirrored halves of each
+// 8- and 16-lane row, then the four rows as (r0 + r1) + (r2 + r3).  Every
+// lane returns the same value; all 64 lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_d(double x, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                          __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+__device__ __forceinline__ double wave_sum(double x) {
+  x += dpp_d<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_d<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_d<0x141>(x);  // row_half_mirror
+  x += dpp_d<0x140>(x);  // row_mirror
+  return (lane_d(x, 0) + lane_d(x, 16)) + (lane_d(x, 32) + lane_d(x, 48));
+}
+
+// Store the block sum of slot row r as red[q*nblk + blk].
 __device__ __forceinline__ void block_store(const DevPlan& P, int tid, int W, int r, double* red, int q,
                                             double value) {
   const int s = tid + W * r;
   const double sum = wave_sum(value);
-  if ((tid & 63) == 0 && s < P.S) red[4 * (s >> 6) + q] = sum;
+  if ((tid & 63) == 0 && s < P.S) red[q * P.nblk + (s >> 6)] = sum;
 }
 
-// Sum the stored block values in block order (call after a barrier); the same
+// Total of the stored block sums (call after a barrier): lane l adds blocks
+// l, l+64, ... in order, then the lanes are summed as in wave_sum.  The same
 // in every thread and independent of the workgroup size.
 __device__ __forceinline__ double block_total(const DevPlan& P, const double* red, int q) {
-  double s = 0.0;
-  for (int b = 0; b < P.nblk; ++b) s += red[4 * b + q];
-  return s;
+  const int lane = threadIdx.x & 63;
+  double v = 0.0;
+  for (int b = lane; b < P.nblk; b += 64) v += red[q * P.nblk + b];
+  return wave_sum(v);
 }
 
 __device__ __forceinline__ double time_tol(double t) { return 10.0 * DBL_EPSILON * fmax(fabs(t), 1.0); }
 
 // Evaluate the parameter graph + coefficients for time t into slot copy `sl`
-// and coefficient row `cf`; only_dynamic skips the time-independent parts.
+// and coefficient row `cf` (compounds first, in dependency order, then the
+// coefficients); only_dynamic skips the time-independent expressions.
 __device__ __forceinline__ void eval_coefficients(const DevPlan& P, dcpx* sl, dcpx* cf, double t,
-                                               bool only_dynamic) {
-  for (int c = 0; c < P.n_compounds; ++c) {
-    if (only_dynamic && !P.compound_dyn[c]) continue;
-    sl[P.compound_slot[c]] = vm_eval(P.instr, P.expr_start[c], P.expr_start[c + 1], sl, t);
-  }
-  for (int k = 0; k < P.K; ++k) {
-    if (only_dynamic && !P.coef_dyn[k]) continue;
-    cf[k] = vm_eval(P.instr, P.expr_start[P.n_compounds + k], P.expr_start[P.n_compounds + k + 1], sl, t);
+                                                  bool only_dynamic, dcpx* stk) {
+  const int n_expr = P.n_compounds + P.K;
+  for (int e = 0; e < n_expr; ++e) {
+    if (only_dynamic && !P.expr_dyn[e]) continue;
+    const dcpx v = vm_eval(P.instr, P.expr_start[e], P.expr_start[e + 1], sl, t, stk);
+    if (e < P.n_compounds) sl[P.compound_slot[e]] = v;
+    else cf[e - P.n_compounds] = v;
   }
   cf[P.K] = mk(1.0, 0.0);
 }
@@ -431,7 +511,9 @@ __device__ __forceinline__ void apply_overrides(const DevCall& C, int64_t traj, 
   for (int j = 0; j < C.n_override; ++j) {
     double vr, vi;
     if (C.n_axes > 0) {
-      const int64_t ai = C.axis_offset[j] + (traj / C.axis_stride[j]) % C.axis_len[j];
+      // n_traj < 2^31 (checked by the host), so 32-bit division suffices
+      const uint32_t q = (uint32_t)traj / (uint32_t)C.axis_stride[j];
+      const int64_t ai = C.axis_offset[j] + q % (uint32_t)C.axis_len[j];
       vr = C.axis_values[2 * ai];
       vi = C.axis_values[2 * ai + 1];
     } else {
@@ -475,6 +557,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
   const int KK = P.K + 1;
   Slots<R> S;
   load_slots<R>(P, S, tid, W);
+  stage_tables(P, L, tid, W);
 
   for (;;) {
     __syncthreads();
@@ -490,7 +573,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     init_slots(P, L.slots, tid, W);
     if (tid == 0) {
       apply_overrides(C, traj, L.slots);
-      eval_coefficients(P, L.slots, L.coef, t0, false);
+      eval_coefficients(P, L.slots, L.coef, t0, false, L.vm);
     }
     for (int i = tid; i < P.n; i += W) {
       double w = 0.0;
@@ -512,6 +595,16 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
       y_r[r] = S.valid(r) ? y0[P.slot_re[s]] : 0.0;
       y_i[r] = (S.valid(r) && !S.diag(r)) ? y0[P.slot_im[s]] : 0.0;
     }
+    // a restricted plan needs y0 to vanish outside its elements
+    bool bad_support = false;
+    if (P.restricted) {
+      if (tid == 0) L.misc[1] = 0;
+      __syncthreads();
+      for (int k = tid; k < P.dim; k += W)
+        if (!P.upper_active[P.packed_upper[k]] && y0[k] != 0.0) L.misc[1] = 1;
+      __syncthreads();
+      bad_support = L.misc[1] != 0;
+    }
 
     int64_t accepted = 0, rejected = 0, rhs_calls = 0;
     int status = 0;
@@ -520,13 +613,16 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     int buf = 0;
 #define PUB (L.pub + buf * 2 * P.n_pub)
 
-    if (t1 < t0) {
+    if (bad_support) {
+      status = LB_TRAJ_BAD_SUPPORT;
+      fail_x = t0;
+    } else if (t1 < t0) {
       status = LB_TRAJ_BAD_SPAN;
       fail_x = t0;
     } else if (t1 > t0) {
       double x = t0, h;
       // ---- k1 = f(t0, y0) (also f0 of hinit); the barrier inside orders the LDS fills above
-      rhs_eval<R>(P, S, y_r, y_i, k_r[1], k_i[1], PUB, L.selfc, L.coef, tid, W);
+      rhs_eval<R>(P, L, S, y_r, y_i, k_r[1], k_i[1], PUB, L.coef, tid, W);
       buf ^= 1;
       if (isfinite(C.dt) && C.dt > 0.0) {
         h = fmin(C.dt, t1 - t0);
@@ -544,14 +640,14 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         const double d0 = block_total(P, L.red, 0), d1 = block_total(P, L.red, 1), hmax = fabs(t1 - x);
         double h0 = (d0 < 1.0e-10 || d1 < 1.0e-10) ? 1.0e-6 : 0.01 * sqrt(d0 / d1);
         h0 = fmin(h0, hmax);
-        if (tid == 0) eval_coefficients(P, L.slots + P.n_slots, L.coef + KK, x + h0, true);
+        if (tid == 0) eval_coefficients(P, L.slots + P.n_slots, L.coef + KK, x + h0, true, L.vm);
         double yt_r[R], yt_i[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           yt_r[r] = y_r[r] + k_r[1][r] * h0;
           yt_i[r] = y_i[r] + k_i[1][r] * h0;
         }
-        rhs_eval<R>(P, S, yt_r, yt_i, k_r[2], k_i[2], PUB, L.selfc, L.coef + KK, tid, W);
+        rhs_eval<R>(P, L, S, yt_r, yt_i, k_r[2], k_i[2], PUB, L.coef + KK, tid, W);
         buf ^= 1;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -590,31 +686,54 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         // time-dependent coefficients at the six stage times, one lane per stage
         if (P.time_dependent && tid < 6) {
           const int s = tid + 2;
-          eval_coefficients(P, L.slots + s * P.n_slots, L.coef + s * KK, x + h * T::c(s), true);
+          eval_coefficients(P, L.slots + s * P.n_slots, L.coef + s * KK, x + h * T::c(s), true,
+                            L.vm + tid * P.vm_depth);
         }
         const bool need_stiff = ((accepted + 1) % 1000 == 0) || ia > 0;
         double yt_r[R], yt_i[R], yn_r[R], yn_i[R];
-#define LB_STAGE(s)                                                                          \
-  stage_combo<R, SOLVER, s>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                              \
-  rhs_eval<R>(P, S, yt_r, yt_i, k_r[s], k_i[s], PUB, L.selfc, L.coef + (s) * KK, tid, W); \
-  buf ^= 1;
-        LB_STAGE(2)
-        LB_STAGE(3)
-        LB_STAGE(4)
-        LB_STAGE(5)
-        LB_STAGE(6)
-#undef LB_STAGE
-        stage_combo<R, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yn_r, yn_i);
-        if (need_stiff) {
-          // |yn - y_stage6|^2 per block (dopri5.rs:255-264)
+        // Stages 2..7 share one RHS site (a rolled loop keeps the step's code
+        // small enough for the instruction cache); stage 7 is the solution,
+        // whose RHS is the FSAL derivative.
+#pragma unroll 1
+        for (int s = 2; s <= 7; ++s) {
+          switch (s) {
+            case 2: stage_combo<R, SOLVER, 2>(h, y_r, y_i, k_r, k_i, yt_r, yt_i); break;
+            case 3: stage_combo<R, SOLVER, 3>(h, y_r, y_i, k_r, k_i, yt_r, yt_i); break;
+            case 4: stage_combo<R, SOLVER, 4>(h, y_r, y_i, k_r, k_i, yt_r, yt_i); break;
+            case 5: stage_combo<R, SOLVER, 5>(h, y_r, y_i, k_r, k_i, yt_r, yt_i); break;
+            case 6: stage_combo<R, SOLVER, 6>(h, y_r, y_i, k_r, k_i, yt_r, yt_i); break;
+            default:
+              stage_combo<R, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yn_r, yn_i);
+              if (need_stiff) {
+                // |yn - y_stage6|^2 per block (dopri5.rs:255-264)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                  const double a0 = yn_r[r] - yt_r[r], a1 = yn_i[r] - yt_i[r];
+                  block_store(P, tid, W, r, L.red, 3, a0 * a0 + a1 * a1);
+                }
+              }
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                yt_r[r] = yn_r[r];
+                yt_i[r] = yn_i[r];
+              }
+              break;
+          }
+          double ko_r[R], ko_i[R];
+          rhs_eval<R>(P, L, S, yt_r, yt_i, ko_r, ko_i, PUB, L.coef + s * KK, tid, W);
+          buf ^= 1;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const double a0 = yn_r[r] - yt_r[r], a1 = yn_i[r] - yt_i[r];
-            block_store(P, tid, W, r, L.red, 3, a0 * a0 + a1 * a1);
+            switch (s) {
+              case 2: k_r[2][r] = ko_r[r]; k_i[2][r] = ko_i[r]; break;
+              case 3: k_r[3][r] = ko_r[r]; k_i[3][r] = ko_i[r]; break;
+              case 4: k_r[4][r] = ko_r[r]; k_i[4][r] = ko_i[r]; break;
+              case 5: k_r[5][r] = ko_r[r]; k_i[5][r] = ko_i[r]; break;
+              case 6: k_r[6][r] = ko_r[r]; k_i[6][r] = ko_i[r]; break;
+              default: k_r[7][r] = ko_r[r]; k_i[7][r] = ko_i[r]; break;
+            }
           }
         }
-        rhs_eval<R>(P, S, yn_r, yn_i, k_r[7], k_i[7], PUB, L.selfc, L.coef + 7 * KK, tid, W);
-        buf ^= 1;
         rhs_calls += 6;
         // ---- error norm (dopri5.rs:88-102, tsit5.rs:104-119), integrand, stiffness
 #pragma unroll
@@ -694,6 +813,22 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     // ---- outputs
     const bool ok = status == 0;
     const double nanv = __builtin_nan("");
+    if (P.restricted) {
+      // elements outside the restricted plan are exactly zero
+      if (C.output_kind == LB_OUT_POPULATIONS) {
+        for (int i = tid; i < P.n; i += W)
+          if (!P.upper_active[P.packed_upper[i]]) C.out_values[traj * P.n + i] = ok ? 0.0 : nanv;
+      } else if (C.output_kind == LB_OUT_FULL) {
+        for (int k = tid; k < P.dim; k += W)
+          if (!P.upper_active[P.packed_upper[k]]) C.out_values[traj * P.dim + k] = ok ? 0.0 : nanv;
+      } else if (C.output_kind == LB_OUT_SELECTED) {
+        for (int q = tid; q < C.n_selected; q += W)
+          if (!P.upper_active[C.sel_upper[q]]) {
+            C.out_values[2 * (traj * C.n_selected + q)] = ok ? 0.0 : nanv;
+            C.out_values[2 * (traj * C.n_selected + q) + 1] = ok ? 0.0 : nanv;
+          }
+      }
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (!S.valid(r)) continue;
@@ -734,6 +869,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_rhs_kernel(const DevPlan P, co
   const Lds L = carve(P);
   Slots<R> S;
   load_slots<R>(P, S, tid, W);
+  stage_tables(P, L, tid, W);
   for (int64_t b = blockIdx.x; b < C.batch; b += gridDim.x) {
     __syncthreads();
     init_slots(P, L.slots, tid, W);
@@ -741,7 +877,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_rhs_kernel(const DevPlan P, co
       for (int j = 0; j < C.n_override; ++j)
         L.slots[C.override_slot[j]] = mk(C.override_values[2 * (b * C.n_override + j)],
                                          C.override_values[2 * (b * C.n_override + j) + 1]);
-      eval_coefficients(P, L.slots, L.coef, C.t[b], false);
+      eval_coefficients(P, L.slots, L.coef, C.t[b], false, L.vm);
     }
     __syncthreads();
     fold_self(P, tid, W, L.coef, L.selfc);
@@ -753,7 +889,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_rhs_kernel(const DevPlan P, co
       xr[r] = S.valid(r) ? y[P.slot_re[s]] : 0.0;
       xi[r] = (S.valid(r) && !S.diag(r)) ? y[P.slot_im[s]] : 0.0;
     }
-    rhs_eval<R>(P, S, xr, xi, kr, ki, L.pub, L.selfc, L.coef, tid, W);
+    rhs_eval<R>(P, L, S, xr, xi, kr, ki, L.pub, L.coef, tid, W);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (!S.valid(r)) continue;
@@ -764,53 +900,60 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_rhs_kernel(const DevPlan P, co
   }
 }
 
+#if LB_PART == 0
 __global__ void lb_queue_reset_kernel(int32_t* q) { *q = 0; }
+#endif
 
 // ---------------------------------------------------------------------------
 // Kernel variants: (id, slots per thread R, largest workgroup, waves-per-SIMD
 // hint).  The hint caps registers per lane (512 / hint); the host picks a
 // variant with R * W >= plan slots (lb_host.cpp) and the autotuner measures.
 // ---------------------------------------------------------------------------
-#define LB_CONFIGS(X)                                                                            \
-  X(0, 1, 1024, 4) X(1, 2, 512, 4) X(2, 2, 512, 2) X(3, 3, 1024, 4) X(4, 4, 256, 4)                \
-  X(5, 4, 256, 2) X(6, 4, 256, 1) X(7, 6, 512, 2) X(8, 6, 512, 1) X(9, 8, 128, 1) X(10, 12, 256, 1)
+// Variant table (id, slots per thread R, largest workgroup, waves-per-SIMD hint)
+// in lb_variants.h.  The library compiles this file LB_NPART times; part
+// LB_PART instantiates the variants with id % LB_NPART == LB_PART, so the
+// parts build in parallel.
+#ifndef LB_PART
+#define LB_PART 0
+#define LB_NPART 1
+#endif
+#define LB_CAT2(a, b) a##b
+#define LB_CAT(a, b) LB_CAT2(a, b)
+#define LB_PART_FN(name) LB_CAT(name##_p, LB_PART)
 
-extern "C" int lb_dev_variant_count(void);
-extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
-extern "C" void* lb_dev_solve_kernel(int id, int solver);
-extern "C" void* lb_dev_rhs_kernel(int id);
-extern "C" void* lb_dev_queue_reset_kernel(void);
-
-extern "C" int lb_dev_variant_count(void) {
-  int n = 0;
-#define X(i, r, w, v) ++n;
-  LB_CONFIGS(X)
-#undef X
-  return n;
+namespace {
+template <int I, int RR, int WW, int VV>
+void* solve_ptr(int solver) {
+  if constexpr (I % LB_NPART == LB_PART)
+    return solver == 1 ? (void*)&lb_solve_kernel<RR, 1, WW, VV> : (void*)&lb_solve_kernel<RR, 0, WW, VV>;
+  else
+    return nullptr;
 }
-
-extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves) {
-#define X(i, r, w, v)                                                                           \
-  if (id == i) { *R = r; *maxw = w; *waves = v; return 1; }
-  LB_CONFIGS(X)
-#undef X
-  return 0;
+template <int I, int RR, int WW, int VV>
+void* rhs_ptr() {
+  if constexpr (I % LB_NPART == LB_PART)
+    return (void*)&lb_rhs_kernel<RR, WW, VV>;
+  else
+    return nullptr;
 }
+}  // namespace
 
-extern "C" void* lb_dev_solve_kernel(int id, int solver) {
-#define X(i, r, w, v)                                                                               \
-  if (id == i) return solver == 1 ? (void*)&lb_solve_kernel<r, 1, w, v> : (void*)&lb_solve_kernel<r, 0, w, v>;
-  LB_CONFIGS(X)
-#undef X
-  return nullptr;
-}
-
-extern "C" void* lb_dev_rhs_kernel(int id) {
+extern "C" void* LB_PART_FN(lb_dev_solve_kernel)(int id, int solver) {
 #define X(i, r, w, v) \
-  if (id == i) return (void*)&lb_rhs_kernel<r, w, v>;
+  if (id == i) return solve_ptr<i, r, w, v>(solver);
   LB_CONFIGS(X)
 #undef X
   return nullptr;
 }
 
+extern "C" void* LB_PART_FN(lb_dev_rhs_kernel)(int id) {
+#define X(i, r, w, v) \
+  if (id == i) return rhs_ptr<i, r, w, v>();
+  LB_CONFIGS(X)
+#undef X
+  return nullptr;
+}
+
+#if LB_PART == 0
 extern "C" void* lb_dev_queue_reset_kernel(void) { return (void*)&lb_queue_reset_kernel; }
+#endif

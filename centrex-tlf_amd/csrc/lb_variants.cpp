@@ -1,0 +1,42 @@
+// Host view of the variant table and of the kernels compiled in the parts of
+// lb_kernels.hip (one part per object file, LB_NPARTS_BUILT in total).
+#include "lb_variants.h"
+
+extern "C" void* lb_dev_solve_kernel_p0(int, int);
+extern "C" void* lb_dev_solve_kernel_p1(int, int);
+extern "C" void* lb_dev_solve_kernel_p2(int, int);
+extern "C" void* lb_dev_solve_kernel_p3(int, int);
+extern "C" void* lb_dev_rhs_kernel_p0(int);
+extern "C" void* lb_dev_rhs_kernel_p1(int);
+extern "C" void* lb_dev_rhs_kernel_p2(int);
+extern "C" void* lb_dev_rhs_kernel_p3(int);
+
+static_assert(LB_NPARTS_BUILT == 4, "lb_variants.cpp lists four kernel parts");
+
+extern "C" int lb_dev_variant_count(void) {
+  int n = 0;
+#define X(i, r, w, v) ++n;
+  LB_CONFIGS(X)
+#undef X
+  return n;
+}
+
+extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves) {
+#define X(i, r, w, v)                                                                           \
+  if (id == i) { *R = r; *maxw = w; *waves = v; return 1; }
+  LB_CONFIGS(X)
+#undef X
+  return 0;
+}
+
+extern "C" void* lb_dev_solve_kernel(int id, int solver) {
+  void* (*parts[])(int, int) = {lb_dev_solve_kernel_p0, lb_dev_solve_kernel_p1, lb_dev_solve_kernel_p2,
+                                lb_dev_solve_kernel_p3};
+  return id < 0 ? nullptr : parts[id % LB_NPARTS_BUILT](id, solver);
+}
+
+extern "C" void* lb_dev_rhs_kernel(int id) {
+  void* (*parts[])(int) = {lb_dev_rhs_kernel_p0, lb_dev_rhs_kernel_p1, lb_dev_rhs_kernel_p2,
+                           lb_dev_rhs_kernel_p3};
+  return id < 0 ? nullptr : parts[id % LB_NPARTS_BUILT](id);
+}

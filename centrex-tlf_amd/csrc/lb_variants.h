@@ -1,0 +1,10 @@
+// Compiled solver variants: X(id, slots per thread R, largest workgroup,
+// waves-per-SIMD hint).  The hint caps the registers per lane at 512 / hint;
+// the host picks a variant whose R * W covers the plan's slots.
+#pragma once
+
+#define LB_CONFIGS(X)                                                                            \
+  X(0, 1, 1024, 4) X(1, 2, 512, 4) X(2, 2, 512, 2) X(3, 3, 1024, 4) X(4, 4, 256, 4)                \
+  X(5, 4, 256, 2) X(6, 4, 256, 1) X(7, 6, 512, 2) X(8, 6, 512, 1) X(9, 8, 128, 1) X(10, 12, 256, 1)
+
+#define LB_NPARTS_BUILT 4

@@ -33,6 +33,8 @@ extern "C" {
 #define LB_TRAJ_UNDERFLOW 2 /* "Stopped at x=... Step size underflow."     dopri5.rs:229-231 */
 #define LB_TRAJ_STIFF 3     /* "Stiff at x=..."                             dopri5.rs:255-277 */
 #define LB_TRAJ_BAD_SPAN 4  /* t1 < t0 ("only forward integration")         dopri5.rs:175-177 */
+#define LB_TRAJ_BAD_SUPPORT 5 /* y0 is nonzero outside lb_solve_args.y0_support (no reference
+                                 counterpart: the mask is a GPU-side promise) */
 
 /* Solvers (rust/src/ode/batch.rs:7-27). */
 #define LB_SOLVER_DOPRI5 0
@@ -179,6 +181,12 @@ typedef struct lb_solve_args {
   int32_t variant;                /* kernel variant: 0 = plan default, k = variant k-1
                                      (slots per thread / occupancy; results are
                                      bitwise independent of it) */
+  const uint8_t* y0_support;      /* host, dim bytes, or NULL: nonzero where any
+                                     trajectory's y0 may be nonzero.  The solve then
+                                     runs only the elements reachable from that
+                                     support through the RHS program (all others
+                                     stay exactly zero) and reports
+                                     LB_TRAJ_BAD_SUPPORT for a y0 outside it. */
 } lb_solve_args;
 
 /*

@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from centrex_tlf_amd.lindblad import prepare_lindblad_problem, two_level_system
+from centrex_tlf_amd.lindblad.native import kernel_variants
 from oracle.lb_oracle import Oracle
 
 from ._dense import CHAIN_PARAMETERS, chain_system, random_density
@@ -113,9 +114,11 @@ def test_two_level_grid_step_totals_and_scan(golden_dir):
     np.testing.assert_allclose(values.cpu().numpy()[:, 0], want, rtol=0, atol=1e-11)
 
 
+@pytest.mark.parametrize("restrict", [True, False], ids=["restricted", "full"])
 @pytest.mark.parametrize("solver", ["dopri5", "tsit5"])
-def test_tlf_n39_grid_subset_matches_oracle(solver):
-    """Seeded 16-point subset of the 64x64 (delta, Omega0) grid of the n=39 workload."""
+def test_tlf_n39_grid_subset_matches_oracle(solver, restrict):
+    """Seeded 16-point subset of the 64x64 (delta, Omega0) grid of the n=39 workload,
+    on the plan restricted to the reachable elements and on the full plan."""
     require_gpu()
     system, params, rho0, t_span, weights = tlf_n39()
     prepared = prepare_lindblad_problem(system, params, backend="host")
@@ -135,9 +138,9 @@ def test_tlf_n39_grid_subset_matches_oracle(solver):
                                  integral_weights=weights, parameter_slots=["delta0", "Omega0"],
                                  parameter_batch=pb, **kw)
     pops, stats, _ = plan.solve(y0, *t_span, solver=solver, output_kind="populations",
-                                override_slots=slots, override_values=pb, **kw)
+                                override_slots=slots, override_values=pb, restrict_support=restrict, **kw)
     ints, stats2, _ = plan.solve(y0, *t_span, solver=solver, output_kind="integral", weights=weights,
-                                 override_slots=slots, override_values=pb, **kw)
+                                 override_slots=slots, override_values=pb, restrict_support=restrict, **kw)
     pops, stats, ints = pops.cpu().numpy(), stats.cpu().numpy(), ints.cpu().numpy()
     assert np.all(stats[:, 3] == 0)
     np.testing.assert_array_equal(stats[:, 0], ref_pop["accepted"])
@@ -155,11 +158,25 @@ def test_workgroup_size_and_plain_mode_are_bitwise_identical():
     pb = np.asarray([[0.3 * GAMMA, 2.0 * GAMMA], [-1.0 * GAMMA, 0.5 * GAMMA], [2.0, 3.0 * GAMMA]], dtype=complex)
     y0 = prepared.layout.pack(rho0)
     kw = dict(abstol=1e-10, reltol=1e-8, dt=1e-8, output_kind="full", override_slots=slots, override_values=pb)
+    def same(a, b, what):
+        assert torch.equal(a, b), (what, float((a - b).abs().max()))
+
     base, st0, _ = plan.solve(y0, *t_span, **kw)
     for threads in (256, 512, 1024):
         other, st, _ = plan.solve(y0, *t_span, threads_per_traj=threads, **kw)
-        assert torch.equal(base, other), threads
-        assert torch.equal(st0, st)
+        same(base, other, threads)
+        same(st0, st, threads)
+    # every compiled kernel variant that covers the plan gives the same bits
+    full, stf, _ = plan.solve(y0, *t_span, restrict_support=False, **kw)
+    n_checked = 0
+    for v in kernel_variants():
+        if v["slots_per_thread"] * v["max_threads"] < plan.info["slot_count"]:
+            continue
+        other, st, _ = plan.solve(y0, *t_span, restrict_support=False, variant=v["variant"], **kw)
+        same(full, other, v)
+        same(stf, st, v)
+        n_checked += 1
+    assert n_checked >= 3
     plain, st, _ = plan.solve(y0, *t_span, mode="plain", **kw)
     assert torch.equal(base, plain)
 
@@ -171,3 +188,39 @@ def test_failure_status_maxiters():
     _, stats, fail_t = plan.solve(y0, 0.0, 0.5, abstol=1e-10, reltol=1e-8, dt=1e-3, maxiters=1,
                                   output_kind="integral", weights=[(1, 0.3)])
     assert int(stats.cpu().numpy()[0, 3]) == 1
+
+
+def test_support_violation_is_reported():
+    """A y0 that is nonzero outside the declared support is refused per trajectory."""
+    require_gpu()
+    system, params, rho0, t_span, weights = tlf_n39()
+    prepared = prepare_lindblad_problem(system, params, backend="host")
+    plan = _device_plan(prepared)
+    y0 = prepared.layout.pack(rho0)
+    # declare only the population of an undriven X(J=1) state, whose closure is
+    # itself, while y0 populates X(J=0)
+    spectator = system.indices(J=1, electronic="X")[0]
+    assert y0[spectator] == 0.0
+    support = np.zeros(y0.size, dtype=bool)
+    support[spectator] = True
+    pops, stats, _ = plan.solve(y0, *t_span, abstol=1e-10, reltol=1e-8, dt=1e-8, n_traj=2, support=support)
+    stats = stats.cpu().numpy()
+    assert np.all(stats[:, 3] == 5)
+    assert np.all(np.isnan(pops.cpu().numpy()))
+
+
+def test_restricted_plan_zero_fills_dropped_elements():
+    """Full-state output of a restricted solve: dropped elements are exactly zero and
+    the kept ones agree with the full plan."""
+    require_gpu()
+    system, params, rho0, t_span, weights = tlf_n39()
+    prepared = prepare_lindblad_problem(system, params, backend="host")
+    plan = _device_plan(prepared)
+    y0 = prepared.layout.pack(rho0)
+    kw = dict(abstol=1e-10, reltol=1e-8, dt=1e-8, output_kind="full", n_traj=1)
+    res, st_r, _ = plan.solve(y0, *t_span, **kw)
+    full, st_f, _ = plan.solve(y0, *t_span, restrict_support=False, **kw)
+    res, full = res.cpu().numpy()[0], full.cpu().numpy()[0]
+    zero_in_full = full == 0.0
+    assert np.all(res[zero_in_full] == 0.0)
+    np.testing.assert_allclose(res, full, rtol=0, atol=1e-9)
