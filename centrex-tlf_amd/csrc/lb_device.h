@@ -80,7 +80,7 @@ struct DevPlan {
   const DevInstr* instr;
   const int32_t* expr_start;    // n_compounds + K + 1
   const int32_t* compound_slot; // n_compounds
-  const uint8_t* expr_dyn;      // n_compounds + K: 1 if time-dependent
+  const int32_t* expr_dyn;      // n_compounds + K: 1 if time-dependent
   int32_t vm_depth;             // deepest expression stack
   const double* base;           // n_base*2
 };

@@ -23,6 +23,7 @@
 
 extern "C" int lb_dev_variant_count(void);
 extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
+extern "C" int lb_dev_variant_rows(int id);
 extern "C" void* lb_dev_solve_kernel(int id, int solver);
 extern "C" void* lb_dev_rhs_kernel(int id);
 extern "C" void* lb_dev_queue_reset_kernel(void);
@@ -54,7 +55,7 @@ struct PlanSource {
   std::vector<double> factors_re, factors_im, base;
   std::vector<int32_t> up_re, up_im;  // upper index -> packed indices (-1 imag on the diagonal)
   LbProgram prog;
-  std::vector<uint8_t> expr_dyn;
+  std::vector<int32_t> expr_dyn;
   std::vector<int32_t> compound_slot;
   bool time_dependent = false;
   int n_dyn_coef = 0;
@@ -145,10 +146,12 @@ int check_expression(const lb_plan_desc* d, int start, int end, int n_slots, std
   return 0;
 }
 
-// Smallest workgroup (multiple of 64) with which variant `id` covers S slots, or 0.
-int variant_threads(int id, int S) {
+// Smallest workgroup (multiple of 64) with which variant `id` covers S slots
+// and `rows` gather rows, or 0.
+int variant_threads(int id, int S, int rows) {
   int R, maxw, waves;
   if (!lb_dev_variant(id, &R, &maxw, &waves)) return 0;
+  if (const int g = lb_dev_variant_rows(id)) return (S <= 64 && rows <= g) ? 64 : 0;
   int W = ((S + R - 1) / R + 63) / 64 * 64;
   if (W < 64) W = 64;
   return W <= maxw ? W : 0;
@@ -157,16 +160,19 @@ int variant_threads(int id, int S) {
 // Default variant: the most registers per lane that still keeps two
 // workgroups' worth of waves per SIMD pair, then the least padding.  The
 // autotuner (centrex_tlf_amd.lindblad.autotune) replaces this by measurement.
-void pick_config(int S, int* id_out, int* W_out) {
+void pick_config(int S, int rows, int* id_out, int* W_out) {
   int best = -1, bestW = 0;
   int64_t bestScore = INT64_MAX;
   for (int id = 0; id < lb_dev_variant_count(); ++id) {
     int R, maxw, waves;
     lb_dev_variant(id, &R, &maxw, &waves);
-    int W = variant_threads(id, S);
+    int W = variant_threads(id, S, rows);
     if (!W) continue;
     const int64_t waste = (int64_t)R * W - S;
-    const int64_t score = waste + (waves == 2 ? 0 : 256) + 8 * R;
+    const int g = lb_dev_variant_rows(id);
+    // single-wave variants first (fewest registers that hold the rows)
+    const int64_t score = g ? -1000 + g + (waves == 4 ? 0 : 1)
+                            : waste + (waves == 2 ? 0 : 256) + 8 * R;
     if (score < bestScore) { bestScore = score; best = id; bestW = W; }
   }
   *id_out = best;
@@ -181,7 +187,7 @@ int64_t lds_bytes_for(const DevPlan& P) {
   const int64_t cells = P.lds_tables ? (int64_t)P.g_rows * 64 : 0;
   return 8 * (int64_t)(4 * P.n_pub) + 8 * 2 * (int64_t)P.S + 8 * (int64_t)((P.n + 1) & ~1) +
          16 * 8 * (int64_t)(P.K + 1) + 16 * 8 * (int64_t)P.n_slots + 8 * 4 * (int64_t)P.nblk +
-         16 * 8 * (int64_t)P.vm_depth + 16 * cells + 4 * ((cells + 3) & ~3) + 16;
+         16 * 8 * (int64_t)P.vm_depth + 16 * 128 + 16 * cells + 4 * ((cells + 3) & ~3) + 16;
 }
 
 int set_lds_limit(const void* fn, size_t lds) {
@@ -248,12 +254,17 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   std::vector<int32_t> slot_up(S, -1), slot_re(S, -1), slot_im(S, -1), slot_pub(S, -1);
   std::vector<int32_t> pub_of_upper(U, -1);
   int n_pub = 0;
+  // A plan of one block publishes every element at its slot index, so the
+  // gather words name the owning lane (the single-wave kernels read inputs
+  // by cross-lane permutes).
+  const bool one_block = S == 64;
   for (int s = 0; s < n_act; ++s) {
     const int o = order[s];
     slot_up[s] = o;
     slot_re[s] = src.up_re[o];
     slot_im[s] = src.up_im[o];
-    if (published[o]) { pub_of_upper[o] = n_pub; slot_pub[s] = n_pub++; }
+    if (one_block) { pub_of_upper[o] = s; slot_pub[s] = s; n_pub = s + 1; }
+    else if (published[o]) { pub_of_upper[o] = n_pub; slot_pub[s] = n_pub++; }
   }
   if (n_pub > (1 << 20)) return fail(LB_ERR_UNSUPPORTED, "too many published elements");
   int max_self = 1;
@@ -344,7 +355,7 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   up(act, &P.upper_active);
   up(packed_upper, &P.packed_upper);
   if (rc != LB_OK) return rc;
-  pick_config(S, &im->default_variant, &im->default_W);
+  pick_config(S, rows_total, &im->default_variant, &im->default_W);
   im->lds_bytes = lds_bytes_for(P);
   im->n_pub = n_pub;
   im->rows_total = rows_total;
@@ -448,7 +459,7 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
     if (lb_build_program(d, dyn_slot, compound_dyn, coef_dyn, &src.prog, why) != 0)
       return fail(LB_ERR_INVALID, why);
   }
-  src.expr_dyn = src.prog.compound_dyn;
+  src.expr_dyn.assign(src.prog.compound_dyn.begin(), src.prog.compound_dyn.end());
   src.expr_dyn.insert(src.expr_dyn.end(), src.prog.coef_dyn.begin(), src.prog.coef_dyn.end());
   if (src.expr_dyn.empty()) src.expr_dyn.push_back(0);
   src.compound_slot = src.prog.compound_slot;
@@ -502,7 +513,9 @@ int resolve_config(const DevImage* p, int variant, int threads, int* id, int* W)
   if (variant > 0) {
     *id = variant - 1;
     if (!lb_dev_variant(*id, &R, &maxw, &waves)) return fail(LB_ERR_INVALID, "unknown kernel variant");
-    *W = threads ? threads : variant_threads(*id, p->dev.S);
+    *W = threads ? threads : variant_threads(*id, p->dev.S, p->dev.g_rows);
+    if (lb_dev_variant_rows(*id) && (*W != 64 || !variant_threads(*id, p->dev.S, p->dev.g_rows)))
+      return fail(LB_ERR_UNSUPPORTED, "single-wave variant needs a plan of at most 64 elements");
     if (!*W || *W > maxw || (int64_t)R * *W < p->dev.S)
       return fail(LB_ERR_UNSUPPORTED, "kernel variant does not cover the plan with this workgroup size");
     return LB_OK;
@@ -510,6 +523,7 @@ int resolve_config(const DevImage* p, int variant, int threads, int* id, int* W)
   // threads given: the variant with the fewest slots per thread that covers the plan
   for (int v = 0; v < lb_dev_variant_count(); ++v) {
     lb_dev_variant(v, &R, &maxw, &waves);
+    if (lb_dev_variant_rows(v)) continue;
     if (threads <= maxw && (int64_t)R * threads >= p->dev.S) {
       *id = v;
       *W = threads;

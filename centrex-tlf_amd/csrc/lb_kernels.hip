@@ -24,6 +24,14 @@ struct dcpx {
   double re, im;
 };
 
+// Read-only plan tables indexed by wave-uniform values are read through the
+// constant address space, so the loads go through the scalar cache instead
+// of a vector load plus readfirstlane.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* cst(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
+}
+
 __device__ __forceinline__ dcpx mk(double r, double i) { return dcpx{r, i}; }
 __device__ __forceinline__ dcpx cadd(dcpx a, dcpx b) { return mk(a.re + b.re, a.im + b.im); }
 __device__ __forceinline__ dcpx csub(dcpx a, dcpx b) { return mk(a.re - b.re, a.im - b.im); }
@@ -168,12 +176,13 @@ __device__ __forceinline__ dcpx dev_powc(dcpx a, double c, double cs, double sn)
 // builder has checked ops, helper ids and the stack depth.  The top of the
 // stack lives in registers and the entries below it in this lane's LDS
 // window `stk` (nothing goes to scratch memory).
-__device__ __forceinline__ dcpx vm_eval(const DevInstr* ins, int start, int end, const dcpx* slots,
+__device__ __forceinline__ dcpx vm_eval(const __attribute__((address_space(4))) DevInstr* ins, int start,
+                                        int end, const dcpx* slots,
                                         double t, dcpx* stk) {
   dcpx top = mk(0.0, 0.0);
   int d = 0;  // entries on the stack; entry d-1 is `top`
   for (int p = start; p < end; ++p) {
-    const DevInstr q = ins[p];
+    const auto& q = ins[p];
     const int op = q.op;
     if (op <= 4) {
       if (d > 0) stk[d - 1] = top;
@@ -316,6 +325,7 @@ struct Lds {
   dcpx* slots;    // 8 x n_slots
   double* red;    // 4 x nblk block sums
   dcpx* vm;       // 8 lanes x vm_depth expression stacks
+  double2* park;  // 128 entries: single-wave state parked around the expression machine
   double2* gf;    // gather factors (when the plan stages its tables in LDS)
   uint32_t* gw;   // gather words
   int* misc;      // [0] queue value
@@ -331,16 +341,18 @@ __device__ __forceinline__ Lds carve(const DevPlan& P) {
   L.slots = L.coef + 8 * (P.K + 1);
   L.red = reinterpret_cast<double*>(L.slots + 8 * P.n_slots);
   L.vm = reinterpret_cast<dcpx*>(L.red + 4 * P.nblk);
-  L.gf = reinterpret_cast<double2*>(L.vm + 8 * P.vm_depth);
+  L.park = reinterpret_cast<double2*>(L.vm + 8 * P.vm_depth);
+  L.gf = L.park + 128;
   const size_t cells = P.lds_tables ? (size_t)P.g_rows * 64 : 0;
   L.gw = reinterpret_cast<uint32_t*>(L.gf + cells);
   L.misc = reinterpret_cast<int*>(L.gw + ((cells + 3) & ~(size_t)3));
   return L;
 }
 
-// One gather term: acc += f * c * x_in (rhs.rs:815-876 term semantics).
-__device__ __forceinline__ void gather_term(uint32_t w, double2 f, const dcpx* coef, const double* pubbuf,
-                                            double& accr, double& acci) {
+// One gather term: acc += f * c * x_in (rhs.rs:815-876 term semantics), with
+// the input element (x_r, x_raw_i) already fetched.
+__device__ __forceinline__ void gather_apply(uint32_t w, double2 f, const dcpx* coef, double x_r,
+                                             double x_raw_i, double& accr, double& acci) {
   const uint32_t cid = w >> 24;
   double vr = f.x, vi = f.y;
   if (cid != LB_STATIC_COEF) {
@@ -349,14 +361,55 @@ __device__ __forceinline__ void gather_term(uint32_t w, double2 f, const dcpx* c
     vr = fma(f.x, c.re, -(f.y * c.im));
     vi = fma(f.x, c.im, f.y * c.re);
   }
-  const uint32_t p = w & 0xFFFFFu;
-  const double x_r = pubbuf[2 * p];
-  double x_i = (w & (1u << 20)) ? 0.0 : pubbuf[2 * p + 1];
+  double x_i = (w & (1u << 20)) ? 0.0 : x_raw_i;
   if (w & (1u << 21)) x_i = -x_i;
   // explicit fused operations: the library is built without implicit
   // contraction, so every kernel variant rounds identically
   accr = fma(-vi, x_i, fma(vr, x_r, accr));
   acci = fma(vi, x_r, fma(vr, x_i, acci));
+}
+
+// The same term with its input read from the LDS publication buffer.
+__device__ __forceinline__ void gather_term(uint32_t w, double2 f, const dcpx* coef, const double* pubbuf,
+                                            double& accr, double& acci) {
+  const uint32_t p = w & 0xFFFFFu;
+  gather_apply(w, f, coef, pubbuf[2 * p], pubbuf[2 * p + 1], accr, acci);
+}
+
+// Single-wave form for plans of at most 64 elements: lane = slot, the gather
+// inputs come from the owning lanes by cross-lane permutes instead of a
+// publish / barrier / LDS-read round trip.  The publication index of a plan
+// this small is its slot, so the table words are shared with the multi-wave
+// form, and the arithmetic is the same.  All rows' table entries and inputs
+// are fetched before the first term is applied.
+template <int G>
+__device__ __forceinline__ void rhs_wave(const Lds& L, int rows, int lane, double2 selfv, bool diag, double xr,
+                                         double xi, double& kr, double& ki, const dcpx* coef) {
+  double accr = fma(selfv.x, xr, -(selfv.y * xi));
+  double acci = fma(selfv.x, xi, selfv.y * xr);
+  uint32_t w[G];
+  double2 f[G];
+  double sr[G], si[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    if (j < rows) {
+      w[j] = L.gw[j * 64 + lane];
+      f[j] = L.gf[j * 64 + lane];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    if (j < rows) {
+      const int src = (int)(w[j] & 63u);
+      sr[j] = __shfl(xr, src, 64);
+      si[j] = __shfl(xi, src, 64);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < G; ++j)
+    if (j < rows) gather_apply(w[j], f[j], coef, sr[j], si[j], accr, acci);
+  kr = accr;
+  ki = diag ? 0.0 : acci;
 }
 
 // The gather rows of one block, in row order.  The tables are read through
@@ -409,7 +462,7 @@ __device__ __forceinline__ void rhs_eval(const DevPlan& P, const Lds& L, const S
     const double2 a = reinterpret_cast<const double2*>(L.selfc)[s];
     double accr = fma(a.x, xr[r], -(a.y * xi[r]));
     double acci = fma(a.x, xi[r], a.y * xr[r]);
-    gather_block(gw, gf, P.blk_row0[blk], P.blk_rows[blk], lane, coef, pubbuf, accr, acci);
+    gather_block(gw, gf, cst(P.blk_row0)[blk], cst(P.blk_rows)[blk], lane, coef, pubbuf, accr, acci);
     kr[r] = accr;
     ki[r] = S.diag(r) ? 0.0 : acci;
   }
@@ -475,9 +528,9 @@ __device__ __forceinline__ void eval_coefficients(const DevPlan& P, dcpx* sl, dc
                                                   bool only_dynamic, dcpx* stk) {
   const int n_expr = P.n_compounds + P.K;
   for (int e = 0; e < n_expr; ++e) {
-    if (only_dynamic && !P.expr_dyn[e]) continue;
-    const dcpx v = vm_eval(P.instr, P.expr_start[e], P.expr_start[e + 1], sl, t, stk);
-    if (e < P.n_compounds) sl[P.compound_slot[e]] = v;
+    if (only_dynamic && !cst(P.expr_dyn)[e]) continue;
+    const dcpx v = vm_eval(cst(P.instr), cst(P.expr_start)[e], cst(P.expr_start)[e + 1], sl, t, stk);
+    if (e < P.n_compounds) sl[cst(P.compound_slot)[e]] = v;
     else cf[e - P.n_compounds] = v;
   }
   cf[P.K] = mk(1.0, 0.0);
@@ -549,8 +602,11 @@ __device__ __forceinline__ void fold_self(const DevPlan& P, int tid, int W, cons
 // ---------------------------------------------------------------------------
 // Persistent integrator.
 // ---------------------------------------------------------------------------
-template <int R, int SOLVER, int MAXW, int WAVES>
+// G > 0 selects the single-wave form (R = 1, 64 threads, G gather rows in
+// registers); G = 0 is the general multi-wave form.
+template <int R, int SOLVER, int MAXW, int WAVES, int G>
 __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, const DevCall C) {
+  static_assert(G == 0 || (R == 1 && MAXW == 64), "the single-wave form holds one slot per lane");
   using T = Tab<SOLVER>;
   const int tid = threadIdx.x, W = blockDim.x;
   const Lds L = carve(P);
@@ -558,6 +614,8 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
   Slots<R> S;
   load_slots<R>(P, S, tid, W);
   stage_tables(P, L, tid, W);
+  const int wrows = G > 0 ? cst(P.blk_rows)[0] : 0;
+  double2 selfv = double2{0.0, 0.0};
 
   for (;;) {
     __syncthreads();
@@ -585,6 +643,10 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     for (int i = tid; i < 7 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
     for (int i = tid; i < 7 * KK; i += W) L.coef[KK + i] = L.coef[i % KK];
     fold_self(P, tid, W, L.coef, L.selfc);
+    if constexpr (G > 0) {
+      __syncthreads();  // coefficient rows and self folds are read below
+      selfv = reinterpret_cast<const double2*>(L.selfc)[tid];
+    }
 
     // ---- state
     double y_r[R], y_i[R];
@@ -612,6 +674,15 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     double k_r[8][R], k_i[8][R];
     int buf = 0;
 #define PUB (L.pub + buf * 2 * P.n_pub)
+#define LB_RHS(xr, xi, kr, ki, cf)                                                         \
+  do {                                                                                     \
+    if constexpr (G > 0) {                                                                 \
+      rhs_wave<(G > 0 ? G : 1)>(L, wrows, tid, selfv, S.diag(0), xr[0], xi[0], kr[0], ki[0], cf); \
+    } else {                                                                               \
+      rhs_eval<R>(P, L, S, xr, xi, kr, ki, PUB, cf, tid, W);                               \
+      buf ^= 1;                                                                            \
+    }                                                                                      \
+  } while (0)
 
     if (bad_support) {
       status = LB_TRAJ_BAD_SUPPORT;
@@ -622,8 +693,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     } else if (t1 > t0) {
       double x = t0, h;
       // ---- k1 = f(t0, y0) (also f0 of hinit); the barrier inside orders the LDS fills above
-      rhs_eval<R>(P, L, S, y_r, y_i, k_r[1], k_i[1], PUB, L.coef, tid, W);
-      buf ^= 1;
+      LB_RHS(y_r, y_i, k_r[1], k_i[1], L.coef);
       if (isfinite(C.dt) && C.dt > 0.0) {
         h = fmin(C.dt, t1 - t0);
       } else {
@@ -641,14 +711,14 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         double h0 = (d0 < 1.0e-10 || d1 < 1.0e-10) ? 1.0e-6 : 0.01 * sqrt(d0 / d1);
         h0 = fmin(h0, hmax);
         if (tid == 0) eval_coefficients(P, L.slots + P.n_slots, L.coef + KK, x + h0, true, L.vm);
+        if constexpr (G > 0) __syncthreads();
         double yt_r[R], yt_i[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           yt_r[r] = y_r[r] + k_r[1][r] * h0;
           yt_i[r] = y_i[r] + k_i[1][r] * h0;
         }
-        rhs_eval<R>(P, L, S, yt_r, yt_i, k_r[2], k_i[2], PUB, L.coef + KK, tid, W);
-        buf ^= 1;
+        LB_RHS(yt_r, yt_i, k_r[2], k_i[2], L.coef + KK);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const double sr = C.abstol + fabs(y_r[r]) * C.reltol, si = C.abstol + fabs(y_i[r]) * C.reltol;
@@ -683,12 +753,35 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         if (0.1 * fabs(h) <= DBL_EPSILON * fabs(x)) { status = LB_TRAJ_UNDERFLOW; fail_x = x; break; }
         if (x + 1.01 * h - t1 > 0.0) { h = t1 - x; last = true; }
         ++ns;
+        // The expression machine needs registers of its own: the single-wave
+        // form parks its state in LDS meanwhile (the multi-wave form keeps
+        // it in the other waves' registers).
+        if constexpr (G > 0) {
+          L.park[tid] = double2{y_r[0], y_i[0]};
+          L.park[64 + tid] = double2{k_r[1][0], k_i[1][0]};
+        }
         // time-dependent coefficients at the six stage times, one lane per stage
         if (P.time_dependent && tid < 6) {
           const int s = tid + 2;
           eval_coefficients(P, L.slots + s * P.n_slots, L.coef + s * KK, x + h * T::c(s), true,
                             L.vm + tid * P.vm_depth);
         }
+        if constexpr (G > 0) {
+          __syncthreads();  // stage coefficients are read by every lane
+          const double2 py = L.park[tid], pk = L.park[64 + tid];
+          y_r[0] = py.x;
+          y_i[0] = py.y;
+          k_r[1][0] = pk.x;
+          k_i[1][0] = pk.y;
+        }
+        // the stage vectors of the previous step are dead from here on
+#pragma unroll
+        for (int j = 2; j <= 7; ++j)
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            k_r[j][r] = 0.0;
+            k_i[j][r] = 0.0;
+          }
         const bool need_stiff = ((accepted + 1) % 1000 == 0) || ia > 0;
         double yt_r[R], yt_i[R], yn_r[R], yn_i[R];
         // Stages 2..7 share one RHS site (a rolled loop keeps the step's code
@@ -720,8 +813,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
               break;
           }
           double ko_r[R], ko_i[R];
-          rhs_eval<R>(P, L, S, yt_r, yt_i, ko_r, ko_i, PUB, L.coef + s * KK, tid, W);
-          buf ^= 1;
+          LB_RHS(yt_r, yt_i, ko_r, ko_i, L.coef + s * KK);
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             switch (s) {
@@ -808,6 +900,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         h = hn;
       }
     }
+#undef LB_RHS
 #undef PUB
 
     // ---- outputs
@@ -925,7 +1018,21 @@ namespace {
 template <int I, int RR, int WW, int VV>
 void* solve_ptr(int solver) {
   if constexpr (I % LB_NPART == LB_PART)
-    return solver == 1 ? (void*)&lb_solve_kernel<RR, 1, WW, VV> : (void*)&lb_solve_kernel<RR, 0, WW, VV>;
+    return solver == 1 ? (void*)&lb_solve_kernel<RR, 1, WW, VV, 0> : (void*)&lb_solve_kernel<RR, 0, WW, VV, 0>;
+  else
+    return nullptr;
+}
+template <int I, int GG, int VV>
+void* wave_solve_ptr(int solver) {
+  if constexpr (I % LB_NPART == LB_PART)
+    return solver == 1 ? (void*)&lb_solve_kernel<1, 1, 64, VV, GG> : (void*)&lb_solve_kernel<1, 0, 64, VV, GG>;
+  else
+    return nullptr;
+}
+template <int I, int VV>
+void* wave_rhs_ptr() {
+  if constexpr (I % LB_NPART == LB_PART)
+    return (void*)&lb_rhs_kernel<1, 64, VV>;
   else
     return nullptr;
 }
@@ -943,6 +1050,10 @@ extern "C" void* LB_PART_FN(lb_dev_solve_kernel)(int id, int solver) {
   if (id == i) return solve_ptr<i, r, w, v>(solver);
   LB_CONFIGS(X)
 #undef X
+#define Y(i, g, v) \
+  if (id == i) return wave_solve_ptr<i, g, v>(solver);
+  LB_WAVE_CONFIGS(Y)
+#undef Y
   return nullptr;
 }
 
@@ -951,6 +1062,10 @@ extern "C" void* LB_PART_FN(lb_dev_rhs_kernel)(int id) {
   if (id == i) return rhs_ptr<i, r, w, v>();
   LB_CONFIGS(X)
 #undef X
+#define Y(i, g, v) \
+  if (id == i) return wave_rhs_ptr<i, v>();
+  LB_WAVE_CONFIGS(Y)
+#undef Y
   return nullptr;
 }
 

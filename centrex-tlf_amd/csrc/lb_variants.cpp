@@ -18,6 +18,9 @@ extern "C" int lb_dev_variant_count(void) {
 #define X(i, r, w, v) ++n;
   LB_CONFIGS(X)
 #undef X
+#define Y(i, g, v) ++n;
+  LB_WAVE_CONFIGS(Y)
+#undef Y
   return n;
 }
 
@@ -26,6 +29,19 @@ extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves) {
   if (id == i) { *R = r; *maxw = w; *waves = v; return 1; }
   LB_CONFIGS(X)
 #undef X
+#define Y(i, g, v)                                                                              \
+  if (id == i) { *R = 1; *maxw = 64; *waves = v; return 1; }
+  LB_WAVE_CONFIGS(Y)
+#undef Y
+  return 0;
+}
+
+// Gather rows a single-wave variant holds in registers (0: multi-wave variant).
+extern "C" int lb_dev_variant_rows(int id) {
+#define Y(i, g, v) \
+  if (id == i) return g;
+  LB_WAVE_CONFIGS(Y)
+#undef Y
   return 0;
 }
 

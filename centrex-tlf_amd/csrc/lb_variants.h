@@ -7,4 +7,8 @@
   X(0, 1, 1024, 4) X(1, 2, 512, 4) X(2, 2, 512, 2) X(3, 3, 1024, 4) X(4, 4, 256, 4)                \
   X(5, 4, 256, 2) X(6, 4, 256, 1) X(7, 6, 512, 2) X(8, 6, 512, 1) X(9, 8, 128, 1) X(10, 12, 256, 1)
 
+// Single-wave variants for plans of at most 64 elements: Y(id, gather rows
+// held in registers, waves-per-SIMD hint); one slot per lane, 64 threads.
+#define LB_WAVE_CONFIGS(Y) Y(11, 8, 4) Y(12, 8, 2) Y(13, 16, 2)
+
 #define LB_NPARTS_BUILT 4
