@@ -11,12 +11,20 @@ photon count (trapezoid over accepted steps) from one fused launch.  The
 initial state, the plan tables and the scan axes are resident on the GPU
 before the timed region.
 
+The solver integrates only the density-matrix elements reachable from the
+initial state through the RHS program (23 of the 780 upper-triangle elements
+here; all others stay exactly zero), in single-wave workgroups.
+
 Also reported: the per-trajectory ("plain") GPU path on a seeded sample, the
-CPU oracle (this project's C restatement of the reference's Rust/rayon path)
-timed on the host cores on a bounded seeded sample, a parity check of the
-sample against the oracle, and the FP64 roofline of the solver kernel.
+CPU oracle (this project's C restatement of the reference's Rust/rayon path,
+which integrates every element) timed on the host cores on a bounded seeded
+sample, a parity check of the sample against the oracle, and the FP64
+roofline of the solver kernel: `achieved` counts the reference algorithm's
+FLOPs (rhs_calls * F_rhs + attempted * dim * F_rk over the full plan), and
+`executed_*` the FLOPs of the restricted tables the kernel actually runs.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --copy-calibration   # plain 1 GiB copy kernel (HBM counter scale)
 """
 
 from __future__ import annotations
@@ -67,6 +75,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--plain-sample", type=int, default=128)
     ap.add_argument("--profile", action="store_true", help="GPU steps only (for rocprofv3)")
+    ap.add_argument("--full-plan", action="store_true", help="integrate every element (no restriction)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 1 GiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards "
@@ -84,6 +95,15 @@ def main() -> int:
         print(json.dumps({"error": "no HIP device"}))
         return 1
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
+                          "launches": 5}))
+        return 0
     system, params, rho0, t_span, weights = tlf_n39()
     prepared = prepare_lindblad_problem(system, params, backend="gpu")
     plan = prepared.device_plan()
@@ -96,14 +116,18 @@ def main() -> int:
     y0 = torch.as_tensor(prepared.layout.pack(rho0), device=dev)
     tol = dict(abstol=1e-10, reltol=1e-8, dt=1e-8)
     n_traj = axes.count
+    support = plan.support_of(prepared.layout.pack(rho0))
+    run_kw = dict(restrict_support=False) if args.full_plan else dict(support=support)
+    rinfo = info if args.full_plan else plan.support_info(support)
 
     def step(**kw):
         return plan.solve(y0, t_span[0], t_span[1], solver=args.solver, output_kind="populations",
                           weights=weights, with_integral=True, override_slots=slots, grid_axes=axes,
                           threads_per_traj=args.threads_per_traj, workgroups=args.workgroups,
-                          variant=args.variant, **tol, **kw)
+                          variant=args.variant, **run_kw, **tol, **kw)
 
     log(f"bench: plan {info}")
+    log(f"bench: solved tables {rinfo}")
     for i in range(args.warmup):
         out = step()
         torch.cuda.synchronize()
@@ -113,9 +137,7 @@ def main() -> int:
     torch.cuda.synchronize()
     t_begin = time.perf_counter()
     for i in range(args.steps):
-        starts[i].record()
-        out = step()
-        ends[i].record()
+        out = step(events=(starts[i], ends[i]))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_begin
     kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
@@ -127,6 +149,7 @@ def main() -> int:
         return 1
     log(f"bench: {args.steps} steps in {elapsed:.3f} s")
     flops = algorithmic_flops(stats_np, info["flops_per_rhs"], info["dim"], args.solver)
+    executed = algorithmic_flops(stats_np, rinfo["flops_per_rhs"], rinfo["dim"], args.solver)
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
     achieved_tflops = flops / avg_kernel_s / 1e12
     traffic = None
@@ -154,7 +177,9 @@ def main() -> int:
             "atol": 1e-10,
             "outputs": "final populations + photon count (one launch)",
             "parallelism": "single_gpu",
-            "threads_per_traj": args.threads_per_traj or info["threads_per_traj"],
+            "integrated_elements": f"{rinfo['upper_len']} of {info['upper_len']} upper-triangle elements",
+            "kernel_variant": args.variant or rinfo["kernel_variant"],
+            "threads_per_traj": args.threads_per_traj or rinfo["threads_per_traj"],
         },
         "roofline": {
             "bound": "mfma",
@@ -166,6 +191,11 @@ def main() -> int:
             "traffic": traffic,
             "kernel_ms": float(np.mean(kernel_ms)),
             "flops_per_launch": flops,
+            "flops_definition": "sum over trajectories of rhs_calls*F_rhs + attempted*dim*F_rk, "
+                                f"F_rhs={info['flops_per_rhs']} (full plan), dim={info['dim']}, "
+                                f"F_rk={F_RK[args.solver]}",
+            "executed_flops_per_launch": executed,
+            "executed_frac": executed / avg_kernel_s / 1e12 / FP64_PEAK_TFLOPS,
         },
         "totals": {
             "accepted_steps": int(stats_np[:, 0].sum()),
@@ -184,12 +214,13 @@ def main() -> int:
     pb = np.stack([deltas[sample // 64], omegas[sample % 64]], axis=1).astype(np.complex128)
     pb_dev = torch.as_tensor(pb, device=dev)
     plan.solve(y0, *t_span, solver=args.solver, output_kind="populations", weights=weights,
-               with_integral=True, override_slots=slots, override_values=pb_dev[:2], mode="plain", **tol)
+               with_integral=True, override_slots=slots, override_values=pb_dev[:2], mode="plain",
+               **run_kw, **tol)
     torch.cuda.synchronize()
     t_p = time.perf_counter()
     p_pops, p_stats, _, p_phot = plan.solve(y0, *t_span, solver=args.solver, output_kind="populations",
                                             weights=weights, with_integral=True, override_slots=slots,
-                                            override_values=pb_dev, mode="plain", **tol)
+                                            override_values=pb_dev, mode="plain", **run_kw, **tol)
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t_p
     plain_rate = len(sample) / plain_s

@@ -57,6 +57,8 @@ def library() -> C.CDLL:
     L.lb_last_error.restype = C.c_char_p
     L.lb_plan_info.restype = C.c_int
     L.lb_plan_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.lb_plan_support_info.restype = C.c_int
+    L.lb_plan_support_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.POINTER(C.c_int64)]
     L.lb_rhs_eval.restype = C.c_int
     L.lb_rhs_eval.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32,
                               C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]
@@ -139,6 +141,21 @@ class DevicePlan:
         self.n = self.info["n_states"]
         self.dim = self.info["dim"]
 
+    def support_of(self, y0) -> np.ndarray:
+        """uint8 mask (dim) of the packed entries that are nonzero in any row of ``y0``."""
+        a = np.asarray(y0.cpu() if hasattr(y0, "cpu") else y0)
+        nz = (a != 0) if a.ndim == 1 else (a != 0).any(axis=0)
+        return np.ascontiguousarray(nz.astype(np.uint8).reshape(-1))
+
+    def support_info(self, support) -> dict[str, int]:
+        """Plan facts of the restricted tables lb_solve uses for this initial support."""
+        mask = np.ascontiguousarray(np.asarray(support, dtype=bool).astype(np.uint8).reshape(-1))
+        if mask.size != self.dim:
+            raise ValueError("support must have one entry per packed index")
+        info = (C.c_int64 * 16)()
+        _check(library().lb_plan_support_info(self._handle, ptr(mask, C.c_uint8), info))
+        return dict(zip(_INFO_KEYS, [int(v) for v in info]))
+
     def __del__(self):
         h = getattr(self, "_handle", None)
         if h is not None and h.value and _lib is not None:
@@ -177,7 +194,7 @@ class DevicePlan:
               grid_axes: Sequence[np.ndarray] | None = None, n_traj: int | None = None,
               mode: str = "persistent", threads_per_traj: int = 0, workgroups: int = 0,
               stream=None, with_integral: bool = False, variant: int = 0,
-              restrict_support: bool = True, support=None):
+              restrict_support: bool = True, support=None, events=None):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
@@ -286,7 +303,11 @@ class DevicePlan:
             keep.append(support)
             a.y0_support = ptr(support, C.c_uint8)
         sp = _stream_ptr() if stream is None else C.c_void_p(stream.cuda_stream)
+        if events is not None:  # (start, end) torch events recorded right around the launch
+            events[0].record(stream)
         _check(library().lb_solve(self._handle, C.byref(a), sp))
+        if events is not None:
+            events[1].record(stream)
         if with_integral:
             return values, stats, fail_t, integral
         return values, stats, fail_t

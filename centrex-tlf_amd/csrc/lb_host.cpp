@@ -69,6 +69,7 @@ struct DevImage {
   int default_variant = -1, default_W = 0;
   int64_t lds_bytes = 0;
   int64_t n_pub = 0, gather_terms = 0, self_terms = 0, rows_total = 0, flops = 0, active = 0;
+  std::vector<uint8_t> active_mask;  // upper_len
   std::map<std::pair<int, int>, int> per_cu;  // (variant, W) -> resident workgroups per CU
 };
 
@@ -170,8 +171,9 @@ void pick_config(int S, int rows, int* id_out, int* W_out) {
     if (!W) continue;
     const int64_t waste = (int64_t)R * W - S;
     const int g = lb_dev_variant_rows(id);
-    // single-wave variants first (fewest registers that hold the rows)
-    const int64_t score = g ? -1000 + g + (waves == 4 ? 0 : 1)
+    // single-wave variants first: the fewest register rows that hold the
+    // plan, at two waves per SIMD (256 registers per lane, no spills)
+    const int64_t score = g ? -1000 + g + (waves == 2 ? 0 : 1)
                             : waste + (waves == 2 ? 0 : 256) + 8 * R;
     if (score < bestScore) { bestScore = score; best = id; bestW = W; }
   }
@@ -361,6 +363,7 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   im->rows_total = rows_total;
   im->flops = flops;
   im->active = n_act;
+  im->active_mask = act;
   for (int o = 0; o < U; ++o) { im->self_terms += n_self[o]; im->gather_terms += n_gather[o]; }
   return LB_OK;
 }
@@ -493,6 +496,31 @@ extern "C" void lb_plan_destroy(lb_plan* p) { delete p; }
 extern "C" int lb_plan_info(const lb_plan* p, int64_t* info) {
   if (!p || !info) return fail(LB_ERR_INVALID, "null argument");
   std::memcpy(info, p->info, sizeof(p->info));
+  return LB_OK;
+}
+
+namespace {
+int select_image(lb_plan* p, const uint8_t* y0_support, const DevImage** out);
+}
+
+extern "C" int lb_plan_support_info(const lb_plan* p, const uint8_t* y0_support, int64_t* info) {
+  if (!p || !info) return fail(LB_ERR_INVALID, "null argument");
+  const DevImage* im = nullptr;
+  int rc = select_image(const_cast<lb_plan*>(p), y0_support, &im);
+  if (rc) return rc;
+  std::memcpy(info, p->info, sizeof(p->info));
+  int64_t terms = 0, packed = 0;  // terms and packed reals of the active outputs
+  for (int o = 0; o < p->src.U; ++o)
+    if (im->active_mask[o]) {
+      terms += p->src.output_ptrs[o + 1] - p->src.output_ptrs[o];
+      packed += p->src.up_im[o] < 0 ? 1 : 2;
+    }
+  info[1] = packed;
+  info[3] = terms;
+  info[2] = im->active;
+  info[6] = im->n_pub; info[7] = im->gather_terms; info[8] = im->self_terms;
+  info[9] = im->dev.S; info[10] = im->rows_total; info[11] = im->flops; info[13] = im->lds_bytes;
+  info[14] = im->default_variant + 1; info[15] = im->default_W;
   return LB_OK;
 }
 

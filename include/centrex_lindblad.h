@@ -117,6 +117,15 @@ const char* lb_last_error(void);
 int lb_plan_info(const lb_plan* plan, int64_t* info);
 
 /*
+ * The same facts for the plan as lb_solve runs it with initial support
+ * `y0_support` (host, dim bytes, as in lb_solve_args): 1 becomes the number
+ * of packed reals integrated, 2 the active upper elements, 3 their term
+ * count, 6-11 and 13-15 those of the restricted tables.  NULL gives the full
+ * plan.  No reference counterpart.
+ */
+int lb_plan_support_info(const lb_plan* plan, const uint8_t* y0_support, int64_t* info);
+
+/*
  * Batched RHS evaluation: out[b] = d(rho)/dt at (y[b], t[b]) with per-batch
  * overrides of base parameter slots.  Replaces lindblad_rhs_py /
  * rhs_packed_py (python_api.rs:276-352) for a batch; used to check the RHS
