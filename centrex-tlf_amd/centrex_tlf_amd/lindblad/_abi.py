@@ -26,10 +26,11 @@ __all__ = [
 
 LB_OK = 0
 LB_SOLVER = {"dopri5": 0, "tsit5": 1}
-LB_OUT = {"populations": 0, "selected": 1, "integral": 2, "full": 3}
+LB_OUT = {"populations": 0, "selected": 1, "integral": 2, "full": 3, "rate": 4}
 LB_MODE_PERSISTENT = 0
 LB_MODE_PLAIN = 1
-LB_TRAJ = {0: "ok", 1: "maxiters", 2: "underflow", 3: "stiff", 4: "bad_span", 5: "bad_support"}
+LB_TRAJ = {0: "ok", 1: "maxiters", 2: "underflow", 3: "stiff", 4: "bad_span", 5: "bad_support",
+           6: "saveat", 7: "capacity"}
 
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
@@ -100,6 +101,12 @@ class LbSolveArgs(C.Structure):
         ("workgroups", C.c_int32),
         ("out_integral", C.c_void_p),
         ("variant", C.c_int32),
+        ("n_saveat", C.c_int32),
+        ("saveat", C.c_void_p),
+        ("trace", C.c_int32),
+        ("capacity", C.c_int64),
+        ("out_times", C.c_void_p),
+        ("out_counts", C.c_void_p),
         ("y0_support", C.POINTER(C.c_uint8)),
     ]
 

@@ -194,7 +194,8 @@ class DevicePlan:
               grid_axes: Sequence[np.ndarray] | None = None, n_traj: int | None = None,
               mode: str = "persistent", threads_per_traj: int = 0, workgroups: int = 0,
               stream=None, with_integral: bool = False, variant: int = 0,
-              restrict_support: bool = True, support=None, events=None):
+              restrict_support: bool = True, support=None, events=None,
+              save_plan=None, trace: bool = False, capacity: int = 0, with_times: bool = False):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
@@ -205,6 +206,13 @@ class DevicePlan:
         With ``restrict_support`` the solve runs only the density-matrix
         elements reachable from the nonzero entries of ``y0`` (the rest stay
         exactly zero); otherwise every element is integrated.
+
+        Points are pushed at t0 (``save_start``) and every accepted step, or
+        at the times of ``save_plan`` (dense output inside steps).  With
+        ``trace`` every pushed point writes a row: the result is then
+        ``(values[B, capacity, width], stats, fail_t, integral, counts, times)``
+        (``times`` only with ``with_times``, ``integral`` only with
+        ``with_integral``).
         """
         import torch
 
@@ -250,9 +258,16 @@ class DevicePlan:
             width = self.dim
         elif kind == LB_OUT["selected"]:
             width = 2 * len(selected)
+        elif kind == LB_OUT["rate"] and not trace:
+            raise ValueError("rate output needs trace=True")
         else:
             width = 1
-        values = torch.full((B, width), float("nan"), dtype=torch.float64, device=dev)
+        if trace:
+            if capacity <= 0:
+                raise ValueError("trace output needs a positive capacity")
+            values = torch.zeros((B, int(capacity), width), dtype=torch.float64, device=dev)
+        else:
+            values = torch.full((B, width), float("nan"), dtype=torch.float64, device=dev)
         stats = torch.zeros((B, 4), dtype=torch.int64, device=dev)
         fail_t = torch.zeros(B, dtype=torch.float64, device=dev)
         keep: list[np.ndarray] = [slots]
@@ -299,6 +314,20 @@ class DevicePlan:
         a.threads_per_traj = int(threads_per_traj)
         a.workgroups = int(workgroups)
         a.variant = int(variant)
+        counts = times = plan_t = None
+        if save_plan is not None:
+            plan_t = torch.as_tensor(np.asarray(save_plan, dtype=np.float64).reshape(-1), device=dev).contiguous()
+            if plan_t.numel() > 0:
+                a.n_saveat = int(plan_t.numel())
+                a.saveat = _dptr(plan_t)
+        if trace:
+            counts = torch.zeros(B, dtype=torch.int32, device=dev)
+            a.trace = 1
+            a.capacity = int(capacity)
+            a.out_counts = _dptr(counts)
+            if with_times:
+                times = torch.full((B, int(capacity)), float("nan"), dtype=torch.float64, device=dev)
+                a.out_times = _dptr(times)
         if support is not None:
             keep.append(support)
             a.y0_support = ptr(support, C.c_uint8)
@@ -308,6 +337,8 @@ class DevicePlan:
         _check(library().lb_solve(self._handle, C.byref(a), sp))
         if events is not None:
             events[1].record(stream)
+        if trace:
+            return values, stats, fail_t, integral, counts, times
         if with_integral:
             return values, stats, fail_t, integral
         return values, stats, fail_t

@@ -116,6 +116,13 @@ struct DevCall {
   int32_t want_integral;   // accumulate the trapezoid
   int32_t* queue;          // work-queue counter
   int64_t traj_offset;     // plain mode: first trajectory of this launch
+  // pushed points: every accepted step (n_saveat == 0) or a save plan
+  int32_t n_saveat;
+  const double* saveat;    // device, n_saveat ascending times (t0 first when saving the start)
+  int32_t trace;           // 1: one output row per pushed point
+  int64_t capacity;        // rows per trajectory in trace mode
+  double* out_times;       // device n_traj x capacity, or null
+  int32_t* out_counts;     // device n_traj (trace mode)
 };
 
 struct DevRhsCall {

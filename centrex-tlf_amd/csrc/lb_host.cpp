@@ -678,7 +678,9 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
     return fail(LB_ERR_INVALID, "override_values are required");
   }
   c.output_kind = a->output_kind;
-  if (a->output_kind < 0 || a->output_kind > 3) return fail(LB_ERR_INVALID, "unknown output kind");
+  if (a->output_kind < 0 || a->output_kind > 4) return fail(LB_ERR_INVALID, "unknown output kind");
+  if (a->output_kind == LB_OUT_RATE && (!a->trace || a->n_weights <= 0))
+    return fail(LB_ERR_INVALID, "rate output needs trace mode and weights");
   if (a->n_weights > LB_MAX_WEIGHTS) return fail(LB_ERR_UNSUPPORTED, "at most 64 integral weights");
   c.n_weights = a->n_weights;
   for (int i = 0; i < a->n_weights; ++i) {
@@ -707,6 +709,18 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   c.want_integral = (a->output_kind == LB_OUT_INTEGRAL || a->out_integral != nullptr) ? 1 : 0;
   if (c.want_integral && a->n_weights <= 0)
     return fail(LB_ERR_INVALID, "integral outputs require integral weights");
+  if (a->n_saveat < 0) return fail(LB_ERR_INVALID, "n_saveat must be non-negative");
+  if (a->n_saveat > 0 && !a->saveat) return fail(LB_ERR_INVALID, "saveat times are required");
+  c.n_saveat = a->n_saveat;
+  c.saveat = a->saveat;
+  c.trace = a->trace ? 1 : 0;
+  if (c.trace) {
+    if (a->capacity <= 0 || !a->out_counts)
+      return fail(LB_ERR_INVALID, "trace output needs a positive capacity and out_counts");
+    c.capacity = a->capacity;
+    c.out_times = a->out_times;
+    c.out_counts = a->out_counts;
+  }
   c.queue = p->queue;
   int id, W;
   if ((rc = resolve_config(im, a->variant, a->threads_per_traj, &id, &W))) return rc;

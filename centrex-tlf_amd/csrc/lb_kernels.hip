@@ -597,6 +597,76 @@ __device__ __forceinline__ void fold_self(const DevPlan& P, int tid, int W, cons
   }
 }
 
+// Dense output at theta in [0, 1] of the accepted step [x, x + h]
+// (rust/src/ode/dopri5.rs:38-43, 104-143: Hairer's continuous extension;
+// rust/src/ode/tsit5.rs:39-60, 121-142: Tsitouras' interpolant).
+template <int R, int SOLVER>
+__device__ __forceinline__ void dense_point(double th, double h, const double (&y_r)[R], const double (&y_i)[R],
+                                            const double (&yn_r)[R], const double (&yn_i)[R],
+                                            const double (&k_r)[8][R], const double (&k_i)[8][R],
+                                            double (&o_r)[R], double (&o_i)[R]) {
+  if constexpr (SOLVER == 0) {
+    const double D1 = -12715105075.0 / 11282082432.0, D3 = 87487479700.0 / 32700410799.0,
+                 D4 = -10690763975.0 / 1880347072.0, D5 = 701980252875.0 / 199316789632.0,
+                 D6 = -1453857185.0 / 822651844.0, D7 = 69997945.0 / 29380423.0;
+    const double th1 = 1.0 - th;
+    auto one = [&](double y, double yn, double k1, double k3, double k4, double k5, double k6, double k7) {
+      const double d = yn - y;
+      const double b = k1 * h - d;
+      const double r3 = -k7 * h + d - b;
+      const double r4 = h * (D1 * k1 + D3 * k3 + D4 * k4 + D5 * k5 + D6 * k6 + D7 * k7);
+      return y + (d + (b + (r3 + r4 * th1) * th) * th1) * th;
+    };
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      o_r[r] = one(y_r[r], yn_r[r], k_r[1][r], k_r[3][r], k_r[4][r], k_r[5][r], k_r[6][r], k_r[7][r]);
+      o_i[r] = one(y_i[r], yn_i[r], k_i[1][r], k_i[3][r], k_i[4][r], k_i[5][r], k_i[6][r], k_i[7][r]);
+    }
+  } else {
+    const double t2 = th * th, t3 = t2 * th, t4 = t3 * th;
+    const double b1 = 1.0 * th + -2.763706197274826 * t2 + 2.9132554618219126 * t3 + -1.0530884977290216 * t4;
+    const double b2 = 0.1317 * t2 + -0.2234 * t3 + 0.1017 * t4;
+    const double b3 = 3.9302962368947516 * t2 + -5.941033872131505 * t3 + 2.490627285651253 * t4;
+    const double b4 = -12.411077166933676 * t2 + 30.33818863028232 * t3 + -16.548102889244902 * t4;
+    const double b5 = 37.50931341651104 * t2 + -88.1789048947664 * t3 + 47.37952196281928 * t4;
+    const double b6 = -27.896526289197286 * t2 + 65.09189467479366 * t3 + -34.87065786149661 * t4;
+    const double b7 = 1.5 * t2 + -4.0 * t3 + 2.5 * t4;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      o_r[r] = y_r[r] + h * (b1 * k_r[1][r] + b2 * k_r[2][r] + b3 * k_r[3][r] + b4 * k_r[4][r] +
+                             b5 * k_r[5][r] + b6 * k_r[6][r] + b7 * k_r[7][r]);
+      o_i[r] = y_i[r] + h * (b1 * k_i[1][r] + b2 * k_i[2][r] + b3 * k_i[3][r] + b4 * k_i[4][r] +
+                             b5 * k_i[5][r] + b6 * k_i[6][r] + b7 * k_i[7][r]);
+    }
+  }
+}
+
+// One output row of a trace (populations / selected / full; the integral
+// row is written by the caller).
+template <int R>
+__device__ __forceinline__ void write_row(const DevPlan& P, const DevCall& C, const Slots<R>& S, int tid, int W,
+                                          int64_t row, const double (&vr)[R], const double (&vi)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!S.valid(r)) continue;
+    const int s = tid + W * r;
+    if (C.output_kind == LB_OUT_POPULATIONS) {
+      if (S.diag(r)) C.out_values[row * P.n + S.state(r)] = vr[r];
+    } else if (C.output_kind == LB_OUT_FULL) {
+      C.out_values[row * P.dim + P.slot_re[s]] = vr[r];
+      if (!S.diag(r)) C.out_values[row * P.dim + P.slot_im[s]] = vi[r];
+    } else if (C.output_kind == LB_OUT_SELECTED) {
+      const int up = P.slot_up[s];
+      for (int q = 0; q < C.n_selected; ++q) {
+        if (C.sel_upper[q] != up) continue;
+        const double im = (C.sel_flags[q] & 1) ? 0.0 : ((C.sel_flags[q] & 2) ? -vi[r] : vi[r]);
+        C.out_values[2 * (row * C.n_selected + q)] = vr[r];
+        C.out_values[2 * (row * C.n_selected + q) + 1] = im;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -671,6 +741,42 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     int64_t accepted = 0, rejected = 0, rhs_calls = 0;
     int status = 0;
     double fail_x = 0.0, integral = 0.0, last_t = __builtin_nan(""), last_v = 0.0;
+    // pushed points (rust/src/ode/output.rs): the trapezoid integral and the
+    // trace rows.  Pushing at every accepted step folds the integrand into
+    // the error-norm reduction; a save plan (saveat) reduces per point.
+    int64_t n_pushed = 0, si = 0;
+    const bool step_push = C.n_saveat == 0;
+    const bool rate = C.output_kind == LB_OUT_RATE;
+    auto push = [&](double tp, const double (&vr)[R], const double (&vi)[R], bool reduce_integrand) {
+      double v = last_v;
+      if ((C.want_integral || rate) && reduce_integrand) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          block_store(P, tid, W, r, L.red, 1, S.diag(r) && S.valid(r) ? L.wts[S.state(r)] * vr[r] : 0.0);
+        __syncthreads();
+        v = block_total(P, L.red, 1);
+        __syncthreads();  // the next point reuses the reduction slots
+        if (C.want_integral) {
+          if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (tp - last_t);
+          last_t = tp;
+          last_v = v;
+        }
+      }
+      if (C.trace) {
+        if (n_pushed < C.capacity) {
+          const int64_t row = traj * C.capacity + n_pushed;
+          if (C.output_kind == LB_OUT_INTEGRAL) {
+            if (tid == 0) C.out_values[row] = integral;
+          } else if (rate) {
+            if (tid == 0) C.out_values[row] = v;  // weighted rate (output.rs:337-384)
+          } else {
+            write_row<R>(P, C, S, tid, W, row, vr, vi);
+          }
+          if (tid == 0 && C.out_times) C.out_times[row] = tp;
+        }
+        ++n_pushed;
+      }
+    };
     double k_r[8][R], k_i[8][R];
     int buf = 0;
 #define PUB (L.pub + buf * 2 * P.n_pub)
@@ -690,7 +796,9 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     } else if (t1 < t0) {
       status = LB_TRAJ_BAD_SPAN;
       fail_x = t0;
-    } else if (t1 > t0) {
+    } else if (t1 == t0) {
+      if (C.save_start) push(t0, y_r, y_i, true);
+    } else {
       double x = t0, h;
       // ---- k1 = f(t0, y0) (also f0 of hinit); the barrier inside orders the LDS fills above
       LB_RHS(y_r, y_i, k_r[1], k_i[1], L.coef);
@@ -732,14 +840,14 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         h = fmin(100.0 * fabs(h0), fmin(h1, hmax));
       }
       rhs_calls = 1;
-      // ---- first trapezoid point (output.rs:315-330)
-      if (C.want_integral && C.save_start) {
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-          block_store(P, tid, W, r, L.red, 1, S.diag(r) && S.valid(r) ? L.wts[S.state(r)] * y_r[r] : 0.0);
-        __syncthreads();
-        last_t = t0;
-        last_v = block_total(P, L.red, 1);
+      // ---- points at t0 (dopri5.rs:212-220)
+      if (!step_push) {
+        while (si < C.n_saveat && fabs(C.saveat[si] - t0) <= time_tol(t0)) {
+          push(C.saveat[si], y_r, y_i, true);
+          ++si;
+        }
+      } else if (C.save_start) {
+        push(t0, y_r, y_i, true);
       }
       // ---- controller state (common.rs:32-70)
       const double beta = 0.04, alpha = 0.2 - beta * 0.75, facc1 = 1.0 / 0.2, facc2 = 1.0 / 10.0;
@@ -844,7 +952,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
           const double si = C.abstol + fmax(fabs(y_i[r]), fabs(yn_i[r])) * C.reltol;
           const double qr = er / sr, qi = ei / si;
           block_store(P, tid, W, r, L.red, 0, S.valid(r) ? qr * qr + (S.diag(r) ? 0.0 : qi * qi) : 0.0);
-          if (C.want_integral)
+          if (C.want_integral && step_push)
             block_store(P, tid, W, r, L.red, 1, S.diag(r) && S.valid(r) ? L.wts[S.state(r)] * yn_r[r] : 0.0);
           if (need_stiff) {
             const double a0 = k_r[7][r] - k_r[6][r], a1 = k_i[7][r] - k_i[6][r];
@@ -876,12 +984,33 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
               if (nst == 6) ia = 0;
             }
           }
+          const double xo = x;
           x += h;
-          if (C.want_integral) {
-            const double v = block_total(P, L.red, 1);
-            if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (x - last_t);
-            last_t = x;
-            last_v = v;
+          if (step_push) {
+            if (C.want_integral) {
+              const double v = block_total(P, L.red, 1);
+              if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (x - last_t);
+              last_t = x;
+              last_v = v;
+            }
+            if (C.trace) push(x, yn_r, yn_i, rate);
+          } else {
+            // save plan points inside (xo, x] (dopri5.rs:318-344)
+            while (si < C.n_saveat && C.saveat[si] <= x + time_tol(x)) {
+              const double ts = C.saveat[si];
+              if (ts >= xo - time_tol(xo)) {
+                if (fabs(ts - xo) <= time_tol(xo)) {
+                  push(ts, y_r, y_i, true);
+                } else if (fabs(ts - x) <= time_tol(x)) {
+                  push(ts, yn_r, yn_i, true);
+                } else {
+                  double ds_r[R], ds_i[R];
+                  dense_point<R, SOLVER>((ts - xo) / h, h, y_r, y_i, yn_r, yn_i, k_r, k_i, ds_r, ds_i);
+                  push(ts, ds_r, ds_i, true);
+                }
+              }
+              ++si;
+            }
           }
 #pragma unroll
           for (int r = 0; r < R; ++r) {
@@ -904,8 +1033,27 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
 #undef PUB
 
     // ---- outputs
+    if (status == 0 && !step_push && si != C.n_saveat) {
+      status = LB_TRAJ_SAVEAT;
+      fail_x = (double)si;  // plan points reached
+    }
+    if (status == 0 && C.trace && n_pushed > C.capacity) status = LB_TRAJ_CAPACITY;
     const bool ok = status == 0;
     const double nanv = __builtin_nan("");
+    if (C.trace) {
+      // rows were written as the points were pushed; dropped elements stay
+      // at the zeros the caller provides
+      if (tid == 0) {
+        C.out_counts[traj] = (int32_t)n_pushed;
+        if (C.out_integral) C.out_integral[traj] = ok ? integral : nanv;
+        C.out_stats[4 * traj + 0] = accepted;
+        C.out_stats[4 * traj + 1] = rejected;
+        C.out_stats[4 * traj + 2] = rhs_calls;
+        C.out_stats[4 * traj + 3] = status;
+        C.out_fail_t[traj] = fail_x;
+      }
+      continue;
+    }
     if (P.restricted) {
       // elements outside the restricted plan are exactly zero
       if (C.output_kind == LB_OUT_POPULATIONS) {

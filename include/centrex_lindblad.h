@@ -35,6 +35,11 @@ extern "C" {
 #define LB_TRAJ_BAD_SPAN 4  /* t1 < t0 ("only forward integration")         dopri5.rs:175-177 */
 #define LB_TRAJ_BAD_SUPPORT 5 /* y0 is nonzero outside lb_solve_args.y0_support (no reference
                                  counterpart: the mask is a GPU-side promise) */
+#define LB_TRAJ_SAVEAT 6      /* "ended before all saveat: ..."            dopri5.rs:369-376;
+                                 out_fail_t holds the number of plan points reached */
+#define LB_TRAJ_CAPACITY 7    /* more trace rows than lb_solve_args.capacity (the integration
+                                 completed; counts and stats are exact, rows past capacity
+                                 were dropped).  No reference counterpart. */
 
 /* Solvers (rust/src/ode/batch.rs:7-27). */
 #define LB_SOLVER_DOPRI5 0
@@ -45,6 +50,8 @@ extern "C" {
 #define LB_OUT_SELECTED 1    /* final selected entries, complex, width = count   */
 #define LB_OUT_INTEGRAL 2    /* trapezoid of sum_k w_k rho_kk over pushed points */
 #define LB_OUT_FULL 3        /* final packed state, width dim                     */
+#define LB_OUT_RATE 4        /* sum_k w_k rho_kk at each pushed point (trace only;
+                                output.rs:337-384 WeightedRateOutput), width 1    */
 
 /* Launch modes. */
 #define LB_MODE_PERSISTENT 0  /* one launch, persistent workgroups + work queue */
@@ -190,6 +197,16 @@ typedef struct lb_solve_args {
   int32_t variant;                /* kernel variant: 0 = plan default, k = variant k-1
                                      (slots per thread / occupancy; results are
                                      bitwise independent of it) */
+  int32_t n_saveat;               /* > 0: push points at these times (dense output inside steps),
+                                     otherwise at t0 (save_start) and every accepted step */
+  const double* saveat;           /* device, ascending, within [t0, t1] (the save plan of
+                                     rust/src/ode/common.rs:124-151, t0 included when saving
+                                     the start) */
+  int32_t trace;                  /* 1: out_values holds one row per pushed point,
+                                     n_traj x capacity x width (output_when="saveat") */
+  int64_t capacity;               /* rows per trajectory in trace mode */
+  double* out_times;              /* device, n_traj x capacity times of the rows, or NULL */
+  int32_t* out_counts;            /* device, n_traj: rows pushed (trace mode) */
   const uint8_t* y0_support;      /* host, dim bytes, or NULL: nonzero where any
                                      trajectory's y0 may be nonzero.  The solve then
                                      runs only the elements reachable from that

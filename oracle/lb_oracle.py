@@ -232,7 +232,7 @@ class Oracle:
         }
         if output == "selected":
             result["values"] = result["values"].view(np.complex128)
-        if output_when == "final":
+        if output_when == "final" and count > 0:
             result["final"] = result["values"][:, -1, :]
         return result
 
