@@ -76,6 +76,8 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=128)
     ap.add_argument("--profile", action="store_true", help="GPU steps only (for rocprofv3)")
     ap.add_argument("--full-plan", action="store_true", help="integrate every element (no restriction)")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="launch eagerly instead of replaying a HIP graph of one step")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -134,10 +136,22 @@ def main() -> int:
         log(f"bench: warmup {i + 1}/{args.warmup} done")
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    graph = None
+    if args.graph:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            out = step()
+        torch.cuda.synchronize()
+        log("bench: captured one step in a HIP graph")
     torch.cuda.synchronize()
     t_begin = time.perf_counter()
     for i in range(args.steps):
-        out = step(events=(starts[i], ends[i]))
+        if graph is not None:
+            starts[i].record()
+            graph.replay()
+            ends[i].record()
+        else:
+            out = step(events=(starts[i], ends[i]))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_begin
     kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
@@ -179,6 +193,7 @@ def main() -> int:
             "parallelism": "single_gpu",
             "integrated_elements": f"{rinfo['upper_len']} of {info['upper_len']} upper-triangle elements",
             "kernel_variant": args.variant or rinfo["kernel_variant"],
+            "hip_graph": bool(args.graph),
             "threads_per_traj": args.threads_per_traj or rinfo["threads_per_traj"],
         },
         "roofline": {
