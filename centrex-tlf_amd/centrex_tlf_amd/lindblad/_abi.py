@@ -63,6 +63,8 @@ class LbPlanDesc(C.Structure):
         ("input_conj", _u8p),
         ("factors_re", _f64p),
         ("factors_im", _f64p),
+        ("n_diff_params", C.c_int32),
+        ("diff_param_slots", _i32p),
     ]
 
 
@@ -108,6 +110,15 @@ class LbSolveArgs(C.Structure):
         ("out_times", C.c_void_p),
         ("out_counts", C.c_void_p),
         ("y0_support", C.POINTER(C.c_uint8)),
+        ("tape_t", C.c_void_p),
+        ("tape_h", C.c_void_p),
+        ("tape_y", C.c_void_p),
+        ("tape_n", C.c_void_p),
+        ("tape_cap", C.c_int64),
+        ("ckpt_every", C.c_int32),
+        ("out_grad", C.c_void_p),
+        ("replay_h", C.c_void_p),
+        ("replay_n", C.c_void_p),
     ]
 
 
@@ -158,4 +169,7 @@ def make_plan_desc(flat: dict[str, Any]) -> tuple[LbPlanDesc, list[np.ndarray]]:
     d.input_conj = ptr(arr("input_conj", np.uint8), C.c_uint8)
     d.factors_re = ptr(arr("factors_re", np.float64), C.c_double)
     d.factors_im = ptr(arr("factors_im", np.float64), C.c_double)
+    d.n_diff_params = int(flat.get("n_diff_params", 0))
+    if d.n_diff_params:
+        d.diff_param_slots = ptr(arr("diff_param_slots", np.int32), C.c_int32)
     return d, keep

@@ -15,7 +15,7 @@ import numpy as np
 
 from ._abi import LB_MODE_PERSISTENT, LB_MODE_PLAIN, LB_OUT, LB_SOLVER, LbSolveArgs, make_plan_desc, ptr
 
-__all__ = ["DevicePlan", "NativeError", "library", "library_path", "torch_device"]
+__all__ = ["DevicePlan", "NativeError", "Tape", "library", "library_path", "torch_device"]
 
 LIB_NAME = "liblindblad_gpu.so"
 _PKG_DIR = Path(__file__).resolve().parents[1]
@@ -64,6 +64,8 @@ def library() -> C.CDLL:
                               C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]
     L.lb_solve.restype = C.c_int
     L.lb_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.lb_adjoint.restype = C.c_int
+    L.lb_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.lb_variant_info.restype = C.c_int
     L.lb_variant_info.argtypes = [C.c_int32] + [C.POINTER(C.c_int32)] * 3
     _lib = L
@@ -121,6 +123,25 @@ class GridAxes:
         self.count = int(np.prod(self.lengths))
         self.values = torch.view_as_real(
             torch.as_tensor(np.concatenate(arrays), device=torch_device())).contiguous()
+
+
+class Tape:
+    """Device arrays of an adjoint tape for B trajectories (see lb_solve_args)."""
+
+    def __init__(self, n_traj: int, slots: int, cap: int, every: int = 16):
+        import torch
+
+        dev = torch_device()
+        self.cap, self.every = int(cap), int(every)
+        n_ckpt = (self.cap + self.every - 1) // self.every
+        self.t = torch.zeros((n_traj, self.cap), dtype=torch.float64, device=dev)
+        self.h = torch.zeros((n_traj, self.cap), dtype=torch.float64, device=dev)
+        self.y = torch.zeros((n_traj, n_ckpt, int(slots), 2), dtype=torch.float64, device=dev)
+        self.n = torch.zeros(n_traj, dtype=torch.int32, device=dev)
+
+    @property
+    def nbytes(self) -> int:
+        return sum(int(x.numel() * x.element_size()) for x in (self.t, self.h, self.y, self.n))
 
 
 class DevicePlan:
@@ -195,7 +216,8 @@ class DevicePlan:
               mode: str = "persistent", threads_per_traj: int = 0, workgroups: int = 0,
               stream=None, with_integral: bool = False, variant: int = 0,
               restrict_support: bool = True, support=None, events=None,
-              save_plan=None, trace: bool = False, capacity: int = 0, with_times: bool = False):
+              save_plan=None, trace: bool = False, capacity: int = 0, with_times: bool = False,
+              tape=None, adjoint_grad=None, replay=None):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
@@ -213,6 +235,10 @@ class DevicePlan:
         ``(values[B, capacity, width], stats, fail_t, integral, counts, times)``
         (``times`` only with ``with_times``, ``integral`` only with
         ``with_integral``).
+
+        ``tape`` (a :class:`Tape`) records the accepted steps for the adjoint;
+        with ``adjoint_grad`` (a (B, P) float64 tensor) the call runs the
+        adjoint of that recorded tape instead of a solve (``lb_adjoint``).
         """
         import torch
 
@@ -331,9 +357,22 @@ class DevicePlan:
         if support is not None:
             keep.append(support)
             a.y0_support = ptr(support, C.c_uint8)
+        if replay is not None:  # integrate with the step sizes of a recorded tape
+            a.replay_h, a.replay_n = _dptr(replay.h), _dptr(replay.n)
+            a.tape_cap = int(replay.cap)
+        if tape is not None:
+            a.tape_t, a.tape_h = _dptr(tape.t), _dptr(tape.h)
+            a.tape_y, a.tape_n = _dptr(tape.y), _dptr(tape.n)
+            a.tape_cap, a.ckpt_every = int(tape.cap), int(tape.every)
         sp = _stream_ptr() if stream is None else C.c_void_p(stream.cuda_stream)
         if events is not None:  # (start, end) torch events recorded right around the launch
             events[0].record(stream)
+        if adjoint_grad is not None:
+            a.out_grad = _dptr(adjoint_grad)
+            _check(library().lb_adjoint(self._handle, C.byref(a), sp))
+            if events is not None:
+                events[1].record(stream)
+            return adjoint_grad
         _check(library().lb_solve(self._handle, C.byref(a), sp))
         if events is not None:
             events[1].record(stream)

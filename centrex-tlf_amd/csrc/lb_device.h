@@ -27,6 +27,7 @@
 #define LB_MAX_WEIGHTS 64
 #define LB_MAX_SELECTED 128
 #define LB_MAX_STACK 24
+#define LB_MAX_DIFF 8
 #define LB_STATIC_COEF 255u
 
 // gather word: bits 0..19 LDS publication index, bit 20 real input (diagonal),
@@ -53,6 +54,8 @@ struct DevPlan {
   int32_t nblk;     // S / 64
   int32_t n_pub;    // published elements
   int32_t K;        // Hamiltonian coefficients
+  int32_t KD;       // coefficient programs: K, plus K * n_diff derivative programs
+  int32_t n_diff;   // differentiated parameters (0 for an ordinary plan)
   int32_t n_slots, n_base, n_compounds;
   int32_t max_self; // self terms per slot (ELL width)
   int32_t time_dependent;
@@ -76,6 +79,13 @@ struct DevPlan {
   int32_t restricted;
   const uint8_t* upper_active;   // upper_len
   const int32_t* packed_upper;   // dim: packed index -> upper index
+  // differentiable plans: for each input slot, the terms reading it (ELL per
+  // 64-slot block; word as lb_gword with the OUTPUT slot in bits 0..19)
+  int32_t t_rows_total;
+  const int32_t* t_blk_row0;
+  const int32_t* t_blk_rows;
+  const uint32_t* t_word;
+  const double* t_f;
   // expression machine
   const DevInstr* instr;
   const int32_t* expr_start;    // n_compounds + K + 1
@@ -123,6 +133,22 @@ struct DevCall {
   int64_t capacity;        // rows per trajectory in trace mode
   double* out_times;       // device n_traj x capacity, or null
   int32_t* out_counts;     // device n_traj (trace mode)
+  // tape for the adjoint: accepted-step start times and sizes, and the state
+  // at the start of every ckpt_every-th accepted step (slot order, re/im)
+  double* tape_t;          // n_traj x tape_cap
+  double* tape_h;          // n_traj x tape_cap
+  double* tape_y;          // n_traj x ceil(tape_cap / ckpt_every) x S x 2
+  int32_t* tape_n;         // n_traj accepted steps recorded
+  int64_t tape_cap;
+  int32_t ckpt_every;
+  // adjoint workspace (per workgroup) and output
+  double* seg_y;           // workgroups x ckpt_every x 6 x S x 2: stage inputs of a segment
+  double* seg_c;           // workgroups x ckpt_every x 6 x (KD + 1) x 2: coefficient rows
+  double* out_grad;        // n_traj x n_diff: d(integral)/d(parameter)
+  // replay: take the accepted step sizes of a recorded tape (stride tape_cap)
+  // instead of the step-size controller; every step is accepted
+  const double* replay_h;
+  const int32_t* replay_n;
 };
 
 struct DevRhsCall {

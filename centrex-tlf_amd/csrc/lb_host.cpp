@@ -24,6 +24,8 @@
 extern "C" int lb_dev_variant_count(void);
 extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
 extern "C" int lb_dev_variant_rows(int id);
+extern "C" int lb_dev_adjoint_variant(int id, int* R, int* maxw);
+extern "C" void* lb_dev_adjoint_kernel(int id, int solver);
 extern "C" void* lb_dev_solve_kernel(int id, int solver);
 extern "C" void* lb_dev_rhs_kernel(int id);
 extern "C" void* lb_dev_queue_reset_kernel(void);
@@ -49,6 +51,7 @@ int fail(int code, const std::string& msg) {
 // Validated copy of a plan description plus the lowered expression program.
 struct PlanSource {
   int n = 0, U = 0, K = 0, n_base = 0;
+  int n_diff = 0;  // differentiated parameters (derivative programs follow the K coefficients)
   int64_t T = 0;
   std::vector<int64_t> output_ptrs, input_indices, coeff_indices;
   std::vector<uint8_t> coeff_conj, input_conj, coef_dyn;
@@ -71,6 +74,9 @@ struct DevImage {
   int64_t n_pub = 0, gather_terms = 0, self_terms = 0, rows_total = 0, flops = 0, active = 0;
   std::vector<uint8_t> active_mask;  // upper_len
   std::map<std::pair<int, int>, int> per_cu;  // (variant, W) -> resident workgroups per CU
+  void* ws_y = nullptr;  // adjoint workspace (grow-only)
+  void* ws_c = nullptr;
+  size_t ws_y_len = 0, ws_c_len = 0;
 };
 
 struct lb_plan {
@@ -83,7 +89,10 @@ struct lb_plan {
   int64_t info[16] = {0};
   ~lb_plan() {
     auto release = [](DevImage* im) {
-      if (im) for (void* a : im->allocations) (void)hipFree(a);
+      if (!im) return;
+      for (void* a : im->allocations) (void)hipFree(a);
+      if (im->ws_y) (void)hipFree(im->ws_y);
+      if (im->ws_c) (void)hipFree(im->ws_c);
     };
     release(full.get());
     for (auto& kv : restricted) release(kv.second.get());
@@ -309,6 +318,49 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
       }
     }
   }
+  // transposed tables of a differentiable plan: for each input slot, the terms
+  // that read it (self-folded terms excluded), output named by its slot
+  std::vector<int32_t> t_row0(nblk, 0), t_rows(nblk, 0);
+  std::vector<uint32_t> t_word;
+  std::vector<double> t_f;
+  int t_total = 0;
+  if (src.n_diff > 0) {
+    std::vector<int32_t> slot_of(U, -1);
+    for (int s = 0; s < n_act; ++s) slot_of[order[s]] = s;
+    std::vector<std::vector<int64_t>> readers(S);
+    for (int o = 0; o < U; ++o) {
+      if (!act[o]) continue;
+      for (int64_t k = src.output_ptrs[o]; k < src.output_ptrs[o + 1]; ++k) {
+        const int in = (int)src.input_indices[k];
+        if (in == o && !is_dyn(k) && !src.input_conj[k]) continue;
+        readers[slot_of[in]].push_back(k);
+      }
+    }
+    for (int b = 0; b < nblk; ++b) {
+      int rows = 0;
+      for (int l = 0; l < 64; ++l) rows = std::max<int>(rows, (int)readers[64 * b + l].size());
+      t_row0[b] = t_total;
+      t_rows[b] = rows;
+      t_total += rows;
+    }
+    t_word.assign((size_t)t_total * 64, lb_gword(0, true, false, false, LB_STATIC_COEF));
+    t_f.assign((size_t)t_total * 64 * 2, 0.0);
+    std::vector<int32_t> out_of(src.T, -1);
+    for (int o = 0; o < U; ++o)
+      for (int64_t k = src.output_ptrs[o]; k < src.output_ptrs[o + 1]; ++k) out_of[k] = o;
+    for (int s = 0; s < n_act; ++s) {
+      const int b = s / 64, l = s % 64, in = order[s];
+      int j = 0;
+      for (int64_t k : readers[s]) {
+        const int64_t ci = src.coeff_indices[k];
+        const size_t e = (size_t)(t_row0[b] + j++) * 64 + l;
+        t_word[e] = lb_gword((uint32_t)slot_of[out_of[k]], src.up_im[in] < 0, src.input_conj[k] != 0,
+                             src.coeff_conj[k] != 0, ci < 0 ? LB_STATIC_COEF : (uint32_t)ci);
+        t_f[2 * e] = src.factors_re[k];
+        t_f[2 * e + 1] = src.factors_im[k];
+      }
+    }
+  }
   // packed index -> upper index, and the active flags (outputs of dropped
   // elements are written as zeros)
   std::vector<int32_t> packed_upper((size_t)n * n, 0);
@@ -325,6 +377,8 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   P.nblk = nblk;
   P.n_pub = n_pub;
   P.K = src.K;
+  P.KD = src.K * (1 + src.n_diff);
+  P.n_diff = src.n_diff;
   P.n_slots = src.prog.n_slots;
   P.n_base = src.n_base;
   P.n_compounds = src.prog.n_compounds;
@@ -356,6 +410,11 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   up(src.base, &P.base);
   up(act, &P.upper_active);
   up(packed_upper, &P.packed_upper);
+  P.t_rows_total = t_total;
+  up(t_row0, &P.t_blk_row0);
+  up(t_rows, &P.t_blk_rows);
+  up(t_word, &P.t_word);
+  up(t_f, &P.t_f);
   if (rc != LB_OK) return rc;
   pick_config(S, rows_total, &im->default_variant, &im->default_W);
   im->lds_bytes = lds_bytes_for(P);
@@ -377,13 +436,22 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   const int n = d->n_states, U = n * (n + 1) / 2;
   if (d->n_base < 0 || d->n_base > d->n_slots) return fail(LB_ERR_INVALID, "n_base exceeds n_slots");
   if (d->n_coefficients > 254) return fail(LB_ERR_UNSUPPORTED, "more than 254 Hamiltonian coefficients");
-  const int n_expr = d->n_compounds + d->n_coefficients;
+  if (d->n_diff_params < 0 || d->n_diff_params > LB_MAX_DIFF)
+    return fail(LB_ERR_UNSUPPORTED, "at most 8 differentiated parameters");
+  for (int m = 0; m < d->n_diff_params; ++m)
+    if (!d->diff_param_slots || d->diff_param_slots[m] < 0 || d->diff_param_slots[m] >= d->n_base)
+      return fail(LB_ERR_INVALID, "differentiated parameters must be base slots");
+  // the expression list seen by the program builder: compounds, coefficients,
+  // then the derivative programs of a differentiable plan
+  lb_plan_desc dd = *d;
+  dd.n_coefficients = d->n_coefficients * (1 + d->n_diff_params);
+  const int n_expr = d->n_compounds + dd.n_coefficients;
   for (int c = 0; c < d->n_compounds; ++c)
     if (d->compound_slots[c] < d->n_base || d->compound_slots[c] >= d->n_slots)
       return fail(LB_ERR_INVALID, "compound slot out of range");
   for (int e = 0; e < n_expr; ++e) {
     std::string why;
-    int rc = check_expression(d, d->expr_ptrs[e], d->expr_ptrs[e + 1], d->n_slots, why);
+    int rc = check_expression(&dd, dd.expr_ptrs[e], dd.expr_ptrs[e + 1], dd.n_slots, why);
     if (rc == -1) return fail(LB_ERR_INVALID, "expression " + std::to_string(e) + ": " + why);
     if (rc == -2) return fail(LB_ERR_UNSUPPORTED, "expression " + std::to_string(e) + ": " + why);
   }
@@ -405,6 +473,7 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   src.n = n;
   src.U = U;
   src.K = d->n_coefficients;
+  src.n_diff = d->n_diff_params;
   src.n_base = d->n_base;
   src.T = T;
   src.output_ptrs.assign(d->output_ptrs, d->output_ptrs + U + 1);
@@ -416,7 +485,7 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   src.factors_im.assign(d->factors_im, d->factors_im + T);
 
   // ---- time dependence (plan.rs:1235-1291)
-  std::vector<uint8_t> dyn_slot(d->n_slots, 0), compound_dyn(d->n_compounds, 0), coef_dyn(d->n_coefficients, 0);
+  std::vector<uint8_t> dyn_slot(d->n_slots, 0), compound_dyn(d->n_compounds, 0), coef_dyn(dd.n_coefficients, 0);
   bool time_dependent = false;
   auto reads_dynamic = [&](int e, bool* uses_time) {
     bool dyn = false;
@@ -434,11 +503,11 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
     time_dependent |= ut;
   }
   int n_dyn_coef = 0;
-  for (int k = 0; k < d->n_coefficients; ++k) {
+  for (int k = 0; k < dd.n_coefficients; ++k) {
     bool ut;
     coef_dyn[k] = reads_dynamic(d->n_compounds + k, &ut);
     time_dependent |= ut;
-    n_dyn_coef += coef_dyn[k];
+    if (k < d->n_coefficients) n_dyn_coef += coef_dyn[k];
   }
   src.coef_dyn = coef_dyn;
   src.time_dependent = time_dependent;
@@ -459,7 +528,7 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   // ---- expression machine (time-independent subexpressions cached per trajectory)
   {
     std::string why;
-    if (lb_build_program(d, dyn_slot, compound_dyn, coef_dyn, &src.prog, why) != 0)
+    if (lb_build_program(&dd, dyn_slot, compound_dyn, coef_dyn, &src.prog, why) != 0)
       return fail(LB_ERR_INVALID, why);
   }
   src.expr_dyn.assign(src.prog.compound_dyn.begin(), src.prog.compound_dyn.end());
@@ -630,9 +699,11 @@ extern "C" int lb_rhs_eval(const lb_plan* p, int64_t batch, const double* y, con
   return LB_OK;
 }
 
-extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) {
-  if (!p || !a) return fail(LB_ERR_INVALID, "null argument");
-  if (a->n_traj <= 0) return LB_OK;
+namespace {
+
+// Validate a solve request and fill the kernel's call record; picks the
+// device image (full or restricted to the y0 support closure).
+int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out, DevCall* cp) {
   if (!a->y0 || !a->out_values || !a->out_stats || !a->out_fail_t)
     return fail(LB_ERR_INVALID, "y0 and output buffers are required");
   if (a->maxiters <= 0) return fail(LB_ERR_INVALID, "maxiters must be positive");
@@ -644,7 +715,8 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   const DevImage* im = nullptr;
   int rc = select_image(const_cast<lb_plan*>(p), a->y0_support, &im);
   if (rc) return rc;
-  DevCall c{};
+  DevCall& c = *cp;
+  c = DevCall{};
   c.n_traj = a->n_traj;
   c.t0 = a->t0;
   c.t1 = a->t1;
@@ -721,7 +793,36 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
     c.out_times = a->out_times;
     c.out_counts = a->out_counts;
   }
+  if (a->tape_h) {
+    if (!a->tape_t || !a->tape_y || !a->tape_n || a->tape_cap <= 0 || a->ckpt_every <= 0)
+      return fail(LB_ERR_INVALID, "an adjoint tape needs tape_t, tape_y, tape_n, tape_cap and ckpt_every");
+    c.tape_t = a->tape_t;
+    c.tape_h = a->tape_h;
+    c.tape_y = a->tape_y;
+    c.tape_n = a->tape_n;
+    c.tape_cap = a->tape_cap;
+    c.ckpt_every = a->ckpt_every;
+  }
+  if (a->replay_h) {
+    if (!a->replay_n || a->tape_cap <= 0) return fail(LB_ERR_INVALID, "a replay needs replay_n and tape_cap");
+    c.replay_h = a->replay_h;
+    c.replay_n = a->replay_n;
+    c.tape_cap = a->tape_cap;
+  }
   c.queue = p->queue;
+  *im_out = im;
+  return LB_OK;
+}
+
+}  // namespace
+
+extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) {
+  if (!p || !a) return fail(LB_ERR_INVALID, "null argument");
+  if (a->n_traj <= 0) return LB_OK;
+  const DevImage* im = nullptr;
+  DevCall c{};
+  int rc = fill_call(p, a, &im, &c);
+  if (rc) return rc;
   int id, W;
   if ((rc = resolve_config(im, a->variant, a->threads_per_traj, &id, &W))) return rc;
   void* fn = lb_dev_solve_kernel(id, a->solver);
@@ -764,6 +865,82 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   c.traj_offset = 0;
   void* rargs[] = {&c.queue};
   HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
+  void* args[] = {&P, &c};
+  HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
+  HIP_TRY(hipGetLastError());
+  return LB_OK;
+}
+
+namespace {
+
+// Must match carve_adj() in lb_kernels.hip.
+int64_t lds_bytes_adj(const DevPlan& P) {
+  return 8 * (2 * (int64_t)P.n_pub) + 8 * 2 * (int64_t)P.S + 8 * 2 * (int64_t)P.S +
+         8 * (int64_t)((P.n + 1) & ~1) + 16 * 7 * (int64_t)(P.KD + 1) + 16 * 7 * (int64_t)P.n_slots +
+         8 * (int64_t)LB_MAX_DIFF * 16 + 16 * 8 * (int64_t)P.vm_depth + 16;
+}
+
+}  // namespace
+
+extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream) {
+  if (!p || !a) return fail(LB_ERR_INVALID, "null argument");
+  if (a->n_traj <= 0) return LB_OK;
+  if (p->src.n_diff <= 0) return fail(LB_ERR_INVALID, "the plan has no differentiated parameters");
+  if (!a->tape_h || !a->out_grad) return fail(LB_ERR_INVALID, "lb_adjoint needs the recorded tape and out_grad");
+  if (a->output_kind != LB_OUT_INTEGRAL || a->n_weights <= 0)
+    return fail(LB_ERR_INVALID, "the adjoint differentiates the weighted integral output");
+  if (a->n_saveat > 0) return fail(LB_ERR_UNSUPPORTED, "the adjoint supports pushes at every accepted step only");
+  const DevImage* im = nullptr;
+  DevCall c{};
+  int rc = fill_call(p, a, &im, &c);
+  if (rc) return rc;
+  c.out_grad = a->out_grad;
+  // the adjoint variant with the fewest slots per thread that covers the plan
+  const int S = im->dev.S;
+  int id = -1, W = 0, R = 0, maxw = 0;
+  for (int v = 0; lb_dev_adjoint_variant(v, &R, &maxw); ++v) {
+    const int w = std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
+    if (w <= maxw) { id = v; W = w; break; }
+  }
+  if (id < 0) return fail(LB_ERR_UNSUPPORTED, "plan is too large for the adjoint kernel variants");
+  void* fn = lb_dev_adjoint_kernel(id, a->solver);
+  const size_t lds = (size_t)lds_bytes_adj(im->dev);
+  if ((rc = set_lds_limit(fn, lds))) return rc;
+  int grid = a->workgroups;
+  if (grid <= 0) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, W, lds));
+    grid = std::max(1, per_cu) * std::max(1, p->cus);
+  }
+  grid = (int)std::min<int64_t>(grid, a->n_traj);
+  // workspace: ckpt_every recomputed steps per resident workgroup
+  const size_t need_y = (size_t)grid * a->ckpt_every * 6 * S * 2;
+  const size_t need_c = (size_t)grid * a->ckpt_every * 6 * (im->dev.KD + 1) * 2;
+  {
+    std::lock_guard<std::mutex> lock(const_cast<lb_plan*>(p)->mu);
+    DevImage* mi = const_cast<DevImage*>(im);
+    if (mi->ws_y_len < need_y) {
+      if (mi->ws_y) (void)hipFree(mi->ws_y);
+      mi->ws_y = nullptr;
+      mi->ws_y_len = 0;
+      HIP_TRY(hipMalloc(&mi->ws_y, need_y * sizeof(double)));
+      mi->ws_y_len = need_y;
+    }
+    if (mi->ws_c_len < need_c) {
+      if (mi->ws_c) (void)hipFree(mi->ws_c);
+      mi->ws_c = nullptr;
+      mi->ws_c_len = 0;
+      HIP_TRY(hipMalloc(&mi->ws_c, need_c * sizeof(double)));
+      mi->ws_c_len = need_c;
+    }
+    c.seg_y = static_cast<double*>(mi->ws_y);
+    c.seg_c = static_cast<double*>(mi->ws_c);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  DevPlan P = im->dev;
+  c.traj_offset = 0;
+  void* rargs[] = {&c.queue};
+  HIP_TRY(hipLaunchKernel(lb_dev_queue_reset_kernel(), dim3(1), dim3(1), rargs, 0, s));
   void* args[] = {&P, &c};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
   HIP_TRY(hipGetLastError());

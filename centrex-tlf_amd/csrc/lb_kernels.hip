@@ -524,14 +524,21 @@ __device__ __forceinline__ double time_tol(double t) { return 10.0 * DBL_EPSILON
 // Evaluate the parameter graph + coefficients for time t into slot copy `sl`
 // and coefficient row `cf` (compounds first, in dependency order, then the
 // coefficients); only_dynamic skips the time-independent expressions.
+// n_prog coefficient programs are evaluated (P.K for the solver, P.KD with
+// the derivative programs for the adjoint); program j lands in cf[j] for
+// j < K and in cf[j + 1] after the constant entry cf[K].
 __device__ __forceinline__ void eval_coefficients(const DevPlan& P, dcpx* sl, dcpx* cf, double t,
-                                                  bool only_dynamic, dcpx* stk) {
-  const int n_expr = P.n_compounds + P.K;
+                                                  bool only_dynamic, dcpx* stk, int n_prog = -1) {
+  const int n_expr = P.n_compounds + (n_prog < 0 ? P.K : n_prog);
   for (int e = 0; e < n_expr; ++e) {
     if (only_dynamic && !cst(P.expr_dyn)[e]) continue;
     const dcpx v = vm_eval(cst(P.instr), cst(P.expr_start)[e], cst(P.expr_start)[e + 1], sl, t, stk);
-    if (e < P.n_compounds) sl[cst(P.compound_slot)[e]] = v;
-    else cf[e - P.n_compounds] = v;
+    if (e < P.n_compounds) {
+      sl[cst(P.compound_slot)[e]] = v;
+    } else {
+      const int j = e - P.n_compounds;
+      cf[j < P.K ? j : j + 1] = v;
+    }
   }
   cf[P.K] = mk(1.0, 0.0);
 }
@@ -858,8 +865,16 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
 
       while (!last) {
         if (ns > C.maxiters) { status = LB_TRAJ_MAXITERS; fail_x = x; break; }
-        if (0.1 * fabs(h) <= DBL_EPSILON * fabs(x)) { status = LB_TRAJ_UNDERFLOW; fail_x = x; break; }
-        if (x + 1.01 * h - t1 > 0.0) { h = t1 - x; last = true; }
+        if (C.replay_h) {
+          // recorded schedule: step sizes of a tape, every step accepted
+          const int64_t nrep = C.replay_n[traj];
+          if (accepted >= nrep || accepted >= C.tape_cap) { status = LB_TRAJ_CAPACITY; fail_x = x; break; }
+          h = C.replay_h[traj * C.tape_cap + accepted];
+          last = accepted + 1 == nrep;
+        } else {
+          if (0.1 * fabs(h) <= DBL_EPSILON * fabs(x)) { status = LB_TRAJ_UNDERFLOW; fail_x = x; break; }
+          if (x + 1.01 * h - t1 > 0.0) { h = t1 - x; last = true; }
+        }
         ++ns;
         // The expression machine needs registers of its own: the single-wave
         // form parks its state in LDS meanwhile (the multi-wave form keeps
@@ -960,7 +975,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
           }
         }
         __syncthreads();
-        const double err = sqrt(block_total(P, L.red, 0) / (double)P.dim);
+        const double err = C.replay_h ? 0.0 : sqrt(block_total(P, L.red, 0) / (double)P.dim);
         // ---- controller (common.rs:48-69)
         const double fac11 = pow(err, alpha);
         double fac = fac11 * pow(fac_old, -beta);
@@ -986,6 +1001,29 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
           }
           const double xo = x;
           x += h;
+          if (C.tape_h) {
+            // adjoint tape: step start and size, and a checkpoint of the state
+            // at the start of every ckpt_every-th step
+            const int64_t n = accepted - 1;
+            if (n < C.tape_cap) {
+              if (tid == 0) {
+                C.tape_t[traj * C.tape_cap + n] = xo;
+                C.tape_h[traj * C.tape_cap + n] = h;
+              }
+              if (n % C.ckpt_every == 0) {
+                const int64_t ncap = (C.tape_cap + C.ckpt_every - 1) / C.ckpt_every;
+                double* ck = C.tape_y + ((traj * ncap + n / C.ckpt_every) * P.S) * 2;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                  const int s = tid + W * r;
+                  if (s < P.S) {
+                    ck[2 * s] = y_r[r];
+                    ck[2 * s + 1] = y_i[r];
+                  }
+                }
+              }
+            }
+          }
           if (step_push) {
             if (C.want_integral) {
               const double v = block_total(P, L.red, 1);
@@ -1038,6 +1076,8 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
       fail_x = (double)si;  // plan points reached
     }
     if (status == 0 && C.trace && n_pushed > C.capacity) status = LB_TRAJ_CAPACITY;
+    if (status == 0 && C.tape_h && accepted > C.tape_cap) status = LB_TRAJ_CAPACITY;
+    if (C.tape_n && tid == 0) C.tape_n[traj] = (int32_t)accepted;
     const bool ok = status == 0;
     const double nanv = __builtin_nan("");
     if (C.trace) {
@@ -1097,6 +1137,380 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
       C.out_stats[4 * traj + 2] = rhs_calls;
       C.out_stats[4 * traj + 3] = status;
       C.out_fail_t[traj] = fail_x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the accepted Dopri5/Tsit5 steps.
+//
+// For the weighted trapezoid I over the pushed points (t0 when saving the
+// start, then every accepted step), with the accepted step sizes frozen, the
+// kernel returns dI/dp_m for the plan's differentiated base parameters.  The
+// forward solve recorded a tape (step starts and sizes, and the state at the
+// start of every ckpt_every-th step).  Segments are processed last to first:
+// the segment is recomputed from its checkpoint, keeping the stage inputs
+// Y_i and the coefficient rows (values and parameter derivatives) in the
+// workgroup's workspace, then its steps are swept backwards:
+//   kappa_i = h (b_i lambda + sum_{j>i} a_ji mu_j),   mu_i = A(t_i)^T kappa_i,
+//   dI/dp  += kappa_i^T (dA(t_i)/dp) Y_i,             lambda <- lambda + sum_i mu_i + dI/dy_n,
+// with A^T applied through the plan's transposed term tables and dA/dp through
+// the derivative coefficient programs (the RHS is linear in rho).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LdsAdj {
+  double* pub;    // 2*n_pub: published stage input Y_i (forward publication indices)
+  double* kap;    // 2*S: published kappa_i by slot
+  double* selfc;  // 2*S folded self coefficients
+  double* wts;    // n integral weights per state
+  dcpx* coef;     // 7 rows x (KD + 1): [0] t0, [1..6] stage times
+  dcpx* slots;    // 7 x n_slots
+  double* red;    // LB_MAX_DIFF x 16 per-wave sums
+  dcpx* vm;       // 8 lanes x vm_depth
+  int* misc;
+};
+
+__device__ __forceinline__ LdsAdj carve_adj(const DevPlan& P) {
+  extern __shared__ double smem[];
+  LdsAdj L;
+  L.pub = smem;
+  L.kap = L.pub + 2 * (size_t)P.n_pub;
+  L.selfc = L.kap + 2 * (size_t)P.S;
+  L.wts = L.selfc + 2 * (size_t)P.S;
+  L.coef = reinterpret_cast<dcpx*>(L.wts + ((P.n + 1) & ~1));
+  L.slots = L.coef + 7 * (P.KD + 1);
+  L.red = reinterpret_cast<double*>(L.slots + 7 * P.n_slots);
+  L.vm = reinterpret_cast<dcpx*>(L.red + LB_MAX_DIFF * 16);
+  L.misc = reinterpret_cast<int*>(L.vm + 8 * P.vm_depth);
+  return L;
+}
+
+// Forward RHS for the recomputation: publish, barrier, self term + gathers
+// (tables read from global memory), barrier (the buffer is reused).
+template <int R>
+__device__ __forceinline__ void rhs_adj_fwd(const DevPlan& P, const LdsAdj& L, const Slots<R>& S,
+                                            const double (&xr)[R], const double (&xi)[R], double (&kr)[R],
+                                            double (&ki)[R], double* pubbuf, const dcpx* coef, int tid, int W) {
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (S.pub[r] >= 0) {
+      pubbuf[2 * S.pub[r]] = xr[r];
+      pubbuf[2 * S.pub[r] + 1] = xi[r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int blk = wave + (W >> 6) * r;
+    if (blk >= P.nblk) { kr[r] = 0.0; ki[r] = 0.0; continue; }
+    const int s = tid + W * r;
+    const double2 a = reinterpret_cast<const double2*>(L.selfc)[s];
+    double accr = fma(a.x, xr[r], -(a.y * xi[r]));
+    double acci = fma(a.x, xi[r], a.y * xr[r]);
+    gather_block(P.g_word, reinterpret_cast<const double2*>(P.g_f), cst(P.blk_row0)[blk], cst(P.blk_rows)[blk],
+                 lane, coef, pubbuf, accr, acci);
+    kr[r] = accr;
+    ki[r] = S.diag(r) ? 0.0 : acci;
+  }
+  __syncthreads();
+}
+
+// mu = A(t)^T kappa for the slots of this thread, and the parameter
+// contraction g_m += kappa^T (dA/dp_m) Y over the terms of its outputs.
+// kappa and Y are published before the call (kap by slot, pub by publication
+// index) and a barrier separates the publication from this call.
+template <int R>
+__device__ __forceinline__ void adjoint_stage(const DevPlan& P, const LdsAdj& L, const Slots<R>& S,
+                                              const double (&kr)[R], const double (&ki)[R],
+                                              const double (&yr)[R], const double (&yi)[R],
+                                              const dcpx* cf, double (&mr)[R], double (&mi)[R],
+                                              double (&g)[LB_MAX_DIFF], int tid, int W) {
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nd = P.n_diff;
+  const dcpx* dc = cf + P.K + 1;  // dc[k * nd + m] = d c_k / d p_m
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    mr[r] = 0.0;
+    mi[r] = 0.0;
+    const int blk = wave + (W >> 6) * r;
+    if (blk >= P.nblk) continue;
+    const int s = tid + W * r;
+    const bool diag = S.diag(r);
+    // conj(kappa_own)
+    const double ckr = kr[r], cki = -ki[r];
+    // self terms: transposed (conj(kappa) selfc) and their coefficient derivatives
+    const double2 a = reinterpret_cast<const double2*>(L.selfc)[s];
+    {
+      const double hr = ckr * a.x - cki * a.y, hi = ckr * a.y + cki * a.x;
+      mr[r] += hr;
+      if (!diag) mi[r] -= hi;
+    }
+    if (nd > 0) {
+      for (int j = 0; j < P.max_self; ++j) {
+        const size_t e = (size_t)s * P.max_self + j;
+        const int id = P.self_coef[e];
+        if (id == -2) break;
+        if (id < 0) continue;
+        const dcpx f = mk(P.self_f[2 * e], P.self_f[2 * e + 1]);
+        // Q = conj(kappa) f x_own
+        const dcpx x = mk(yr[r], diag ? 0.0 : yi[r]);
+        const dcpx q = cmul(mk(ckr, cki), cmul(f, x));
+        const bool cj = P.self_conj[e] != 0;
+        for (int m = 0; m < LB_MAX_DIFF; ++m) {
+          if (m >= nd) break;
+          const dcpx d = dc[id * nd + m];
+          g[m] += q.re * d.re - q.im * (cj ? -d.im : d.im);
+        }
+      }
+    }
+    // transposed gathers: outputs o reading this slot
+    const int trows = cst(P.t_blk_rows)[blk], trow0 = cst(P.t_blk_row0)[blk];
+    for (int j = 0; j < trows; ++j) {
+      const size_t idx = (size_t)(trow0 + j) * 64 + lane;
+      const uint32_t w = P.t_word[idx];
+      const double2 f = reinterpret_cast<const double2*>(P.t_f)[idx];
+      const uint32_t o = w & 0xFFFFFu;
+      const uint32_t cid = w >> 24;
+      dcpx c = cid == LB_STATIC_COEF ? mk(1.0, 0.0) : cf[cid];
+      if (w & (1u << 22)) c.im = -c.im;
+      // H = conj(kappa_o) f C
+      const dcpx ko = mk(L.kap[2 * o], -L.kap[2 * o + 1]);
+      const dcpx hh = cmul(ko, cmul(mk(f.x, f.y), c));
+      mr[r] += hh.re;
+      if (!(w & (1u << 20))) mi[r] += (w & (1u << 21)) ? hh.im : -hh.im;
+    }
+    // coefficient derivatives of the gather terms of this output
+    if (nd > 0) {
+      const int rows = cst(P.blk_rows)[blk], row0 = cst(P.blk_row0)[blk];
+      for (int j = 0; j < rows; ++j) {
+        const size_t idx = (size_t)(row0 + j) * 64 + lane;
+        const uint32_t w = P.g_word[idx];
+        const uint32_t cid = w >> 24;
+        if (cid == LB_STATIC_COEF) continue;
+        const double2 f = reinterpret_cast<const double2*>(P.g_f)[idx];
+        const uint32_t p = w & 0xFFFFFu;
+        double xr = L.pub[2 * p];
+        double xi = (w & (1u << 20)) ? 0.0 : L.pub[2 * p + 1];
+        if (w & (1u << 21)) xi = -xi;
+        const dcpx q = cmul(mk(ckr, cki), cmul(mk(f.x, f.y), mk(xr, xi)));
+        const bool cj = (w & (1u << 22)) != 0;
+        for (int m = 0; m < LB_MAX_DIFF; ++m) {
+          if (m >= nd) break;
+          const dcpx d = dc[cid * nd + m];
+          g[m] += q.re * d.re - q.im * (cj ? -d.im : d.im);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+template <int R, int SOLVER, int MAXW, int WAVES>
+__global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P, const DevCall C) {
+  using T = Tab<SOLVER>;
+  const int tid = threadIdx.x, W = blockDim.x;
+  const LdsAdj L = carve_adj(P);
+  const int KC = P.KD + 1;
+  Slots<R> S;
+  load_slots<R>(P, S, tid, W);
+  const int every = C.ckpt_every;
+  double* segY = C.seg_y + (size_t)blockIdx.x * every * 6 * P.S * 2;
+  dcpx* segC = reinterpret_cast<dcpx*>(C.seg_c) + (size_t)blockIdx.x * every * 6 * KC;
+  const int64_t ncap = (C.tape_cap + every - 1) / every;
+
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) L.misc[0] = atomicAdd(C.queue, 1);
+    __syncthreads();
+    const int64_t local = L.misc[0];
+    if (local >= C.n_traj) break;
+    const int64_t traj = local + C.traj_offset;
+
+    // ---- parameters, coefficients at t0 (self folds), weights
+    for (int i = tid; i < P.n_slots; i += W)
+      L.slots[i] = i < P.n_base ? mk(P.base[2 * i], P.base[2 * i + 1]) : mk(0.0, 0.0);
+    __syncthreads();
+    if (tid == 0) {
+      apply_overrides(C, traj, L.slots);
+      eval_coefficients(P, L.slots, L.coef, C.t0, false, L.vm, P.KD);
+    }
+    for (int i = tid; i < P.n; i += W) {
+      double w = 0.0;
+      for (int q = 0; q < C.n_weights; ++q)
+        if (C.weight_index[q] == i) w += C.weight_value[q];
+      L.wts[i] = w;
+    }
+    __syncthreads();
+    // stage rows start as the t0 row: the expression machine rewrites only the
+    // time-dependent entries at each stage time
+    for (int i = tid; i < 6 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
+    for (int i = tid; i < 6 * KC; i += W) L.coef[KC + i] = L.coef[i % KC];
+    fold_self(P, tid, W, L.coef, L.selfc);
+    __syncthreads();
+
+    double g[LB_MAX_DIFF];
+#pragma unroll
+    for (int m = 0; m < LB_MAX_DIFF; ++m) g[m] = 0.0;
+    const int N = C.tape_n[traj];
+    const bool ok = N > 0 && N <= C.tape_cap;
+    if (ok) {
+      const double* tt = C.tape_t + traj * C.tape_cap;
+      const double* th = C.tape_h + traj * C.tape_cap;
+      // dI/dy at pushed point n: w * (dt_{n-1} [point n-1 pushed] + dt_n [n < N]) / 2,
+      // dt_n the time between points n and n+1
+      auto dt_of = [&](int n) { return (n + 1 < N ? tt[n + 1] : tt[N - 1] + th[N - 1]) - tt[n]; };
+      auto direct = [&](int n) {
+        double d = 0.0;
+        if (n >= 2 || (n == 1 && C.save_start)) d += dt_of(n - 1);
+        if (n < N) d += dt_of(n);
+        return 0.5 * d;
+      };
+      double lam_r[R], lam_i[R];
+      {
+        const double dN = direct(N);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          lam_r[r] = (S.valid(r) && S.diag(r)) ? L.wts[S.state(r)] * dN : 0.0;
+          lam_i[r] = 0.0;
+        }
+      }
+      const int nseg = (N + every - 1) / every;
+      for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int n0 = sg * every, n1 = min(n0 + every, N);
+        // ---- recompute the segment from its checkpoint
+        double y_r[R], y_i[R];
+        {
+          const double* ck = C.tape_y + ((traj * ncap + sg) * P.S) * 2;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int s = tid + W * r;
+            y_r[r] = s < P.S ? ck[2 * s] : 0.0;
+            y_i[r] = s < P.S ? ck[2 * s + 1] : 0.0;
+          }
+        }
+        for (int n = n0; n < n1; ++n) {
+          const double x = tt[n], h = th[n];
+          __syncthreads();  // previous users of the coefficient rows are done
+          if (tid < 6) {
+            const int st = tid + 1;
+            const double ts = st == 1 ? x : x + h * T::c(st);
+            eval_coefficients(P, L.slots + st * P.n_slots, L.coef + st * KC, ts, true, L.vm + tid * P.vm_depth,
+                              P.KD);
+            for (int q = 0; q < KC; ++q) segC[((n - n0) * 6 + tid) * KC + q] = L.coef[st * KC + q];
+          }
+          __syncthreads();
+          double k_r[8][R], k_i[8][R], yt_r[R], yt_i[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int s = tid + W * r;
+            if (s < P.S) {
+              segY[(((n - n0) * 6 + 0) * P.S + s) * 2] = y_r[r];
+              segY[(((n - n0) * 6 + 0) * P.S + s) * 2 + 1] = y_i[r];
+            }
+          }
+          rhs_adj_fwd<R>(P, L, S, y_r, y_i, k_r[1], k_i[1], L.pub, L.coef + KC, tid, W);
+#define LB_ADJ_FWD(st)                                                                              \
+  stage_combo<R, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                                   \
+  _Pragma("unroll") for (int r = 0; r < R; ++r) {                                                  \
+    const int s = tid + W * r;                                                                     \
+    if (s < P.S) {                                                                                 \
+      segY[(((n - n0) * 6 + (st - 1)) * P.S + s) * 2] = yt_r[r];                                   \
+      segY[(((n - n0) * 6 + (st - 1)) * P.S + s) * 2 + 1] = yt_i[r];                               \
+    }                                                                                              \
+  }                                                                                                \
+  rhs_adj_fwd<R>(P, L, S, yt_r, yt_i, k_r[st], k_i[st], L.pub, L.coef + (st) * KC, tid, W);
+          LB_ADJ_FWD(2)
+          LB_ADJ_FWD(3)
+          LB_ADJ_FWD(4)
+          LB_ADJ_FWD(5)
+          LB_ADJ_FWD(6)
+#undef LB_ADJ_FWD
+          stage_combo<R, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            y_r[r] = yt_r[r];
+            y_i[r] = yt_i[r];
+          }
+        }
+        // ---- adjoint sweep over the segment, last step first
+        for (int n = n1 - 1; n >= n0; --n) {
+          const double h = th[n];
+          __syncthreads();
+          for (int q = tid; q < 6 * KC; q += W) L.coef[KC + q] = segC[(n - n0) * 6 * KC + q];
+          __syncthreads();
+          double mu_r[7][R], mu_i[7][R];
+#define LB_ADJ_STAGE(i)                                                                             \
+  {                                                                                                 \
+    double kr[R], ki[R], yr[R], yi[R];                                                              \
+    _Pragma("unroll") for (int r = 0; r < R; ++r) {                                                 \
+      double ar = T::a(7, i) * lam_r[r], ai = T::a(7, i) * lam_i[r];                                \
+      _Pragma("unroll") for (int j = i + 1; j <= 6; ++j) {                                          \
+        if (T::a(j, i) != 0.0) {                                                                    \
+          ar = fma(T::a(j, i), mu_r[j][r], ar);                                                     \
+          ai = fma(T::a(j, i), mu_i[j][r], ai);                                                     \
+        }                                                                                           \
+      }                                                                                             \
+      kr[r] = h * ar;                                                                               \
+      ki[r] = S.diag(r) ? 0.0 : h * ai;                                                             \
+      const int s = tid + W * r;                                                                    \
+      yr[r] = s < P.S ? segY[(((n - n0) * 6 + (i - 1)) * P.S + s) * 2] : 0.0;                       \
+      yi[r] = s < P.S ? segY[(((n - n0) * 6 + (i - 1)) * P.S + s) * 2 + 1] : 0.0;                   \
+      if (S.pub[r] >= 0) {                                                                          \
+        L.pub[2 * S.pub[r]] = yr[r];                                                                \
+        L.pub[2 * S.pub[r] + 1] = yi[r];                                                            \
+      }                                                                                             \
+      if (s < P.S) {                                                                                \
+        L.kap[2 * s] = kr[r];                                                                       \
+        L.kap[2 * s + 1] = ki[r];                                                                   \
+      }                                                                                             \
+    }                                                                                               \
+    __syncthreads();                                                                                \
+    adjoint_stage<R>(P, L, S, kr, ki, yr, yi, L.coef + (i) * KC, mu_r[i], mu_i[i], g, tid, W);     \
+    __syncthreads();                                                                                \
+  }
+          LB_ADJ_STAGE(6)
+          LB_ADJ_STAGE(5)
+          LB_ADJ_STAGE(4)
+          LB_ADJ_STAGE(3)
+          LB_ADJ_STAGE(2)
+          LB_ADJ_STAGE(1)
+#undef LB_ADJ_STAGE
+          const double dn = n >= 1 ? direct(n) : 0.0;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            double sr = lam_r[r], si = lam_i[r];
+#pragma unroll
+            for (int i = 1; i <= 6; ++i) {
+              sr += mu_r[i][r];
+              si += mu_i[i][r];
+            }
+            lam_r[r] = sr + ((S.valid(r) && S.diag(r)) ? L.wts[S.state(r)] * dn : 0.0);
+            lam_i[r] = S.diag(r) ? 0.0 : si;
+          }
+        }
+      }
+    }
+    // ---- dI/dp: one block reduction per parameter
+    const int nd = P.n_diff;
+#pragma unroll
+    for (int m = 0; m < LB_MAX_DIFF; ++m) {
+      if (m >= nd) break;
+      const double v = wave_sum(g[m]);
+      if ((tid & 63) == 0) L.red[m * 16 + (tid >> 6)] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int nw = W >> 6;
+      for (int m = 0; m < nd; ++m) {
+        double tot = 0.0;
+        for (int b = 0; b < nw; ++b) tot += L.red[m * 16 + b];
+        C.out_grad[traj * nd + m] = ok ? tot : __builtin_nan("");
+      }
     }
   }
 }
@@ -1170,6 +1584,13 @@ void* solve_ptr(int solver) {
   else
     return nullptr;
 }
+template <int I, int RR, int WW, int VV>
+void* adjoint_ptr(int solver) {
+  if constexpr (I % LB_NPART == LB_PART)
+    return solver == 1 ? (void*)&lb_adjoint_kernel<RR, 1, WW, VV> : (void*)&lb_adjoint_kernel<RR, 0, WW, VV>;
+  else
+    return nullptr;
+}
 template <int I, int GG, int VV>
 void* wave_solve_ptr(int solver) {
   if constexpr (I % LB_NPART == LB_PART)
@@ -1202,6 +1623,14 @@ extern "C" void* LB_PART_FN(lb_dev_solve_kernel)(int id, int solver) {
   if (id == i) return wave_solve_ptr<i, g, v>(solver);
   LB_WAVE_CONFIGS(Y)
 #undef Y
+  return nullptr;
+}
+
+extern "C" void* LB_PART_FN(lb_dev_adjoint_kernel)(int id, int solver) {
+#define Z(i, r, w, v) \
+  if (id == i) return adjoint_ptr<i, r, w, v>(solver);
+  LB_ADJ_CONFIGS(Z)
+#undef Z
   return nullptr;
 }
 

@@ -11,4 +11,7 @@
 // held in registers, waves-per-SIMD hint); one slot per lane, 64 threads.
 #define LB_WAVE_CONFIGS(Y) Y(11, 8, 4) Y(12, 8, 2) Y(13, 16, 2)
 
+// Adjoint kernel variants: Z(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
+#define LB_ADJ_CONFIGS(Z) Z(0, 1, 1024, 2) Z(1, 2, 512, 2) Z(2, 4, 256, 2) Z(3, 8, 256, 1)
+
 #define LB_NPARTS_BUILT 4

@@ -95,6 +95,13 @@ typedef struct lb_plan_desc {
   const uint8_t* input_conj;
   const double* factors_re;
   const double* factors_im;
+  /* Differentiable plans (lb_adjoint): n_diff_params base slots p_m.  The
+   * expression list then holds, after the n_coefficients coefficient
+   * programs, n_coefficients * n_diff_params derivative programs
+   * (k * n_diff_params + m is d c_k / d p_m); compounds may include
+   * derivative compounds they read.  0 for an ordinary plan. */
+  int32_t n_diff_params;
+  const int32_t* diff_param_slots;
 } lb_plan_desc;
 
 typedef struct lb_plan lb_plan;
@@ -213,6 +220,23 @@ typedef struct lb_solve_args {
                                      support through the RHS program (all others
                                      stay exactly zero) and reports
                                      LB_TRAJ_BAD_SUPPORT for a y0 outside it. */
+  /* Adjoint tape: lb_solve records it when tape_h is not NULL (the start time
+   * and size of every accepted step, and the state at the start of every
+   * ckpt_every-th step); lb_adjoint reads it.  All device arrays:
+   * tape_t, tape_h n_traj x tape_cap; tape_y n_traj x ceil(tape_cap /
+   * ckpt_every) x slots x 2 (lb_plan_support_info field 9 gives the slots);
+   * tape_n n_traj.  More accepted steps than tape_cap: LB_TRAJ_CAPACITY. */
+  double* tape_t;
+  double* tape_h;
+  double* tape_y;
+  int32_t* tape_n;
+  int64_t tape_cap;
+  int32_t ckpt_every;
+  double* out_grad;               /* lb_adjoint: device, n_traj x n_diff_params */
+  const double* replay_h;         /* device, n_traj x tape_cap, or NULL: integrate with these
+                                     accepted step sizes (a recorded tape) instead of the
+                                     step-size controller, for checks of the adjoint */
+  const int32_t* replay_n;        /* device, n_traj: steps of each schedule */
 } lb_solve_args;
 
 /*
@@ -223,6 +247,19 @@ typedef struct lb_solve_args {
  * Stream-ordered on `stream` (a hipStream_t, NULL = default stream).
  */
 int lb_solve(const lb_plan* plan, const lb_solve_args* args, void* stream);
+
+/*
+ * Discrete adjoint of a recorded solve (no reference counterpart: the
+ * reference has no gradients).  For the weighted trapezoid integral of each
+ * trajectory (output_kind LB_OUT_INTEGRAL, pushes at t0 when save_start and
+ * at every accepted step, no saveat), with the accepted step sizes of the
+ * tape held fixed, writes out_grad[b, m] = d I_b / d p_m for the plan's
+ * differentiated parameters (lb_plan_desc.n_diff_params).  `args` must
+ * describe the same trajectories, parameters, y0 support and tape as the
+ * lb_solve that recorded the tape.  The library keeps a per-plan workspace
+ * of ckpt_every recomputed steps per resident workgroup.
+ */
+int lb_adjoint(const lb_plan* plan, const lb_solve_args* args, void* stream);
 
 /*
  * Kernel variant `id` (0-based): slots per thread, largest workgroup and the

@@ -7,6 +7,9 @@ builders and committed as plain JSON:
 
 * ``tlf_r0_f1_1o2_f1_n39.json``: X(J=0,1,2) + B(J=1, F1'=1/2, F'=1), one
   z-polarised laser on R(0) F1'=1/2 F'=1 (n = 39).
+* ``tlf_r0_f1_1o2_f1_xy_n39.json``: the same n = 39 levels with the laser's
+  polarisation split into x and y components (symbols PX0, PY0), the system
+  the gradient workload differentiates with respect to the polarisation angle.
 * ``tlf_r0_f1_3o2_n72.json``: X(J=0..3) + B(J=1, F1'=3/2, F'=1,2), two lasers
   on R(0) F1'=3/2 F'=2 and F'=1 with x/y polarisation switching, plus a
   J=1 -> J=2 microwave (n = 72).
@@ -19,7 +22,8 @@ and ``centrex_tlf/couplings/transition.py:110``
 (``generate_transition_selectors``); the decay rate is passed explicitly as
 Gamma = 2*pi*1.56 MHz.
 
-Run from the repository root:  ``python tests/golden/make_tlf_systems.py``
+Run from the repository root:  ``python tests/golden/make_tlf_systems.py [name ...]``
+(names: n39, xy39, n72; default all).
 """
 
 from __future__ import annotations
@@ -87,6 +91,27 @@ def main() -> None:
 
     X = states.ElectronicState.X
     B = states.ElectronicState.B
+    wanted = set(sys.argv[1:]) or {"n39", "xy39", "n72"}
+
+    if "xy39" in wanted:
+        t0 = time.time()
+        trans = [transitions.R0_F1_1o2_F1]
+        selectors = couplings.generate_transition_selectors(
+            trans, [[couplings.polarization_X, couplings.polarization_Y]])
+        obe = lindblad.generate_OBE_system(
+            states.QuantumSelector(J=[0, 1, 2], electronic=X),
+            states.QuantumSelector(J=1, F1=1 / 2, F=1, P=trans[0].P_excited, electronic=B),
+            selectors,
+            Γ=GAMMA,
+        )
+        _dump_system(
+            obe,
+            OUT_DIR / "tlf_r0_f1_1o2_f1_xy_n39.json",
+            "X(J=0,1,2) + B(J=1,F1'=1/2,F'=1); R0_F1_1o2_F1, x and y polarisation components",
+        )
+        print(f"xy n39 built in {time.time() - t0:.1f} s", flush=True)
+    if "n39" not in wanted and "n72" not in wanted:
+        return
 
     t0 = time.time()
     trans = [transitions.R0_F1_1o2_F1]
