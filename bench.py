@@ -63,6 +63,123 @@ def algorithmic_flops(stats, flops_per_rhs: int, dim: int, solver: str) -> float
     return float(np.sum(rhs * flops_per_rhs + attempted * dim * F_RK[solver]))
 
 
+def adjoint_leg(steps: int, warmup: int) -> dict:
+    """Gradients of the photon-count loss on 256 n=39 trajectories.
+
+    System: the n=39 set with the laser polarisation split into x and y
+    (PX = cos theta, PY = sin theta), Gaussian envelope of width sigma.
+    Parameters per trajectory (seeded): delta0 ~ U[-3, 3] Gamma, Omega0 ~
+    U[0.3, 3] Gamma, theta ~ U[0, pi], sigma ~ U[2, 6] / Gamma; targets ~
+    U[0, 1].  Loss sum_b (N_b - target_b)^2, Dopri5 rtol 1e-8 atol 1e-10,
+    checkpoints every 16 accepted steps.  One step = forward with tape +
+    adjoint backward through ``torch.autograd``; the forward alone is the
+    ordinary batched photon-count solve of the same trajectories.
+    """
+    import numpy as np
+    import torch
+
+    from centrex_tlf_amd.lindblad import prepare_lindblad_problem
+    from centrex_tlf_amd.lindblad.adjoint import DifferentiablePhotonCount, photon_count
+    from tests.systems import GAMMA, tlf_n39_xy
+
+    names = ["delta0", "Omega0", "theta", "sigma"]
+    tol = dict(abstol=1e-10, reltol=1e-8, dt=1e-8)
+    n = 256
+    rng = np.random.default_rng(20261202 + 4)
+    p_np = np.stack([rng.uniform(-3, 3, n) * GAMMA, rng.uniform(0.3, 3, n) * GAMMA, rng.uniform(0, np.pi, n),
+                     rng.uniform(2, 6, n) / GAMMA], axis=1)
+    dev = torch.device("cuda", 0)
+    target = torch.as_tensor(rng.uniform(0, 1, n), device=dev)
+    system, params, rho0, t_span, weights = tlf_n39_xy()
+    prepared = prepare_lindblad_problem(system, params, backend="gpu")
+    engine = DifferentiablePhotonCount(prepared, names, weights, t_span, ckpt_every=16, **tol)
+    y0 = torch.as_tensor(prepared.layout.pack(rho0), device=dev)
+    p = torch.as_tensor(p_np, device=dev)
+    plan = prepared.device_plan()
+    slots = [prepared.slot_names.index(m) for m in names]
+    pv = p.to(torch.complex128)
+
+    def forward_only(events=None):
+        return plan.solve(y0, *t_span, output_kind="integral", weights=weights, override_slots=slots,
+                          override_values=pv, events=events, **tol)
+
+    def grad_step():
+        q = p.clone().requires_grad_(True)
+        counts = photon_count(engine, q, y0)
+        loss = ((counts - target) ** 2).sum()
+        loss.backward()
+        return counts, q.grad
+
+    def timed(fn):
+        for _ in range(max(1, warmup)):
+            out = fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps, out
+
+    def kernel_ms(fn):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for e in ev:
+            fn(e)
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    fwd_s, (counts_ref, stats, _) = timed(forward_only)
+    grad_s, (counts, grad) = timed(grad_step)
+    # parts of one gradient step, timed apart
+    tape_s, (_, tape) = timed(lambda: engine.forward(p, y0))
+    bwd_s, _ = timed(lambda: engine.gradient(p, y0, tape))
+    # the kernels alone (HIP events around each launch)
+    fwd_kms = kernel_ms(lambda e: forward_only(events=e))
+    tape_kms = kernel_ms(lambda e: engine.forward(p, y0, events=e))
+    bwd_kms = kernel_ms(lambda e: engine.gradient(p, y0, tape, events=e))
+    # frozen-step check: the adjoint against central differences (h = 1e-6 |p|)
+    # of a replay of the recorded step sizes, on 8 trajectories
+    k = 8
+    pk = p_np[:k]
+    _, tape_k = engine.forward(pk, y0)
+    gk = engine.gradient(pk, y0, tape_k).cpu().numpy()
+    fd = np.zeros_like(pk)
+    for m in range(4):
+        h = 1e-6 * np.maximum(np.abs(pk[:, m]), 1e-3 * np.abs(p_np[:, m]).max())
+        hi, lo = pk.copy(), pk.copy()
+        hi[:, m] += h
+        lo[:, m] -= h
+        fd[:, m] = (engine.replay(hi, y0, tape_k).cpu().numpy() - engine.replay(lo, y0, tape_k).cpu().numpy()) / (2 * h)
+    ck = engine.replay(pk, y0, tape_k).cpu().numpy()
+    err = np.abs(pk * (gk - fd)).max(axis=0) / np.abs(ck).max()
+    st = stats.cpu().numpy()
+    log(f"bench: adjoint {n / grad_s:.1f} grads/s; forward {fwd_s * 1e3:.2f} ms, "
+        f"tape forward {tape_s * 1e3:.2f} ms, backward {bwd_s * 1e3:.2f} ms; kernels {fwd_kms:.2f} / "
+        f"{tape_kms:.2f} / {bwd_kms:.2f} ms")
+    return {
+        "metric": "adjoint grads/s",
+        "value": n / grad_s,
+        "unit": "full gradients (4 parameters) per second",
+        "trajectories": n,
+        "parameters": names,
+        "ms_per_step": grad_s * 1e3,
+        "forward_ms": fwd_s * 1e3,
+        "tape_forward_ms": tape_s * 1e3,
+        "backward_ms": bwd_s * 1e3,
+        "backward_over_forward": bwd_s / fwd_s,
+        "gradient_step_over_forward": grad_s / fwd_s,
+        "kernel_ms": {"forward": fwd_kms, "forward_with_tape": tape_kms, "adjoint": bwd_kms},
+        "kernel_adjoint_over_forward": bwd_kms / fwd_kms,
+        "kernel_gradient_over_forward": (tape_kms + bwd_kms) / fwd_kms,
+        "accepted_steps_min_max": [int(st[:, 0].min()), int(st[:, 0].max())],
+        "checkpoint_every": 16,
+        "tape_bytes": int(tape.nbytes),
+        "counts_max_abs_diff_vs_forward": float((counts.detach() - counts_ref[:, 0]).abs().max().item()),
+        "frozen_step_fd_error": [float(e) for e in err],
+        "fd_error_definition": "max_b |p_m (adjoint - replay FD)| / max_b N_b, 8 trajectories",
+        "config": "tlf_n39_xy, 256 seeded trajectories, Dopri5 rtol 1e-8 atol 1e-10",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +195,9 @@ def main() -> int:
     ap.add_argument("--full-plan", action="store_true", help="integrate every element (no restriction)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="launch eagerly instead of replaying a HIP graph of one step")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
+                    help="skip the adjoint-gradient leg")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint-gradient leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -105,6 +225,9 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "launches": 5}))
+        return 0
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(args.steps, args.warmup)))
         return 0
     system, params, rho0, t_span, weights = tlf_n39()
     prepared = prepare_lindblad_problem(system, params, backend="gpu")
@@ -294,6 +417,8 @@ def main() -> int:
         "accepted_steps_equal": bool(np.array_equal(gpu_steps, ref["accepted"])),
         "bar": "1e-7 absolute, trace 1e-10",
     }
+    if args.adjoint:
+        result["adjoint"] = adjoint_leg(args.steps, args.warmup)
     print(json.dumps(result))
     return 0
 

@@ -65,8 +65,9 @@ class DifferentiablePhotonCount:
         slots = self.plan.support_info(support)["slot_count"]
         return p, y, support, slots
 
-    def forward(self, params, y0):
-        """Photon counts (B,) and the tape of the solve."""
+    def forward(self, params, y0, events=None):
+        """Photon counts (B,) and the tape of the solve (``events``: a (start, end)
+        pair of torch events recorded around the solver launch)."""
         import torch
 
         p, y, support, slots = self._common(params, y0)
@@ -75,7 +76,7 @@ class DifferentiablePhotonCount:
             tape = Tape(p.shape[0], slots, cap, self.every)
             vals, stats, fail_t = self.plan.solve(
                 y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self.slots,
-                override_values=p.to(torch.complex128), support=support, tape=tape, **self.kw)
+                override_values=p.to(torch.complex128), support=support, tape=tape, events=events, **self.kw)
             st = stats[:, 3].cpu().numpy()
             if np.any(st == 7):
                 cap = int(tape.n.max().item()) + 1
@@ -100,15 +101,19 @@ class DifferentiablePhotonCount:
             raise ValueError("replay of the recorded step sizes failed")
         return vals[:, 0]
 
-    def gradient(self, params, y0, tape):
-        """dI_b/dp_b (B, P) for a tape recorded by ``forward`` with the same params."""
+    def gradient(self, params, y0, tape, variant: int = 0, events=None):
+        """dI_b/dp_b (B, P) for a tape recorded by ``forward`` with the same params.
+
+        ``variant`` = k + 1 forces adjoint kernel variant k (0: the library's choice).
+        """
         import torch
 
         p, y, support, _ = self._common(params, y0)
         grad = torch.empty((p.shape[0], len(self.slots)), dtype=torch.float64, device="cuda")
         return self.plan.solve(
             y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self.slots,
-            override_values=p.to(torch.complex128), support=support, tape=tape, adjoint_grad=grad, **self.kw)
+            override_values=p.to(torch.complex128), support=support, tape=tape, adjoint_grad=grad,
+            variant=int(variant), events=events, **self.kw)
 
 
 def _autograd_function():

@@ -24,7 +24,7 @@
 extern "C" int lb_dev_variant_count(void);
 extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
 extern "C" int lb_dev_variant_rows(int id);
-extern "C" int lb_dev_adjoint_variant(int id, int* R, int* maxw);
+extern "C" int lb_dev_adjoint_variant(int id, int* R, int* maxw, int* rows);
 extern "C" void* lb_dev_adjoint_kernel(int id, int solver);
 extern "C" void* lb_dev_solve_kernel(int id, int solver);
 extern "C" void* lb_dev_rhs_kernel(int id);
@@ -77,6 +77,8 @@ struct DevImage {
   void* ws_y = nullptr;  // adjoint workspace (grow-only)
   void* ws_c = nullptr;
   size_t ws_y_len = 0, ws_c_len = 0;
+  std::vector<int32_t> g_rows_host;  // gather rows per block
+  int t_rows_host = 0;               // transposed rows of block 0 (one-block plans)
 };
 
 struct lb_plan {
@@ -192,6 +194,8 @@ void pick_config(int S, int rows, int* id_out, int* W_out) {
 
 // Gather tables up to this size are copied into LDS by every workgroup.
 constexpr int64_t kLdsTableBytes = 32 * 1024;
+// LDS of one workgroup on gfx950.
+constexpr int64_t kMaxLdsBytes = 160 * 1024;
 
 // Must match carve() in lb_kernels.hip.
 int64_t lds_bytes_for(const DevPlan& P) {
@@ -420,6 +424,8 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   im->lds_bytes = lds_bytes_for(P);
   im->n_pub = n_pub;
   im->rows_total = rows_total;
+  im->g_rows_host = blk_rows;
+  im->t_rows_host = t_rows.empty() ? 0 : t_rows[0];
   im->flops = flops;
   im->active = n_act;
   im->active_mask = act;
@@ -880,6 +886,15 @@ int64_t lds_bytes_adj(const DevPlan& P) {
          8 * (int64_t)LB_MAX_DIFF * 16 + 16 * 8 * (int64_t)P.vm_depth + 16;
 }
 
+// LDS of the single-wave adjoint (carve_aw in lb_kernels.hip).
+int64_t lds_bytes_adj_wave(const DevPlan& P, const std::vector<int32_t>& g_rows, int t_rows, int every) {
+  auto odd = [](int64_t v) { return v | 1; };
+  const int64_t g = g_rows.empty() ? 0 : g_rows[0], KC = P.KD + 1;
+  return g * 64 * 16 + (int64_t)t_rows * 64 * 16 + KC * 16 + (int64_t)every * 6 * odd(KC) * 16 +
+         (int64_t)P.n_slots * 16 + 64 * odd(P.n_slots) * 16 + 64 * odd(P.vm_depth) * 16 + 64 * 16 +
+         (int64_t)((P.n + 1) & ~1) * 8 + g * 64 * 4 + (int64_t)t_rows * 64 * 4 + 16;
+}
+
 }  // namespace
 
 extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream) {
@@ -895,16 +910,40 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   int rc = fill_call(p, a, &im, &c);
   if (rc) return rc;
   c.out_grad = a->out_grad;
-  // the adjoint variant with the fewest slots per thread that covers the plan
+  if (a->ckpt_every <= 0) return fail(LB_ERR_INVALID, "ckpt_every must be positive");
+  // A plan of one 64-slot block whose gather rows and LDS fit takes the
+  // single-wave adjoint; otherwise the multi-wave variant with the fewest
+  // slots per thread that covers the plan.  args->variant = k + 1 forces
+  // adjoint variant k (both forms compute the same derivative).
   const int S = im->dev.S;
-  int id = -1, W = 0, R = 0, maxw = 0;
-  for (int v = 0; lb_dev_adjoint_variant(v, &R, &maxw); ++v) {
-    const int w = std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
-    if (w <= maxw) { id = v; W = w; break; }
+  const int64_t lds_wave = lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every);
+  auto wave_fits = [&](int rows) {
+    return im->dev.nblk == 1 && !im->g_rows_host.empty() && im->g_rows_host[0] <= rows &&
+           lds_wave <= kMaxLdsBytes;
+  };
+  int id = -1, W = 0, R = 0, maxw = 0, rows = 0;
+  bool wave = false;
+  if (a->variant > 0) {
+    if (!lb_dev_adjoint_variant(a->variant - 1, &R, &maxw, &rows))
+      return fail(LB_ERR_INVALID, "no such adjoint variant");
+    wave = rows > 0;
+    W = wave ? 64 : std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
+    if (wave ? !wave_fits(rows) : W > maxw)
+      return fail(LB_ERR_INVALID, "the adjoint variant does not cover this plan");
+    id = a->variant - 1;
+  } else {
+    for (int v = 0; lb_dev_adjoint_variant(v, &R, &maxw, &rows); ++v)
+      if (rows > 0 && wave_fits(rows)) { id = v; W = 64; wave = true; break; }
+    for (int v = 0; id < 0 && lb_dev_adjoint_variant(v, &R, &maxw, &rows); ++v) {
+      if (rows > 0) continue;
+      const int w = std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
+      if (w <= maxw) { id = v; W = w; }
+    }
   }
   if (id < 0) return fail(LB_ERR_UNSUPPORTED, "plan is too large for the adjoint kernel variants");
   void* fn = lb_dev_adjoint_kernel(id, a->solver);
-  const size_t lds = (size_t)lds_bytes_adj(im->dev);
+  if (!fn) return fail(LB_ERR_UNSUPPORTED, "adjoint kernel variant is not built");
+  const size_t lds = (size_t)(wave ? lds_wave : lds_bytes_adj(im->dev));
   if ((rc = set_lds_limit(fn, lds))) return rc;
   int grid = a->workgroups;
   if (grid <= 0) {

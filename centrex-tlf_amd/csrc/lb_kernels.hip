@@ -1516,6 +1516,312 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
 }
 
 // ---------------------------------------------------------------------------
+// Single-wave adjoint for plans of one 64-slot block (lane = slot).
+//
+// The same discrete adjoint as lb_adjoint_kernel, arranged for latency: the
+// gather and transposed term tables live in LDS for the whole launch; the
+// coefficient rows of a checkpoint segment (every stage time of its steps is
+// on the tape) are evaluated in one pass, one (step, stage) per lane, each
+// lane with a private copy of the parameter slots and its own expression
+// stack; the recomputation uses the register/cross-lane RHS of the
+// single-wave solver (the same arithmetic, so the stage inputs are those of
+// the recorded solve); the reverse stages read kappa of other elements by
+// cross-lane permutes and fold the parameter contraction of every term into
+// the transposed pass.  The self terms of an element (time-independent
+// coefficients) and their parameter derivatives are folded once per
+// trajectory.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LdsAW {
+  double2* gf;     // g_rows x 64 gather factors
+  double2* tf;     // t_rows x 64 transposed factors
+  dcpx* coef0;     // KC: coefficient row at t0 (time-independent entries)
+  dcpx* rows;      // every * 6 rows of stride KCP: coefficient rows of a segment
+  dcpx* slots0;    // n_slots: parameter slots at t0
+  dcpx* lslots;    // 64 x SSP: per-lane parameter slots
+  dcpx* lvm;       // 64 x VSP: per-lane expression stacks
+  double2* selfc;  // 64 folded self coefficients
+  double* wts;     // n integral weights
+  uint32_t* gw;    // g_rows x 64 gather words
+  uint32_t* tw;    // t_rows x 64 transposed words
+  int* misc;
+};
+
+// Odd strides (in 16-byte units) spread the lanes' rows over the LDS banks.
+__device__ __forceinline__ int odd_up(int v) { return v | 1; }
+
+__device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
+  extern __shared__ double smem[];
+  const int g = cst(P.blk_rows)[0], t = cst(P.t_blk_rows)[0], KC = P.KD + 1;
+  char* q = reinterpret_cast<char*>(smem);
+  LdsAW L;
+  L.gf = reinterpret_cast<double2*>(q); q += (size_t)g * 64 * 16;
+  L.tf = reinterpret_cast<double2*>(q); q += (size_t)t * 64 * 16;
+  L.coef0 = reinterpret_cast<dcpx*>(q); q += (size_t)KC * 16;
+  L.rows = reinterpret_cast<dcpx*>(q); q += (size_t)every * 6 * odd_up(KC) * 16;
+  L.slots0 = reinterpret_cast<dcpx*>(q); q += (size_t)P.n_slots * 16;
+  L.lslots = reinterpret_cast<dcpx*>(q); q += (size_t)64 * odd_up(P.n_slots) * 16;
+  L.lvm = reinterpret_cast<dcpx*>(q); q += (size_t)64 * odd_up(P.vm_depth) * 16;
+  L.selfc = reinterpret_cast<double2*>(q); q += (size_t)64 * 16;
+  L.wts = reinterpret_cast<double*>(q); q += (size_t)((P.n + 1) & ~1) * 8;
+  L.gw = reinterpret_cast<uint32_t*>(q); q += (size_t)g * 64 * 4;
+  L.tw = reinterpret_cast<uint32_t*>(q); q += (size_t)t * 64 * 4;
+  L.misc = reinterpret_cast<int*>(q);
+  return L;
+}
+
+// One reverse stage: mu = A(t_i)^T kappa for this lane's element and
+// g_m += Re(conj(kappa)^T (dA(t_i)/dp_m) Y_i).  `cf` is the stage's
+// coefficient row (values, the constant 1 at K, then d c_k / d p_m at
+// K + 1 + k * nd + m).
+__device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW& L, int trows, int lane,
+                                                   double kr, double ki, double yr, double yi, bool diag,
+                                                   double2 sv, const dcpx (&selfd)[LB_MAX_DIFF],
+                                                   const dcpx* cf, double& mr, double& mi,
+                                                   double (&g)[LB_MAX_DIFF]) {
+  const int nd = P.n_diff;
+  const dcpx* dc = cf + P.K + 1;
+  const double ckr = kr, cki = -ki;  // conj(kappa)
+  mr = ckr * sv.x - cki * sv.y;
+  mi = diag ? 0.0 : -(ckr * sv.y + cki * sv.x);
+  {
+    const dcpx cx = cmul(mk(ckr, cki), mk(yr, diag ? 0.0 : yi));
+#pragma unroll
+    for (int m = 0; m < LB_MAX_DIFF; ++m) {
+      if (m >= nd) break;
+      g[m] += cx.re * selfd[m].re - cx.im * selfd[m].im;
+    }
+  }
+  constexpr int CH = 8;
+  for (int c0 = 0; c0 < trows; c0 += CH) {
+    uint32_t w[CH];
+    double2 f[CH];
+    double kor[CH], koi[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      if (c0 + j < trows) {
+        w[j] = L.tw[(c0 + j) * 64 + lane];
+        f[j] = L.tf[(c0 + j) * 64 + lane];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      if (c0 + j < trows) {
+        const int o = (int)(w[j] & 63u);
+        kor[j] = __shfl(kr, o, 64);
+        koi[j] = __shfl(ki, o, 64);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      if (c0 + j >= trows) continue;
+      const uint32_t cid = w[j] >> 24;
+      const bool cj = (w[j] & (1u << 22)) != 0;
+      // conj(kappa_o) f, then times the coefficient
+      const dcpx fo = cmul(mk(kor[j], -koi[j]), mk(f[j].x, f[j].y));
+      dcpx c = cid == LB_STATIC_COEF ? mk(1.0, 0.0) : cf[cid];
+      if (cj) c.im = -c.im;
+      const dcpx hh = cmul(fo, c);
+      mr += hh.re;
+      if (!(w[j] & (1u << 20))) mi += (w[j] & (1u << 21)) ? hh.im : -hh.im;
+      if (cid != LB_STATIC_COEF) {
+        double xi = (w[j] & (1u << 20)) ? 0.0 : yi;
+        if (w[j] & (1u << 21)) xi = -xi;
+        const dcpx qx = cmul(fo, mk(yr, xi));
+        for (int m = 0; m < LB_MAX_DIFF; ++m) {
+          if (m >= nd) break;
+          const dcpx d = dc[cid * nd + m];
+          g[m] += qx.re * d.re - qx.im * (cj ? -d.im : d.im);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+template <int SOLVER, int G, int WAVES>
+__global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPlan P, const DevCall C) {
+  using T = Tab<SOLVER>;
+  const int lane = threadIdx.x;
+  const int every = C.ckpt_every;
+  const LdsAW L = carve_aw(P, every);
+  const int KC = P.KD + 1, KCP = odd_up(KC), nd = P.n_diff;
+  const int grows = cst(P.blk_rows)[0], trows = cst(P.t_blk_rows)[0];
+  Slots<1> S;
+  load_slots<1>(P, S, lane, 64);
+  const bool diag = S.diag(0);
+  for (int i = lane; i < grows * 64; i += 64) {
+    L.gf[i] = reinterpret_cast<const double2*>(P.g_f)[i];
+    L.gw[i] = P.g_word[i];
+  }
+  for (int i = lane; i < trows * 64; i += 64) {
+    L.tf[i] = reinterpret_cast<const double2*>(P.t_f)[i];
+    L.tw[i] = P.t_word[i];
+  }
+  Lds LV{};  // the gather tables as the single-wave RHS reads them
+  LV.gw = L.gw;
+  LV.gf = L.gf;
+  dcpx* const myslots = L.lslots + lane * odd_up(P.n_slots);
+  dcpx* const myvm = L.lvm + lane * odd_up(P.vm_depth);
+  double* const segY = C.seg_y + (size_t)blockIdx.x * every * 6 * 64 * 2;
+  const int64_t ncap = (C.tape_cap + every - 1) / every;
+
+  for (;;) {
+    __syncthreads();
+    if (lane == 0) L.misc[0] = atomicAdd(C.queue, 1);
+    __syncthreads();
+    const int64_t local = L.misc[0];
+    if (local >= C.n_traj) break;
+    const int64_t traj = local + C.traj_offset;
+
+    // ---- parameters and coefficients at t0, weights
+    for (int i = lane; i < P.n_slots; i += 64)
+      L.slots0[i] = i < P.n_base ? mk(P.base[2 * i], P.base[2 * i + 1]) : mk(0.0, 0.0);
+    __syncthreads();
+    if (lane == 0) {
+      apply_overrides(C, traj, L.slots0);
+      eval_coefficients(P, L.slots0, L.coef0, C.t0, false, myvm, P.KD);
+    }
+    for (int i = lane; i < P.n; i += 64) {
+      double w = 0.0;
+      for (int q = 0; q < C.n_weights; ++q)
+        if (C.weight_index[q] == i) w += C.weight_value[q];
+      L.wts[i] = w;
+    }
+    __syncthreads();
+    for (int i = 0; i < P.n_slots; ++i) myslots[i] = L.slots0[i];
+    fold_self(P, lane, 64, L.coef0, reinterpret_cast<double*>(L.selfc));
+    // parameter derivatives of the folded self coefficient
+    dcpx selfd[LB_MAX_DIFF];
+#pragma unroll
+    for (int m = 0; m < LB_MAX_DIFF; ++m) selfd[m] = mk(0.0, 0.0);
+    for (int j = 0; j < P.max_self; ++j) {
+      const size_t e = (size_t)lane * P.max_self + j;
+      const int id = P.self_coef[e];
+      if (id == -2) break;
+      if (id < 0) continue;
+      const dcpx f = mk(P.self_f[2 * e], P.self_f[2 * e + 1]);
+      const bool cj = P.self_conj[e] != 0;
+      for (int m = 0; m < LB_MAX_DIFF; ++m) {
+        if (m >= nd) break;
+        dcpx d = L.coef0[P.K + 1 + id * nd + m];
+        if (cj) d.im = -d.im;
+        selfd[m] = cadd(selfd[m], cmul(f, d));
+      }
+    }
+    __syncthreads();
+    const double2 selfv = L.selfc[lane];
+
+    double g[LB_MAX_DIFF];
+#pragma unroll
+    for (int m = 0; m < LB_MAX_DIFF; ++m) g[m] = 0.0;
+    const int N = C.tape_n[traj];
+    const bool ok = N > 0 && N <= C.tape_cap;
+    if (ok) {
+      const double* tt = C.tape_t + traj * C.tape_cap;
+      const double* th = C.tape_h + traj * C.tape_cap;
+      auto dt_of = [&](int n) { return (n + 1 < N ? tt[n + 1] : tt[N - 1] + th[N - 1]) - tt[n]; };
+      auto direct = [&](int n) {
+        double d = 0.0;
+        if (n >= 2 || (n == 1 && C.save_start)) d += dt_of(n - 1);
+        if (n < N) d += dt_of(n);
+        return 0.5 * d;
+      };
+      const double wme = (S.valid(0) && diag) ? L.wts[S.state(0)] : 0.0;
+      double lam_r = wme * direct(N), lam_i = 0.0;
+      const int nseg = (N + every - 1) / every;
+      for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int n0 = sg * every, n1 = min(n0 + every, N), ntask = 6 * (n1 - n0);
+        // ---- coefficient rows of the segment: task q = step (q / 6), stage (q % 6 + 1)
+        __syncthreads();  // the previous segment's rows are no longer read
+        for (int q = lane; q < ntask; q += 64) {
+          const int n = n0 + q / 6, st = q % 6 + 1;
+          const double ts = st == 1 ? tt[n] : tt[n] + th[n] * T::c(st);
+          dcpx* row = L.rows + q * KCP;
+          for (int j = 0; j < KC; ++j) row[j] = L.coef0[j];
+          if (P.time_dependent) eval_coefficients(P, myslots, row, ts, true, myvm, P.KD);
+        }
+        __syncthreads();
+        // ---- recompute the segment from its checkpoint, storing the stage inputs
+        double y_r[1], y_i[1];
+        {
+          const double2 ck = reinterpret_cast<const double2*>(C.tape_y + ((traj * ncap + sg) * P.S) * 2)[lane];
+          y_r[0] = ck.x;
+          y_i[0] = ck.y;
+        }
+        for (int n = n0; n < n1; ++n) {
+          const double h = th[n];
+          const dcpx* rw = L.rows + (n - n0) * 6 * KCP;
+          double2* ys = reinterpret_cast<double2*>(segY) + (size_t)(n - n0) * 6 * 64 + lane;
+          double k_r[8][1], k_i[8][1], yt_r[1], yt_i[1];
+          ys[0] = double2{y_r[0], y_i[0]};
+          rhs_wave<G>(LV, grows, lane, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], rw);
+#define LB_AW_FWD(st)                                                                      \
+  stage_combo<1, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                           \
+  ys[(st - 1) * 64] = double2{yt_r[0], yt_i[0]};                                           \
+  rhs_wave<G>(LV, grows, lane, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0], rw + (st - 1) * KCP);
+          LB_AW_FWD(2)
+          LB_AW_FWD(3)
+          LB_AW_FWD(4)
+          LB_AW_FWD(5)
+          LB_AW_FWD(6)
+#undef LB_AW_FWD
+          stage_combo<1, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);
+          y_r[0] = yt_r[0];
+          y_i[0] = yt_i[0];
+        }
+        // ---- adjoint sweep over the segment, last step first
+        for (int n = n1 - 1; n >= n0; --n) {
+          const double h = th[n];
+          const dcpx* rw = L.rows + (n - n0) * 6 * KCP;
+          const double2* ys = reinterpret_cast<const double2*>(segY) + (size_t)(n - n0) * 6 * 64 + lane;
+          double2 Y[6];
+#pragma unroll
+          for (int i = 0; i < 6; ++i) Y[i] = ys[i * 64];
+          double mu_r[7], mu_i[7];
+#define LB_AW_STAGE(i)                                                                       \
+  {                                                                                          \
+    double ar = T::a(7, i) * lam_r, ai = T::a(7, i) * lam_i;                                 \
+    _Pragma("unroll") for (int j = i + 1; j <= 6; ++j) {                                     \
+      if (T::a(j, i) != 0.0) {                                                               \
+        ar = fma(T::a(j, i), mu_r[j], ar);                                                   \
+        ai = fma(T::a(j, i), mu_i[j], ai);                                                   \
+      }                                                                                      \
+    }                                                                                        \
+    adjoint_wave_stage(P, L, trows, lane, h * ar, diag ? 0.0 : h * ai, Y[i - 1].x, Y[i - 1].y, diag, \
+                       selfv, selfd, rw + (i - 1) * KCP, mu_r[i], mu_i[i], g);               \
+  }
+          LB_AW_STAGE(6)
+          LB_AW_STAGE(5)
+          LB_AW_STAGE(4)
+          LB_AW_STAGE(3)
+          LB_AW_STAGE(2)
+          LB_AW_STAGE(1)
+#undef LB_AW_STAGE
+          double sr = lam_r, si = lam_i;
+#pragma unroll
+          for (int i = 1; i <= 6; ++i) {
+            sr += mu_r[i];
+            si += mu_i[i];
+          }
+          lam_r = sr + (n >= 1 ? wme * direct(n) : 0.0);
+          lam_i = diag ? 0.0 : si;
+        }
+      }
+    }
+    // ---- dI/dp: one wave sum per parameter
+#pragma unroll
+    for (int m = 0; m < LB_MAX_DIFF; ++m) {
+      if (m >= nd) break;
+      const double v = wave_sum(g[m]);
+      if (lane == 0) C.out_grad[traj * nd + m] = ok ? v : __builtin_nan("");
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Batched RHS (one evaluation per workgroup).
 // ---------------------------------------------------------------------------
 template <int R, int MAXW, int WAVES>
@@ -1592,6 +1898,13 @@ void* adjoint_ptr(int solver) {
     return nullptr;
 }
 template <int I, int GG, int VV>
+void* adjoint_wave_ptr(int solver) {
+  if constexpr (I % LB_NPART == LB_PART)
+    return solver == 1 ? (void*)&lb_adjoint_wave_kernel<1, GG, VV> : (void*)&lb_adjoint_wave_kernel<0, GG, VV>;
+  else
+    return nullptr;
+}
+template <int I, int GG, int VV>
 void* wave_solve_ptr(int solver) {
   if constexpr (I % LB_NPART == LB_PART)
     return solver == 1 ? (void*)&lb_solve_kernel<1, 1, 64, VV, GG> : (void*)&lb_solve_kernel<1, 0, 64, VV, GG>;
@@ -1631,6 +1944,10 @@ extern "C" void* LB_PART_FN(lb_dev_adjoint_kernel)(int id, int solver) {
   if (id == i) return adjoint_ptr<i, r, w, v>(solver);
   LB_ADJ_CONFIGS(Z)
 #undef Z
+#define V(i, g, v) \
+  if (id == i) return adjoint_wave_ptr<i, g, v>(solver);
+  LB_ADJ_WAVE_CONFIGS(V)
+#undef V
   return nullptr;
 }
 

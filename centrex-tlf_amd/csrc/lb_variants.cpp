@@ -61,12 +61,18 @@ extern "C" void* lb_dev_rhs_kernel(int id) {
   return id < 0 ? nullptr : parts[id % LB_NPARTS_BUILT](id);
 }
 
-// Adjoint variants: Z(id, R, largest workgroup, waves hint).
-extern "C" int lb_dev_adjoint_variant(int id, int* R, int* maxw) {
+// Adjoint variants: Z(id, R, largest workgroup, waves hint), then the
+// single-wave V(id, gather rows, waves hint) with *rows = gather rows (0 for
+// the multi-wave form).
+extern "C" int lb_dev_adjoint_variant(int id, int* R, int* maxw, int* rows) {
 #define Z(i, r, w, v)                                                                           \
-  if (id == i) { *R = r; *maxw = w; return 1; }
+  if (id == i) { *R = r; *maxw = w; *rows = 0; return 1; }
   LB_ADJ_CONFIGS(Z)
 #undef Z
+#define V(i, g, v)                                                                              \
+  if (id == i) { *R = 1; *maxw = 64; *rows = g; return 1; }
+  LB_ADJ_WAVE_CONFIGS(V)
+#undef V
   return 0;
 }
 

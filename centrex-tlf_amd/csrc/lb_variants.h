@@ -14,4 +14,8 @@
 // Adjoint kernel variants: Z(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
 #define LB_ADJ_CONFIGS(Z) Z(0, 1, 1024, 2) Z(1, 2, 512, 2) Z(2, 4, 256, 2) Z(3, 8, 256, 1)
 
+// Single-wave adjoint variants for plans of one 64-slot block: V(id, gather
+// rows held in registers by the recomputation, waves-per-SIMD hint).
+#define LB_ADJ_WAVE_CONFIGS(V) V(4, 16, 2)
+
 #define LB_NPARTS_BUILT 4

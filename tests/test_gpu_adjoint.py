@@ -105,6 +105,20 @@ def test_adjoint_matches_replay_finite_differences(setup):
     np.testing.assert_allclose(counts.cpu().numpy(), ref.cpu().numpy()[:, 0], rtol=1e-14, atol=0)
 
 
+def test_adjoint_kernel_forms_agree(setup):
+    """The single-wave adjoint (variant 4, the default for this plan) and the
+    multi-wave form (variant 0) compute the same derivative."""
+    torch, prepared, y0, t_span, weights = setup
+    engine = _engine(prepared, t_span, weights, TOL)
+    p = _batch(5, seed=9)
+    counts, tape = engine.forward(p, y0)
+    wave = engine.gradient(p, y0, tape, variant=5).cpu().numpy()
+    multi = engine.gradient(p, y0, tape, variant=1).cpu().numpy()
+    auto = engine.gradient(p, y0, tape).cpu().numpy()
+    np.testing.assert_array_equal(auto, wave)
+    _close(p, counts.cpu().numpy(), wave, multi, 1e-12)
+
+
 def test_adjoint_matches_adaptive_finite_differences(setup):
     torch, prepared, y0, t_span, weights = setup
     engine = _engine(prepared, t_span, weights, TIGHT)
