@@ -892,6 +892,7 @@ int64_t lds_bytes_adj_wave(const DevPlan& P, const std::vector<int32_t>& g_rows,
   const int64_t g = g_rows.empty() ? 0 : g_rows[0], KC = P.KD + 1;
   return g * 64 * 16 + (int64_t)t_rows * 64 * 16 + KC * 16 + (int64_t)every * 6 * odd(KC) * 16 +
          (int64_t)P.n_slots * 16 + 64 * odd(P.n_slots) * 16 + 64 * odd(P.vm_depth) * 16 + 64 * 16 +
+         (int64_t)LB_MAX_DIFF * 64 * 16 +
          (int64_t)((P.n + 1) & ~1) * 8 + g * 64 * 4 + (int64_t)t_rows * 64 * 4 + 16;
 }
 
@@ -919,7 +920,7 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   const int64_t lds_wave = lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every);
   auto wave_fits = [&](int rows) {
     return im->dev.nblk == 1 && !im->g_rows_host.empty() && im->g_rows_host[0] <= rows &&
-           lds_wave <= kMaxLdsBytes;
+           im->dev.K <= 8 && lds_wave <= kMaxLdsBytes;  // 8: LB_AW_MAX_COEF
   };
   int id = -1, W = 0, R = 0, maxw = 0, rows = 0;
   bool wave = false;

@@ -1542,6 +1542,7 @@ struct LdsAW {
   dcpx* lslots;    // 64 x SSP: per-lane parameter slots
   dcpx* lvm;       // 64 x VSP: per-lane expression stacks
   double2* selfc;  // 64 folded self coefficients
+  dcpx* selfd;     // LB_MAX_DIFF x 64 parameter derivatives of the folded self coefficients
   double* wts;     // n integral weights
   uint32_t* gw;    // g_rows x 64 gather words
   uint32_t* tw;    // t_rows x 64 transposed words
@@ -1564,6 +1565,7 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
   L.lslots = reinterpret_cast<dcpx*>(q); q += (size_t)64 * odd_up(P.n_slots) * 16;
   L.lvm = reinterpret_cast<dcpx*>(q); q += (size_t)64 * odd_up(P.vm_depth) * 16;
   L.selfc = reinterpret_cast<double2*>(q); q += (size_t)64 * 16;
+  L.selfd = reinterpret_cast<dcpx*>(q); q += (size_t)LB_MAX_DIFF * 64 * 16;
   L.wts = reinterpret_cast<double*>(q); q += (size_t)((P.n + 1) & ~1) * 8;
   L.gw = reinterpret_cast<uint32_t*>(q); q += (size_t)g * 64 * 4;
   L.tw = reinterpret_cast<uint32_t*>(q); q += (size_t)t * 64 * 4;
@@ -1571,14 +1573,18 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
   return L;
 }
 
+// Coefficients the single-wave adjoint accumulates per lane in registers.
+#define LB_AW_MAX_COEF 8
+
 // One reverse stage: mu = A(t_i)^T kappa for this lane's element and
 // g_m += Re(conj(kappa)^T (dA(t_i)/dp_m) Y_i).  `cf` is the stage's
 // coefficient row (values, the constant 1 at K, then d c_k / d p_m at
-// K + 1 + k * nd + m).
+// K + 1 + k * nd + m).  The terms' parameter contractions are first summed
+// per coefficient (Q_k, registers; K <= LB_AW_MAX_COEF), then contracted with
+// the coefficient derivatives once per stage.
 __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW& L, int trows, int lane,
                                                    double kr, double ki, double yr, double yi, bool diag,
-                                                   double2 sv, const dcpx (&selfd)[LB_MAX_DIFF],
-                                                   const dcpx* cf, double& mr, double& mi,
+                                                   double2 sv, const dcpx* cf, double& mr, double& mi,
                                                    double (&g)[LB_MAX_DIFF]) {
   const int nd = P.n_diff;
   const dcpx* dc = cf + P.K + 1;
@@ -1590,10 +1596,14 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
 #pragma unroll
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
       if (m >= nd) break;
-      g[m] += cx.re * selfd[m].re - cx.im * selfd[m].im;
+      const dcpx sd = L.selfd[m * 64 + lane];
+      g[m] += cx.re * sd.re - cx.im * sd.im;
     }
   }
-  constexpr int CH = 8;
+  dcpx Q[LB_AW_MAX_COEF];
+#pragma unroll
+  for (int k = 0; k < LB_AW_MAX_COEF; ++k) Q[k] = mk(0.0, 0.0);
+  constexpr int CH = 4;
   for (int c0 = 0; c0 < trows; c0 += CH) {
     uint32_t w[CH];
     double2 f[CH];
@@ -1625,16 +1635,28 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
       const dcpx hh = cmul(fo, c);
       mr += hh.re;
       if (!(w[j] & (1u << 20))) mi += (w[j] & (1u << 21)) ? hh.im : -hh.im;
-      if (cid != LB_STATIC_COEF) {
-        double xi = (w[j] & (1u << 20)) ? 0.0 : yi;
-        if (w[j] & (1u << 21)) xi = -xi;
-        const dcpx qx = cmul(fo, mk(yr, xi));
-        for (int m = 0; m < LB_MAX_DIFF; ++m) {
-          if (m >= nd) break;
-          const dcpx d = dc[cid * nd + m];
-          g[m] += qx.re * d.re - qx.im * (cj ? -d.im : d.im);
+      // Re(qx d') with d' = conj(d) for a conjugated coefficient is Re(conj(qx) d)
+      double xi = (w[j] & (1u << 20)) ? 0.0 : yi;
+      if (w[j] & (1u << 21)) xi = -xi;
+      const dcpx qx = cmul(fo, mk(yr, xi));
+      const double qi = cj ? -qx.im : qx.im;
+#pragma unroll
+      for (int k = 0; k < LB_AW_MAX_COEF; ++k) {
+        if ((int)cid == k) {
+          Q[k].re += qx.re;
+          Q[k].im += qi;
         }
       }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < LB_AW_MAX_COEF; ++k) {
+    if (k >= P.K) break;
+#pragma unroll
+    for (int m = 0; m < LB_MAX_DIFF; ++m) {
+      if (m >= nd) break;
+      const dcpx d = dc[k * nd + m];
+      g[m] += Q[k].re * d.re - Q[k].im * d.im;
     }
   }
 }
@@ -1694,22 +1716,18 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
     for (int i = 0; i < P.n_slots; ++i) myslots[i] = L.slots0[i];
     fold_self(P, lane, 64, L.coef0, reinterpret_cast<double*>(L.selfc));
     // parameter derivatives of the folded self coefficient
-    dcpx selfd[LB_MAX_DIFF];
-#pragma unroll
-    for (int m = 0; m < LB_MAX_DIFF; ++m) selfd[m] = mk(0.0, 0.0);
-    for (int j = 0; j < P.max_self; ++j) {
-      const size_t e = (size_t)lane * P.max_self + j;
-      const int id = P.self_coef[e];
-      if (id == -2) break;
-      if (id < 0) continue;
-      const dcpx f = mk(P.self_f[2 * e], P.self_f[2 * e + 1]);
-      const bool cj = P.self_conj[e] != 0;
-      for (int m = 0; m < LB_MAX_DIFF; ++m) {
-        if (m >= nd) break;
+    for (int m = 0; m < nd; ++m) {
+      dcpx acc = mk(0.0, 0.0);
+      for (int j = 0; j < P.max_self; ++j) {
+        const size_t e = (size_t)lane * P.max_self + j;
+        const int id = P.self_coef[e];
+        if (id == -2) break;
+        if (id < 0) continue;
         dcpx d = L.coef0[P.K + 1 + id * nd + m];
-        if (cj) d.im = -d.im;
-        selfd[m] = cadd(selfd[m], cmul(f, d));
+        if (P.self_conj[e]) d.im = -d.im;
+        acc = cadd(acc, cmul(mk(P.self_f[2 * e], P.self_f[2 * e + 1]), d));
       }
+      L.selfd[m * 64 + lane] = acc;
     }
     __syncthreads();
     const double2 selfv = L.selfc[lane];
@@ -1777,9 +1795,8 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
           const double h = th[n];
           const dcpx* rw = L.rows + (n - n0) * 6 * KCP;
           const double2* ys = reinterpret_cast<const double2*>(segY) + (size_t)(n - n0) * 6 * 64 + lane;
-          double2 Y[6];
-#pragma unroll
-          for (int i = 0; i < 6; ++i) Y[i] = ys[i * 64];
+          // stage inputs, each loaded one stage ahead of its use
+          double2 ycur = ys[5 * 64], ynext;
           double mu_r[7], mu_i[7];
 #define LB_AW_STAGE(i)                                                                       \
   {                                                                                          \
@@ -1790,8 +1807,10 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
         ai = fma(T::a(j, i), mu_i[j], ai);                                                   \
       }                                                                                      \
     }                                                                                        \
-    adjoint_wave_stage(P, L, trows, lane, h * ar, diag ? 0.0 : h * ai, Y[i - 1].x, Y[i - 1].y, diag, \
-                       selfv, selfd, rw + (i - 1) * KCP, mu_r[i], mu_i[i], g);               \
+    if (i > 1) ynext = ys[(i - 2) * 64];                                                     \
+    adjoint_wave_stage(P, L, trows, lane, h * ar, diag ? 0.0 : h * ai, ycur.x, ycur.y, diag, \
+                       selfv, rw + (i - 1) * KCP, mu_r[i], mu_i[i], g);                      \
+    if (i > 1) ycur = ynext;                                                                 \
   }
           LB_AW_STAGE(6)
           LB_AW_STAGE(5)

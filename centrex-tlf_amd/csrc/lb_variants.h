@@ -9,13 +9,15 @@
 
 // Single-wave variants for plans of at most 64 elements: Y(id, gather rows
 // held in registers, waves-per-SIMD hint); one slot per lane, 64 threads.
-#define LB_WAVE_CONFIGS(Y) Y(11, 8, 4) Y(12, 8, 2) Y(13, 16, 2)
+#define LB_WAVE_CONFIGS(Y) Y(11, 8, 4) Y(12, 8, 2) Y(13, 16, 2) Y(14, 12, 2)
 
 // Adjoint kernel variants: Z(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
 #define LB_ADJ_CONFIGS(Z) Z(0, 1, 1024, 2) Z(1, 2, 512, 2) Z(2, 4, 256, 2) Z(3, 8, 256, 1)
 
-// Single-wave adjoint variants for plans of one 64-slot block: V(id, gather
-// rows held in registers by the recomputation, waves-per-SIMD hint).
-#define LB_ADJ_WAVE_CONFIGS(V) V(4, 16, 2)
+// Single-wave adjoint variants for plans of one 64-slot block and at most
+// eight Hamiltonian coefficients: V(id, gather rows held in registers by the
+// recomputation, waves-per-SIMD hint), in increasing rows (the host takes
+// the first that covers the plan).
+#define LB_ADJ_WAVE_CONFIGS(V) V(4, 12, 2) V(5, 16, 2)
 
 #define LB_NPARTS_BUILT 4
