@@ -119,6 +119,7 @@ class LbSolveArgs(C.Structure):
         ("out_grad", C.c_void_p),
         ("replay_h", C.c_void_p),
         ("replay_n", C.c_void_p),
+        ("queue_order", C.c_void_p),
     ]
 
 

@@ -66,6 +66,10 @@ def library() -> C.CDLL:
     L.lb_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.lb_adjoint.restype = C.c_int
     L.lb_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.lb_launch_info.restype = C.c_int
+    L.lb_launch_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    L.lb_solve_group.restype = C.c_int
+    L.lb_solve_group.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     L.lb_variant_info.restype = C.c_int
     L.lb_variant_info.argtypes = [C.c_int32] + [C.POINTER(C.c_int32)] * 3
     _lib = L
@@ -206,6 +210,19 @@ class DevicePlan:
         return out
 
     # -- integration ---------------------------------------------------------
+    def launch_info(self, support=None, variant: int = 0, threads_per_traj: int = 0) -> dict[str, int]:
+        """The persistent launch a solve would make: variant (1-based), threads,
+        resident workgroups per CU and CUs (``lb_launch_info``)."""
+        out = (C.c_int32 * 4)()
+        sup = None
+        if support is not None:
+            sup = np.ascontiguousarray(np.asarray(support, dtype=bool).astype(np.uint8).reshape(-1))
+            if sup.size != self.dim:
+                raise ValueError("support must have one entry per packed index")
+        _check(library().lb_launch_info(self._handle, None if sup is None else ptr(sup, C.c_uint8),
+                                        int(variant), int(threads_per_traj), out))
+        return {"variant": out[0], "threads_per_traj": out[1], "per_cu": out[2], "cus": out[3]}
+
     def solve(self, y0, t0: float, t1: float | Sequence[float], *, solver: str = "dopri5",
               abstol: float = 1e-7, reltol: float = 1e-4, dt: float = 1e-8, maxiters: int = 100_000,
               save_start: bool = True, output_kind: str = "populations",
@@ -217,7 +234,7 @@ class DevicePlan:
               stream=None, with_integral: bool = False, variant: int = 0,
               restrict_support: bool = True, support=None, events=None,
               save_plan=None, trace: bool = False, capacity: int = 0, with_times: bool = False,
-              tape=None, adjoint_grad=None, replay=None):
+              tape=None, adjoint_grad=None, replay=None, queue_order=None, _launch: bool = True):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
@@ -270,7 +287,12 @@ class DevicePlan:
         if not shared and y0.shape[0] != B:
             raise ValueError(f"y0 has {y0.shape[0]} rows for {B} trajectories")
         t1_arr = None
-        if np.ndim(t1) > 0:
+        if torch.is_tensor(t1) and t1.ndim > 0:  # device tensors pass through (graph capture)
+            t1_arr = t1.to(device=dev, dtype=torch.float64).contiguous()
+            if t1_arr.numel() != B:
+                raise ValueError("per-trajectory t1 must have one entry per trajectory")
+            t1_scalar = float("nan")
+        elif np.ndim(t1) > 0:
             t1_arr = torch.as_tensor(np.asarray(t1, dtype=np.float64), device=dev).contiguous()
             if t1_arr.numel() != B:
                 raise ValueError("per-trajectory t1 must have one entry per trajectory")
@@ -364,6 +386,24 @@ class DevicePlan:
             a.tape_t, a.tape_h = _dptr(tape.t), _dptr(tape.h)
             a.tape_y, a.tape_n = _dptr(tape.y), _dptr(tape.n)
             a.tape_cap, a.ckpt_every = int(tape.cap), int(tape.every)
+        order_t = None
+        if queue_order is not None:  # queue position -> trajectory (e.g. longest first)
+            order_t = (queue_order.to(device=dev, dtype=torch.int32).reshape(-1).contiguous()
+                       if torch.is_tensor(queue_order) else
+                       torch.as_tensor(np.asarray(queue_order, dtype=np.int32).reshape(-1), device=dev).contiguous())
+            if order_t.numel() != B:
+                raise ValueError("queue_order must have one entry per trajectory")
+            a.queue_order = _dptr(order_t)
+        if trace:
+            result = (values, stats, fail_t, integral, counts, times)
+        elif with_integral:
+            result = (values, stats, fail_t, integral)
+        else:
+            result = (values, stats, fail_t)
+        if not _launch:
+            if adjoint_grad is not None or events is not None:
+                raise ValueError("grouped solves take no adjoint or events")
+            return _Deferred(a, keep + [y0, t1_arr, ov, plan_t, order_t], result)
         sp = _stream_ptr() if stream is None else C.c_void_p(stream.cuda_stream)
         if events is not None:  # (start, end) torch events recorded right around the launch
             events[0].record(stream)
@@ -376,8 +416,34 @@ class DevicePlan:
         _check(library().lb_solve(self._handle, C.byref(a), sp))
         if events is not None:
             events[1].record(stream)
-        if trace:
-            return values, stats, fail_t, integral, counts, times
-        if with_integral:
-            return values, stats, fail_t, integral
-        return values, stats, fail_t
+        return result
+
+
+class _Deferred:
+    """Arguments of a solve built but not launched (for ``solve_group``)."""
+
+    def __init__(self, args, keep, result):
+        self.args, self.keep, self.result = args, keep, result
+
+
+def solve_group(calls: Sequence[tuple["DevicePlan", dict[str, Any]]], stream=None, events=None) -> list[tuple]:
+    """Solve several plans concurrently in one call (``lb_solve_group``).
+
+    ``calls`` is a list of ``(plan, kwargs)`` with the keyword arguments of
+    :meth:`DevicePlan.solve` (``y0``, ``t0``, ``t1`` included); each plan may
+    appear once.  Returns each call's result tuple, in order.  ``workgroups``
+    in a call's kwargs bounds that group's share of the device.
+    """
+    if not calls:
+        return []
+    deferred = [plan.solve(_launch=False, **kw) for plan, kw in calls]
+    n = len(calls)
+    handles = (C.c_void_p * n)(*[plan._handle for plan, _ in calls])
+    args = (LbSolveArgs * n)(*[d.args for d in deferred])
+    sp = _stream_ptr() if stream is None else C.c_void_p(stream.cuda_stream)
+    if events is not None:
+        events[0].record(stream)
+    _check(library().lb_solve_group(handles, args, n, sp))
+    if events is not None:
+        events[1].record(stream)
+    return [d.result for d in deferred]

@@ -126,6 +126,8 @@ struct DevCall {
   int32_t want_integral;   // accumulate the trapezoid
   int32_t* queue;          // work-queue counter
   int64_t traj_offset;     // plain mode: first trajectory of this launch
+  int64_t n_total;         // trajectories of the call (all launches of plain mode)
+  const int32_t* queue_order;  // queue position -> trajectory, or NULL
   // pushed points: every accepted step (n_saveat == 0) or a save plan
   int32_t n_saveat;
   const double* saveat;    // device, n_saveat ascending times (t0 first when saving the start)

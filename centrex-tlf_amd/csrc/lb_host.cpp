@@ -727,6 +727,8 @@ int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out,
   c.t0 = a->t0;
   c.t1 = a->t1;
   c.t1_per_traj = a->t1_per_traj;
+  c.n_total = a->n_traj;
+  c.queue_order = a->queue_order;
   c.abstol = a->abstol;
   c.reltol = a->reltol;
   c.dt = a->dt;
@@ -984,6 +986,81 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   void* args[] = {&P, &c};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
   HIP_TRY(hipGetLastError());
+  return LB_OK;
+}
+
+namespace {
+
+// Fork streams and events of lb_solve_group, per device, created on first use
+// (default priority, non-blocking) and kept until the process ends.
+struct GroupStreams {
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;  // [0] fork, [1 + g] join of group g
+};
+std::mutex g_group_mu;
+std::map<int, GroupStreams> g_group;
+
+}  // namespace
+
+extern "C" int lb_solve_group(const lb_plan* const* plans, const lb_solve_args* args, int32_t n_groups,
+                              void* stream) {
+  if (!plans || !args) return fail(LB_ERR_INVALID, "null argument");
+  if (n_groups < 1 || n_groups > LB_MAX_GROUPS) return fail(LB_ERR_INVALID, "n_groups must be 1..16");
+  for (int g = 0; g < n_groups; ++g) {
+    if (!plans[g]) return fail(LB_ERR_INVALID, "null plan in group " + std::to_string(g));
+    if (args[g].mode != LB_MODE_PERSISTENT)
+      return fail(LB_ERR_INVALID, "grouped solves use the persistent mode");
+    for (int h = 0; h < g; ++h)
+      if (plans[h] == plans[g])
+        return fail(LB_ERR_INVALID, "a plan may appear in one group only (it holds one work queue)");
+    if (plans[g]->device != plans[0]->device) return fail(LB_ERR_INVALID, "all plans must be on one device");
+  }
+  if (n_groups == 1) return lb_solve(plans[0], &args[0], stream);
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lock(g_group_mu);
+  GroupStreams& gs = g_group[plans[0]->device];
+  while ((int)gs.streams.size() < n_groups) {
+    hipStream_t t;
+    HIP_TRY(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+    gs.streams.push_back(t);
+  }
+  while ((int)gs.events.size() < n_groups + 1) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    gs.events.push_back(e);
+  }
+  // fork: every group stream starts after the work already queued on `stream`
+  HIP_TRY(hipEventRecord(gs.events[0], s));
+  for (int g = 0; g < n_groups; ++g) HIP_TRY(hipStreamWaitEvent(gs.streams[g], gs.events[0], 0));
+  int rc = LB_OK;
+  for (int g = 0; g < n_groups && rc == LB_OK; ++g) rc = lb_solve(plans[g], &args[g], gs.streams[g]);
+  // join: `stream` continues after every group (also after a failed launch,
+  // so a capture stays balanced)
+  for (int g = 0; g < n_groups; ++g) {
+    HIP_TRY(hipEventRecord(gs.events[1 + g], gs.streams[g]));
+    HIP_TRY(hipStreamWaitEvent(s, gs.events[1 + g], 0));
+  }
+  return rc;
+}
+
+extern "C" int lb_launch_info(const lb_plan* p, const uint8_t* y0_support, int32_t variant,
+                              int32_t threads_per_traj, int32_t* out) {
+  if (!p || !out) return fail(LB_ERR_INVALID, "null argument");
+  const DevImage* im = nullptr;
+  int rc = select_image(const_cast<lb_plan*>(p), y0_support, &im);
+  if (rc) return rc;
+  int id, W;
+  if ((rc = resolve_config(im, variant, threads_per_traj, &id, &W))) return rc;
+  void* fn = lb_dev_solve_kernel(id, LB_SOLVER_DOPRI5);
+  if (!fn) return fail(LB_ERR_UNSUPPORTED, "kernel variant is not built");
+  const size_t lds = (size_t)im->lds_bytes;
+  if ((rc = set_lds_limit(fn, lds))) return rc;
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, W, lds));
+  out[0] = id + 1;
+  out[1] = W;
+  out[2] = per_cu;
+  out[3] = p->cus;
   return LB_OK;
 }
 

@@ -700,7 +700,10 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     __syncthreads();
     const int64_t local = L.misc[0];
     if (local >= C.n_traj) break;
-    const int64_t traj = local + C.traj_offset;
+    // queue position -> trajectory (longest-first orders); entries outside
+    // the batch are skipped, so a bad order cannot address past the arrays
+    const int64_t traj = C.queue_order ? (int64_t)C.queue_order[local + C.traj_offset] : local + C.traj_offset;
+    if (traj < 0 || traj >= C.n_total) continue;
     const double t0 = C.t0;
     const double t1 = C.t1_per_traj ? C.t1_per_traj[traj] : C.t1;
 
@@ -1329,7 +1332,10 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
     __syncthreads();
     const int64_t local = L.misc[0];
     if (local >= C.n_traj) break;
-    const int64_t traj = local + C.traj_offset;
+    // queue position -> trajectory (longest-first orders); entries outside
+    // the batch are skipped, so a bad order cannot address past the arrays
+    const int64_t traj = C.queue_order ? (int64_t)C.queue_order[local + C.traj_offset] : local + C.traj_offset;
+    if (traj < 0 || traj >= C.n_total) continue;
 
     // ---- parameters, coefficients at t0 (self folds), weights
     for (int i = tid; i < P.n_slots; i += W)
@@ -1696,7 +1702,10 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
     __syncthreads();
     const int64_t local = L.misc[0];
     if (local >= C.n_traj) break;
-    const int64_t traj = local + C.traj_offset;
+    // queue position -> trajectory (longest-first orders); entries outside
+    // the batch are skipped, so a bad order cannot address past the arrays
+    const int64_t traj = C.queue_order ? (int64_t)C.queue_order[local + C.traj_offset] : local + C.traj_offset;
+    if (traj < 0 || traj >= C.n_total) continue;
 
     // ---- parameters and coefficients at t0, weights
     for (int i = lane; i < P.n_slots; i += 64)

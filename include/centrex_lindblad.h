@@ -237,6 +237,10 @@ typedef struct lb_solve_args {
                                      accepted step sizes (a recorded tape) instead of the
                                      step-size controller, for checks of the adjoint */
   const int32_t* replay_n;        /* device, n_traj: steps of each schedule */
+  const int32_t* queue_order;     /* device, n_traj, or NULL: the work queue hands out
+                                     trajectories in this order (a permutation of
+                                     0..n_traj-1; e.g. longest first for uneven batches).
+                                     Entries outside [0, n_traj) are skipped. */
 } lb_solve_args;
 
 /*
@@ -260,6 +264,30 @@ int lb_solve(const lb_plan* plan, const lb_solve_args* args, void* stream);
  * of ckpt_every recomputed steps per resident workgroup.
  */
 int lb_adjoint(const lb_plan* plan, const lb_solve_args* args, void* stream);
+
+/*
+ * Several plans in one call (no reference counterpart: the reference solves
+ * one plan per call, batch.py:209).  Group g integrates args[g] with
+ * plans[g] as lb_solve would; the groups run concurrently on library-owned
+ * streams forked from `stream` and joined back into it, so a HIP graph
+ * captured on `stream` holds the whole uneven batch as one launch.  Each
+ * group's args.workgroups bounds its share of the device (0 lets a group fill
+ * the device, which serialises the groups).  A plan may appear in one group
+ * only.  At most LB_MAX_GROUPS groups.
+ */
+#define LB_MAX_GROUPS 16
+
+/*
+ * The launch a persistent lb_solve of this plan would make for the given y0
+ * support (NULL = every element), variant and threads_per_traj (0 = plan
+ * defaults): out[0] variant (1-based), out[1] threads per workgroup,
+ * out[2] resident workgroups per CU, out[3] CUs of the device.  No reference
+ * counterpart: it sizes the groups of lb_solve_group.
+ */
+int lb_launch_info(const lb_plan* plan, const uint8_t* y0_support, int32_t variant,
+                   int32_t threads_per_traj, int32_t* out);
+int lb_solve_group(const lb_plan* const* plans, const lb_solve_args* args, int32_t n_groups,
+                   void* stream);
 
 /*
  * Kernel variant `id` (0-based): slots per thread, largest workgroup and the
