@@ -7,7 +7,16 @@
 //  * In a time-dependent expression, every time-independent subexpression of
 //    two or more instructions is moved into a hidden compound slot that is
 //    evaluated once per trajectory; the expression reads the slot instead.
-//    The operations and their order are unchanged, so the values are too.
+//    Identical subexpressions share one slot.
+//  * A time-dependent subexpression of two or more instructions that occurs
+//    more than once across the plan (the same envelope for two lasers, a
+//    polarisation switch and its complement, a phase factor used by two
+//    couplings) is evaluated once per stage time: the first compound whose
+//    whole expression it is keeps it, otherwise it moves into a hidden
+//    time-dependent compound evaluated right before its first user, and every
+//    occurrence reads the slot.
+//    Each value is still produced by the same operations on the same operands,
+//    so the results are bitwise those of the unshared program.
 //  * `CONST c; POW` with a real constant c becomes POWC, which carries
 //    cos(c*pi) and sin(c*pi): for a negative real base the polar form
 //    x^c = |x|^c (cos(c*atan2(0, x)), sin(c*atan2(0, x))) then needs no
@@ -20,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -63,6 +73,40 @@ void emit(const std::vector<Node>& nodes, int id, std::vector<DevInstr>& out) {
   out.push_back(nodes[id].ins);
 }
 
+// Exact identity of a subtree: every field of every instruction, postfix order.
+std::string key_of(const std::vector<Node>& nodes, int id) {
+  std::string k;
+  std::function<void(int)> walk = [&](int i) {
+    for (int c : nodes[i].kids) walk(c);
+    const DevInstr& q = nodes[i].ins;
+    k.append(reinterpret_cast<const char*>(&q.op), sizeof q.op);
+    k.append(reinterpret_cast<const char*>(&q.index), sizeof q.index);
+    k.append(reinterpret_cast<const char*>(&q.argc), sizeof q.argc);
+    k.append(reinterpret_cast<const char*>(&q.function), sizeof q.function);
+    k.append(reinterpret_cast<const char*>(&q.re), sizeof q.re);
+    k.append(reinterpret_cast<const char*>(&q.im), sizeof q.im);
+    k.append(reinterpret_cast<const char*>(&q.aux), sizeof q.aux);
+  };
+  walk(id);
+  return k;
+}
+
+Node leaf(int slot) {
+  Node n;
+  n.ins = DevInstr{2, slot, 0, 0, 0.0, 0.0, 0.0};
+  return n;
+}
+
+// Subtree sizes (children precede parents in `nodes`).
+void resize_tree(std::vector<Node>& nodes) {
+  std::function<int(int)> size = [&](int i) {
+    int s = 1;
+    for (int c : nodes[i].kids) s += size(c);
+    return nodes[i].size = s;
+  };
+  for (size_t i = 0; i < nodes.size(); ++i) size((int)i);
+}
+
 int depth_of(const std::vector<DevInstr>& prog, size_t start) {
   int sp = 0, peak = 0;
   for (size_t p = start; p < prog.size(); ++p) {
@@ -88,80 +132,142 @@ int lb_build_program(const lb_plan_desc* d, const std::vector<uint8_t>& dyn_slot
   out->coef_dyn.assign(coef_dyn.begin(), coef_dyn.end());
   out->max_depth = 0;
 
-  std::vector<std::vector<DevInstr>> hidden_progs;  // pending hidden compounds
-  std::vector<int> hidden_slots;
-  auto flush_hidden = [&]() {
-    for (size_t h = 0; h < hidden_progs.size(); ++h) {
-      out->expr_start.push_back((int32_t)prog.size());
-      const size_t s0 = prog.size();
-      prog.insert(prog.end(), hidden_progs[h].begin(), hidden_progs[h].end());
-      out->max_depth = std::max(out->max_depth, depth_of(prog, s0));
-      out->compound_slot.push_back(hidden_slots[h]);
-      out->compound_dyn.push_back(0);
-    }
-    hidden_progs.clear();
-    hidden_slots.clear();
+  // A hidden compound: program, slot, time-dependent.
+  struct Hidden {
+    std::vector<DevInstr> prog;
+    int slot;
+    bool dyn;
   };
-
-  std::vector<std::vector<DevInstr>> coef_progs(d->n_coefficients);
-  for (int e = 0; e < n_expr; ++e) {
-    const bool is_coef = e >= d->n_compounds;
-    const bool dynamic = is_coef ? coef_dyn[e - d->n_compounds] != 0 : compound_dyn[e] != 0;
+  struct Expr {
     std::vector<Node> nodes;
-    const int root = parse(d, d->expr_ptrs[e], d->expr_ptrs[e + 1], nodes);
-    if (root < 0) {
+    int root = -1;
+    bool dynamic = false;
+    std::vector<Hidden> before;  // hidden compounds evaluated before this expression
+  };
+  std::vector<Expr> ex(n_expr);
+
+  // ---- phase A: parse; hoist time-independent subtrees (shared by key); POWC
+  std::map<std::string, int> static_slot;
+  for (int e = 0; e < n_expr; ++e) {
+    Expr& X = ex[e];
+    const bool is_coef = e >= d->n_compounds;
+    X.dynamic = is_coef ? coef_dyn[e - d->n_compounds] != 0 : compound_dyn[e] != 0;
+    X.root = parse(d, d->expr_ptrs[e], d->expr_ptrs[e + 1], X.nodes);
+    if (X.root < 0) {
       why = "malformed expression " + std::to_string(e);
       return -1;
     }
-    std::vector<DevInstr> body;
-    if (!dynamic) {
-      emit(nodes, root, body);
-    } else {
-      // staticness and sizes (children precede parents in `nodes`)
-      for (Node& n : nodes) {
-        if (n.ins.op == 4) n.is_static = false;
-        if (n.ins.op == 2 && dyn_slot[n.ins.index]) n.is_static = false;
-        for (int k : n.kids) {
-          n.is_static = n.is_static && nodes[k].is_static;
-          n.size += nodes[k].size;
-        }
-      }
-      std::function<void(int)> hoist = [&](int id) {
-        Node& n = nodes[id];
-        for (int& k : n.kids) {
-          if (nodes[k].is_static && nodes[k].size >= 2) {
+    if (!X.dynamic) continue;
+    std::vector<Node>& nodes = X.nodes;
+    for (Node& n : nodes) {  // children precede parents
+      n.is_static = !(n.ins.op == 4 || (n.ins.op == 2 && dyn_slot[n.ins.index]));
+      for (int k : n.kids) n.is_static = n.is_static && nodes[k].is_static;
+    }
+    resize_tree(nodes);
+    std::function<void(int)> hoist = [&](int id) {
+      for (size_t c = 0; c < nodes[id].kids.size(); ++c) {
+        const int k = nodes[id].kids[c];
+        if (nodes[k].is_static && nodes[k].size >= 2) {
+          const std::string key = key_of(nodes, k);
+          auto it = static_slot.find(key);
+          int slot;
+          if (it != static_slot.end()) {
+            slot = it->second;
+          } else {
             std::vector<DevInstr> sub;
             emit(nodes, k, sub);
-            hidden_progs.push_back(sub);
-            hidden_slots.push_back(next_slot);
-            Node leaf;
-            leaf.ins = DevInstr{2, next_slot, 0, 0, 0.0, 0.0, 0.0};
-            ++next_slot;
-            nodes.push_back(leaf);
-            k = (int)nodes.size() - 1;
-          } else if (!nodes[k].is_static) {
-            hoist(k);
+            slot = next_slot++;
+            static_slot[key] = slot;
+            X.before.push_back(Hidden{sub, slot, false});
           }
+          nodes.push_back(leaf(slot));
+          nodes[id].kids[c] = (int)nodes.size() - 1;
+        } else if (!nodes[k].is_static) {
+          hoist(k);
         }
-        // CONST c ; POW  ->  POWC(c) with cos(c*pi), sin(c*pi)
-        if (n.ins.op == 9 && n.kids.size() == 2) {
-          const Node& ex = nodes[n.kids[1]];
-          if (ex.ins.op == 1 && ex.ins.im == 0.0) {
-            const double c = ex.ins.re;
-            n.ins = DevInstr{20, 0, 0, 0, c, std::cos(c * M_PI), std::sin(c * M_PI)};
-            n.kids.resize(1);
-          }
-        }
-      };
-      if (nodes[root].is_static) {
-        emit(nodes, root, body);
-      } else {
-        hoist(root);
-        emit(nodes, root, body);
       }
-    }
+      // CONST c ; POW  ->  POWC(c) with cos(c*pi), sin(c*pi)
+      Node& n = nodes[id];
+      if (n.ins.op == 9 && n.kids.size() == 2) {
+        const Node& e2 = nodes[n.kids[1]];
+        if (e2.ins.op == 1 && e2.ins.im == 0.0) {
+          const double c = e2.ins.re;
+          n.ins = DevInstr{20, 0, 0, 0, c, std::cos(c * M_PI), std::sin(c * M_PI)};
+          n.kids.resize(1);
+        }
+      }
+    };
+    if (!nodes[X.root].is_static) hoist(X.root);
+    resize_tree(nodes);
+  }
+
+  // ---- phase B: occurrences of every time-dependent subtree of size >= 2
+  std::map<std::string, int> count;
+  std::function<void(const std::vector<Node>&, int, std::map<std::string, int>&, int)> tally =
+      [&](const std::vector<Node>& nodes, int id, std::map<std::string, int>& into, int w) {
+        const Node& n = nodes[id];
+        if (n.is_static || n.size < 2) return;
+        into[key_of(nodes, id)] += w;
+        for (int k : n.kids) tally(nodes, k, into, w);
+      };
+  for (Expr& X : ex)
+    if (X.dynamic && !X.nodes[X.root].is_static) tally(X.nodes, X.root, count, 1);
+
+  // ---- phase C: share repeated time-dependent subtrees, emit
+  std::map<std::string, int> dyn_slot_of;
+  auto emit_hidden = [&](const Hidden& h) {
+    out->expr_start.push_back((int32_t)prog.size());
+    const size_t s0 = prog.size();
+    prog.insert(prog.end(), h.prog.begin(), h.prog.end());
+    out->max_depth = std::max(out->max_depth, depth_of(prog, s0));
+    out->compound_slot.push_back(h.slot);
+    out->compound_dyn.push_back(h.dyn ? 1 : 0);
+  };
+  std::vector<Hidden> coef_hidden;
+  std::vector<std::vector<DevInstr>> coef_progs(d->n_coefficients);
+  for (int e = 0; e < n_expr; ++e) {
+    Expr& X = ex[e];
+    const bool is_coef = e >= d->n_compounds;
+    std::vector<Node>& nodes = X.nodes;
+    std::vector<Hidden> shared;
+    std::function<void(int, bool)> share = [&](int id, bool whole) {
+      Node& n = nodes[id];
+      if (n.is_static || n.size < 2) return;
+      const std::string key = key_of(nodes, id);
+      if (count[key] >= 2) {
+        auto it = dyn_slot_of.find(key);
+        if (it != dyn_slot_of.end()) {  // computed before: read its slot
+          nodes[id] = leaf(it->second);
+          nodes[id].is_static = false;
+          return;
+        }
+        // the other occurrences will read a slot: their subtrees no longer count
+        std::map<std::string, int> inner;
+        for (int k : n.kids) tally(nodes, k, inner, count[key] - 1);
+        for (auto& kv : inner) count[kv.first] -= kv.second;
+        if (whole) {  // this compound's own slot holds the value
+          dyn_slot_of[key] = d->compound_slots[e];
+          for (int k : nodes[id].kids) share(k, false);
+          return;
+        }
+        for (size_t c = 0; c < nodes[id].kids.size(); ++c) share(nodes[id].kids[c], false);
+        Hidden h{{}, next_slot++, true};
+        emit(nodes, id, h.prog);
+        dyn_slot_of[key] = h.slot;
+        shared.push_back(h);
+        nodes[id] = leaf(h.slot);
+        nodes[id].is_static = false;
+        return;
+      }
+      for (size_t c = 0; c < nodes[id].kids.size(); ++c) share(nodes[id].kids[c], false);
+    };
+    if (X.dynamic && !nodes[X.root].is_static) share(X.root, !is_coef);
+    std::vector<DevInstr> body;
+    emit(nodes, X.root, body);
+    std::vector<Hidden> pre = X.before;
+    pre.insert(pre.end(), shared.begin(), shared.end());
     if (!is_coef) {
-      flush_hidden();  // hidden parts of a compound go right before it
+      for (const Hidden& h : pre) emit_hidden(h);  // right before the compound
       out->expr_start.push_back((int32_t)prog.size());
       const size_t s0 = prog.size();
       prog.insert(prog.end(), body.begin(), body.end());
@@ -169,10 +275,11 @@ int lb_build_program(const lb_plan_desc* d, const std::vector<uint8_t>& dyn_slot
       out->compound_slot.push_back(d->compound_slots[e]);
       out->compound_dyn.push_back(compound_dyn[e]);
     } else {
+      coef_hidden.insert(coef_hidden.end(), pre.begin(), pre.end());
       coef_progs[e - d->n_compounds] = body;
     }
   }
-  flush_hidden();  // hidden parts of the coefficients follow all compounds
+  for (const Hidden& h : coef_hidden) emit_hidden(h);  // after all compounds
   out->n_compounds = (int)out->compound_slot.size();
   for (int k = 0; k < d->n_coefficients; ++k) {
     out->expr_start.push_back((int32_t)prog.size());

@@ -3,9 +3,11 @@
 // the host picks a variant whose R * W covers the plan's slots.
 #pragma once
 
+// (Entry 6 was (4, 256, 1); a build of it gave wrong results that no other
+// variant of the same slots and workgroup showed, so it is not compiled.)
 #define LB_CONFIGS(X)                                                                            \
   X(0, 1, 1024, 4) X(1, 2, 512, 4) X(2, 2, 512, 2) X(3, 3, 1024, 4) X(4, 4, 256, 4)                \
-  X(5, 4, 256, 2) X(6, 4, 256, 1) X(7, 6, 512, 2) X(8, 6, 512, 1) X(9, 8, 128, 1) X(10, 12, 256, 1)
+  X(5, 4, 256, 2) X(6, 3, 512, 2) X(7, 6, 512, 2) X(8, 6, 512, 1) X(9, 8, 128, 1) X(10, 12, 256, 1)
 
 // Single-wave variants for plans of at most 64 elements: Y(id, gather rows
 // held in registers, waves-per-SIMD hint); one slot per lane, 64 threads.
