@@ -63,6 +63,106 @@ def algorithmic_flops(stats, flops_per_rhs: int, dim: int, solver: str) -> float
     return float(np.sum(rhs * flops_per_rhs + attempted * dim * F_RK[solver]))
 
 
+def uneven_leg(steps: int, warmup: int, cpu_seconds: float) -> dict:
+    """Uneven batch in one call: 512 n=39 + 512 n=72 trajectories.
+
+    n=72: X(J=0..3) + B(J=1, F1'=3/2, F'=1,2), two lasers whose polarisation
+    switches x <-> y with a 1 MHz square wave, microwave J=1-J=2 at 0.5 Gamma.
+    Every trajectory has its own width sigma ~ N(5, 1.5)/Gamma (clipped above
+    1/Gamma), centre sigma and end time 2 sigma, and a detuning ~ U[-2, 2]
+    Gamma; rtol 1e-8, atol 1e-10.  The launch parameters come from the
+    autotuner (timed on this batch); the untuned launch (each group filling
+    the device in turn) is reported beside it.  Dopri5 is the headline, Tsit5
+    is timed too.  The CPU oracle runs a seeded sample of both systems on the
+    host cores; its populations are compared with the GPU's.
+    """
+    import numpy as np
+    import torch
+
+    from centrex_tlf_amd.lindblad import UnevenGroup, autotune, prepare_lindblad_problem, solve_uneven_batch
+    from oracle.lb_oracle import Oracle, host_threads
+    from tests.systems import GAMMA, tlf_n39, tlf_n72
+
+    tol = dict(abstol=1e-10, reltol=1e-8, dt=1e-8)
+    names = ["sigma", "t_center", "delta0"]
+    rng = np.random.default_rng(20261202 + 2)
+    groups = []
+    for build in (tlf_n39, tlf_n72):
+        system, params, rho0, _, weights = build()
+        prepared = prepare_lindblad_problem(system, params, backend="gpu")
+        sigma = np.clip(rng.normal(5.0, 1.5, 512), 1.0, None) / GAMMA
+        delta = rng.uniform(-2.0, 2.0, 512) * GAMMA
+        pb = np.stack([sigma, sigma, delta], axis=1).astype(complex)
+        groups.append(UnevenGroup(prepared, rho0, 2.0 * sigma, names, pb, weights))
+    n_traj = sum(len(g.t_end) for g in groups)
+    out: dict = {"metric": "uneven batch trajectories/s", "unit": "trajectories/s", "trajectories": n_traj,
+                 "config": "512 tlf_n39 + 512 tlf_n72, per-trajectory sigma and t_end = 2 sigma, rtol 1e-8"}
+
+    def timed(fn):
+        for _ in range(max(1, warmup)):
+            res = fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            res = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps, res
+
+    for solver in ("dopri5", "tsit5"):
+        t_tune = time.perf_counter()
+        tun = autotune(groups, solver=solver, repeats=1, **tol)
+        t_tune = time.perf_counter() - t_tune
+        tuned_s, res = timed(lambda: solve_uneven_batch(groups, solver=solver, tuning=tun, **tol))
+        plain_s, _ = timed(lambda: solve_uneven_batch(groups, solver=solver, tuning=None, longest_first=False,
+                                                      **tol))
+        steps_all = [r.stats[:, 0] for r in res]
+        log(f"bench: uneven {solver}: tuned {tuned_s * 1e3:.1f} ms ({tun.mode}, variants {tun.variants}, "
+            f"workgroups {tun.workgroups}), untuned {plain_s * 1e3:.1f} ms, tuning took {t_tune:.1f} s")
+        out[solver] = {
+            "value": n_traj / tuned_s,
+            "ms_per_batch": tuned_s * 1e3,
+            "untuned_ms_per_batch": plain_s * 1e3,
+            "tuning": {"mode": tun.mode, "variants": tun.variants, "threads": tun.threads,
+                       "workgroups": tun.workgroups, "seconds": t_tune},
+            "accepted_steps_min_max": {"n39": [int(steps_all[0].min()), int(steps_all[0].max())],
+                                       "n72": [int(steps_all[1].min()), int(steps_all[1].max())]},
+        }
+        if solver == "dopri5":
+            out["value"] = n_traj / tuned_s
+            pops = [r.populations for r in res]
+    # ---- CPU oracle on a seeded sample of each system (Dopri5)
+    cores = host_threads()
+    cpu = {}
+    errs = []
+    for k, (grp, label) in enumerate(zip(groups, ("n39", "n72"))):
+        oracle = Oracle(grp.prepared)
+        y0 = grp.prepared.layout.pack(grp.rho0)
+        take = np.sort(rng.choice(len(grp.t_end), size=cores, replace=False))
+        t_c = time.perf_counter()
+        done = 0
+        while True:
+            for b in take:
+                r = oracle.solve_batch(y0, (0.0, float(grp.t_end[b])), output="populations", output_when="final",
+                                       parameter_slots=names, parameter_batch=grp.parameter_batch[b:b + 1],
+                                       threads=1, **tol)
+                errs.append(float(np.abs(r["final"][0] - pops[k][b]).max()))
+                done += 1
+                if time.perf_counter() - t_c > cpu_seconds / 2:
+                    break
+            if time.perf_counter() - t_c > cpu_seconds / 2 or done >= len(take):
+                break
+        cpu[label] = (time.perf_counter() - t_c) / done  # seconds per trajectory, one core
+    # the mixed batch on all host cores: total single-core time / cores
+    cpu_rate = n_traj / ((512 * cpu["n39"] + 512 * cpu["n72"]) / cores)
+    out["cpu_baseline"] = {"value": cpu_rate, "unit": "trajectories/s", "cores": cores, "kind": "port",
+                           "sample": f"oracle, one core per trajectory: {cpu['n39']:.3f} s per n39 and "
+                                     f"{cpu['n72']:.2f} s per n72 trajectory (seeded sample), scaled to the "
+                                     f"1024-trajectory batch on {cores} cores"}
+    out["speedup_vs_cpu"] = out["value"] / cpu_rate
+    out["parity"] = {"trajectories": len(errs), "max_abs_population_error": max(errs), "bar": "1e-7 absolute"}
+    return out
+
+
 def adjoint_leg(steps: int, warmup: int) -> dict:
     """Gradients of the photon-count loss on 256 n=39 trajectories.
 
@@ -198,6 +298,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
                     help="skip the adjoint-gradient leg")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint-gradient leg")
+    ap.add_argument("--no-uneven", dest="uneven", action="store_false", help="skip the uneven-batch leg")
+    ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -228,6 +330,9 @@ def main() -> int:
         return 0
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(args.steps, args.warmup)))
+        return 0
+    if args.uneven_only:
+        print(json.dumps(uneven_leg(min(args.steps, 3), 1, args.cpu_seconds)))
         return 0
     system, params, rho0, t_span, weights = tlf_n39()
     prepared = prepare_lindblad_problem(system, params, backend="gpu")
@@ -419,6 +524,8 @@ def main() -> int:
     }
     if args.adjoint:
         result["adjoint"] = adjoint_leg(args.steps, args.warmup)
+    if args.uneven:
+        result["uneven"] = uneven_leg(min(args.steps, 3), 1, args.cpu_seconds)
     print(json.dumps(result))
     return 0
 
