@@ -13,7 +13,8 @@ before the timed region.
 
 The solver integrates only the density-matrix elements reachable from the
 initial state through the RHS program (23 of the 780 upper-triangle elements
-here; all others stay exactly zero), in single-wave workgroups.
+here; all others stay exactly zero), two trajectories per wave (one per
+32-lane half).
 
 Also reported: the per-trajectory ("plain") GPU path on a seeded sample, the
 CPU oracle (this project's C restatement of the reference's Rust/rayon path,
@@ -420,7 +421,7 @@ def main() -> int:
             "outputs": "final populations + photon count (one launch)",
             "parallelism": "single_gpu",
             "integrated_elements": f"{rinfo['upper_len']} of {info['upper_len']} upper-triangle elements",
-            "kernel_variant": args.variant or rinfo["kernel_variant"],
+            "kernel_variant": args.variant or plan.launch_info(support=None if args.full_plan else support)["variant"],
             "hip_graph": bool(args.graph),
             "threads_per_traj": args.threads_per_traj or rinfo["threads_per_traj"],
         },

@@ -24,6 +24,7 @@
 extern "C" int lb_dev_variant_count(void);
 extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
 extern "C" int lb_dev_variant_rows(int id);
+extern "C" int lb_dev_variant_pair(int id);
 extern "C" int lb_dev_adjoint_variant(int id, int* R, int* maxw, int* rows);
 extern "C" void* lb_dev_adjoint_kernel(int id, int solver);
 extern "C" void* lb_dev_solve_kernel(int id, int solver);
@@ -70,6 +71,7 @@ struct DevImage {
   DevPlan dev{};
   std::vector<void*> allocations;
   int default_variant = -1, default_W = 0;
+  int pair_variant = -1;  // paired variant for eligible calls (plans of at most 32 elements), or -1
   int64_t lds_bytes = 0;
   int64_t n_pub = 0, gather_terms = 0, self_terms = 0, rows_total = 0, flops = 0, active = 0;
   std::vector<uint8_t> active_mask;  // upper_len
@@ -178,6 +180,7 @@ void pick_config(int S, int rows, int* id_out, int* W_out) {
   for (int id = 0; id < lb_dev_variant_count(); ++id) {
     int R, maxw, waves;
     lb_dev_variant(id, &R, &maxw, &waves);
+    if (lb_dev_variant_pair(id)) continue;  // chosen per call (lb_solve)
     int W = variant_threads(id, S, rows);
     if (!W) continue;
     const int64_t waste = (int64_t)R * W - S;
@@ -203,6 +206,13 @@ int64_t lds_bytes_for(const DevPlan& P) {
   return 8 * (int64_t)(4 * P.n_pub) + 8 * 2 * (int64_t)P.S + 8 * (int64_t)((P.n + 1) & ~1) +
          16 * 8 * (int64_t)(P.K + 1) + 16 * 8 * (int64_t)P.n_slots + 8 * 4 * (int64_t)P.nblk +
          16 * 8 * (int64_t)P.vm_depth + 16 * 128 + 16 * cells + 4 * ((cells + 3) & ~3) + 16;
+}
+
+// Must match carve_pair() in lb_kernels.hip.
+int64_t lds_bytes_pair(const DevPlan& P) {
+  const int64_t g = P.g_rows;  // one block: its rows
+  return g * 64 * 16 + 2 * 8 * (int64_t)(P.K + 1) * 16 + 2 * 8 * (int64_t)P.n_slots * 16 +
+         2 * 8 * (int64_t)P.vm_depth * 16 + 64 * 16 + 128 * 16 + 8 * (int64_t)((P.n + 1) & ~1) + g * 64 * 4 + 16;
 }
 
 int set_lds_limit(const void* fn, size_t lds) {
@@ -421,6 +431,15 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   up(t_f, &P.t_f);
   if (rc != LB_OK) return rc;
   pick_config(S, rows_total, &im->default_variant, &im->default_W);
+  // paired variant: the fewest register rows holding a one-block plan of <= 32 elements
+  im->pair_variant = -1;
+  if (nblk == 1 && n_act <= 32) {
+    int best_g = 1 << 30;
+    for (int id = 0; id < lb_dev_variant_count(); ++id) {
+      const int g = lb_dev_variant_rows(id);
+      if (lb_dev_variant_pair(id) && rows_total <= g && g < best_g) { best_g = g; im->pair_variant = id; }
+    }
+  }
   im->lds_bytes = lds_bytes_for(P);
   im->n_pub = n_pub;
   im->rows_total = rows_total;
@@ -603,9 +622,26 @@ namespace {
 
 // variant: 0 = plan default, k > 0 = kernel variant k - 1.  threads: 0 = the
 // variant's smallest covering workgroup.
-int resolve_config(const DevImage* p, int variant, int threads, int* id, int* W) {
+// pair_ok: the call may run two trajectories per wave (persistent, pushes at
+// every accepted step, dt given, no trace / save plan / tape / replay).
+int resolve_config(const DevImage* p, int variant, int threads, bool pair_ok, int* id, int* W) {
   if (threads < 0 || threads % 64 != 0)
     return fail(LB_ERR_INVALID, "threads_per_traj must be a multiple of 64");
+  if (variant <= 0 && threads == 0 && pair_ok && p->pair_variant >= 0) {
+    *id = p->pair_variant;
+    *W = 64;
+    return LB_OK;
+  }
+  if (variant > 0 && lb_dev_variant_pair(variant - 1)) {
+    const int g = lb_dev_variant_rows(variant - 1);
+    if (!pair_ok || p->pair_variant < 0 || p->rows_total > g)
+      return fail(LB_ERR_UNSUPPORTED,
+                  "a paired variant needs a plan of at most 32 elements (and its gather rows) and a persistent "
+                  "call pushing at every accepted step with dt given (no trace, save plan, tape or replay)");
+    *id = variant - 1;
+    *W = 64;
+    return LB_OK;
+  }
   if (variant <= 0 && threads == 0) {
     *id = p->default_variant;
     *W = p->default_W;
@@ -694,7 +730,7 @@ extern "C" int lb_rhs_eval(const lb_plan* p, int64_t batch, const double* y, con
   c.out = out;
   int id, W;
   const DevImage* im = p->full.get();
-  if ((rc = resolve_config(im, 0, 0, &id, &W))) return rc;
+  if ((rc = resolve_config(im, 0, 0, false, &id, &W))) return rc;
   void* fn = lb_dev_rhs_kernel(id);
   DevPlan P = im->dev;
   void* args[] = {&P, &c};
@@ -832,12 +868,16 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   int rc = fill_call(p, a, &im, &c);
   if (rc) return rc;
   int id, W;
-  if ((rc = resolve_config(im, a->variant, a->threads_per_traj, &id, &W))) return rc;
+  const bool pair_ok = a->mode == LB_MODE_PERSISTENT && !a->trace && a->n_saveat == 0 && !a->tape_h &&
+                       !a->replay_h && std::isfinite(a->dt) && a->dt > 0.0 && a->output_kind != LB_OUT_RATE;
+  if ((rc = resolve_config(im, a->variant, a->threads_per_traj, pair_ok, &id, &W))) return rc;
+  const bool pair = lb_dev_variant_pair(id) != 0;
   void* fn = lb_dev_solve_kernel(id, a->solver);
+  if (!fn) return fail(LB_ERR_UNSUPPORTED, "kernel variant is not built");
   void* reset = lb_dev_queue_reset_kernel();
   hipStream_t s = (hipStream_t)stream;
   DevPlan P = im->dev;
-  const size_t lds = (size_t)im->lds_bytes;
+  const size_t lds = (size_t)(pair ? lds_bytes_pair(P) : im->lds_bytes);
   if ((rc = set_lds_limit(fn, lds))) return rc;
   if (a->mode == LB_MODE_PLAIN) {
     // one launch per trajectory: the per-trajectory GPU path
@@ -869,7 +909,7 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
     }
     grid = std::max(1, per_cu) * std::max(1, p->cus);
   }
-  grid = (int)std::min<int64_t>(grid, a->n_traj);
+  grid = (int)std::min<int64_t>(grid, pair ? (a->n_traj + 1) / 2 : a->n_traj);
   c.traj_offset = 0;
   void* rargs[] = {&c.queue};
   HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
@@ -1050,10 +1090,10 @@ extern "C" int lb_launch_info(const lb_plan* p, const uint8_t* y0_support, int32
   int rc = select_image(const_cast<lb_plan*>(p), y0_support, &im);
   if (rc) return rc;
   int id, W;
-  if ((rc = resolve_config(im, variant, threads_per_traj, &id, &W))) return rc;
+  if ((rc = resolve_config(im, variant, threads_per_traj, true, &id, &W))) return rc;
   void* fn = lb_dev_solve_kernel(id, LB_SOLVER_DOPRI5);
   if (!fn) return fail(LB_ERR_UNSUPPORTED, "kernel variant is not built");
-  const size_t lds = (size_t)im->lds_bytes;
+  const size_t lds = (size_t)(lb_dev_variant_pair(id) ? lds_bytes_pair(im->dev) : im->lds_bytes);
   if ((rc = set_lds_limit(fn, lds))) return rc;
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, W, lds));

@@ -1145,6 +1145,420 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
 }
 
 // ---------------------------------------------------------------------------
+// Paired integrator: two trajectories per wave.
+//
+// For plans of at most 32 elements each half-wave (32 lanes) integrates its
+// own trajectory, so one wave's instruction stream -- the expression machine
+// above all, which runs in six lanes per step whatever the plan size --
+// serves two trajectories.  The arithmetic is that of the single-wave form:
+// the same gather order, stage combinations and controller, and half-wave
+// sums built from the same DPP rows as the full-wave sum plus the exact zeros
+// the other half contributes there, so results are bitwise those of every
+// other variant.  Each half is its own state machine: when its trajectory
+// ends it writes the outputs, takes the next one from the queue and
+// initialises it while the other half keeps stepping.  Lanes of one wave
+// order their LDS traffic with wavefront-scope fences (no workgroup barrier
+// inside the divergent halves).
+//
+// Covered: persistent launches with pushes at every accepted step (final
+// outputs and the weighted integral), a given initial step dt; trace, save
+// plans, tapes and replays take the other kernels.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum over this lane's half-wave, rounded as wave_sum rounds a wave whose
+// other half holds zeros: the 16-lane rows by the same DPP tree, then
+// (row_a + row_b) + (0 + 0).
+__device__ __forceinline__ double half_sum(double x, int lane) {
+  x += dpp_d<0xB1>(x);
+  x += dpp_d<0x4E>(x);
+  x += dpp_d<0x141>(x);
+  x += dpp_d<0x140>(x);
+  const double lo = lane_d(x, 0) + lane_d(x, 16), hi = lane_d(x, 32) + lane_d(x, 48);
+  return (lane < 32 ? lo : hi) + (0.0 + 0.0);
+}
+
+// A block total of the single-wave form (one block): the stored sum is the
+// only non-zero lane of a second wave sum.
+__device__ __forceinline__ double half_total(double x, int lane) { return half_sum(x, lane) + 0.0; }
+
+// The single-wave RHS for the element in slot `sl` of this half: inputs from
+// the lanes of the same half.
+template <int G>
+__device__ __forceinline__ void rhs_pair(const uint32_t* gw, const double2* gf, int rows, int sl, int base,
+                                         double2 selfv, bool diag, double xr, double xi, double& kr, double& ki,
+                                         const dcpx* coef) {
+  double accr = fma(selfv.x, xr, -(selfv.y * xi));
+  double acci = fma(selfv.x, xi, selfv.y * xr);
+  uint32_t w[G];
+  double2 f[G];
+  double sr[G], si[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    if (j < rows) {
+      w[j] = gw[j * 64 + sl];
+      f[j] = gf[j * 64 + sl];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    if (j < rows) {
+      const int src = (int)(w[j] & 31u) | base;
+      sr[j] = __shfl(xr, src, 64);
+      si[j] = __shfl(xi, src, 64);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < G; ++j)
+    if (j < rows) gather_apply(w[j], f[j], coef, sr[j], si[j], accr, acci);
+  kr = accr;
+  ki = diag ? 0.0 : acci;
+}
+
+struct LdsPair {
+  double2* gf;    // g_rows x 64 gather factors
+  dcpx* coef;     // 2 halves x 8 rows x (K + 1): [0] t0, [2..7] stage times
+  dcpx* slots;    // 2 halves x 8 x n_slots
+  dcpx* vm;       // 2 halves x 8 lanes x vm_depth
+  double2* selfc; // 2 halves x 32 folded self coefficients
+  double2* park;  // 2 x 64: y and k1 around the expression machine
+  double* wts;    // n integral weights
+  uint32_t* gw;   // g_rows x 64 gather words
+  int* misc;      // [0..1] queue values, [2..3] support flags
+};
+
+__device__ __forceinline__ LdsPair carve_pair(const DevPlan& P) {
+  extern __shared__ double smem[];
+  const int g = cst(P.blk_rows)[0];
+  char* q = reinterpret_cast<char*>(smem);
+  LdsPair L;
+  L.gf = reinterpret_cast<double2*>(q); q += (size_t)g * 64 * 16;
+  L.coef = reinterpret_cast<dcpx*>(q); q += (size_t)2 * 8 * (P.K + 1) * 16;
+  L.slots = reinterpret_cast<dcpx*>(q); q += (size_t)2 * 8 * P.n_slots * 16;
+  L.vm = reinterpret_cast<dcpx*>(q); q += (size_t)2 * 8 * P.vm_depth * 16;
+  L.selfc = reinterpret_cast<double2*>(q); q += (size_t)64 * 16;
+  L.park = reinterpret_cast<double2*>(q); q += (size_t)128 * 16;
+  L.wts = reinterpret_cast<double*>(q); q += (size_t)((P.n + 1) & ~1) * 8;
+  L.gw = reinterpret_cast<uint32_t*>(q); q += (size_t)g * 64 * 4;
+  L.misc = reinterpret_cast<int*>(q);
+  return L;
+}
+
+}  // namespace
+
+template <int SOLVER, int G, int WAVES>
+__global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan P, const DevCall C) {
+  using T = Tab<SOLVER>;
+  const int lane = threadIdx.x, half = lane >> 5, sl = lane & 31, base = lane & 32;
+  const LdsPair L = carve_pair(P);
+  const int KK = P.K + 1;
+  const int grows = cst(P.blk_rows)[0];
+  dcpx* const coef = L.coef + half * 8 * KK;
+  dcpx* const slots = L.slots + half * 8 * P.n_slots;
+  Slots<1> S;
+  load_slots<1>(P, S, sl, 64);
+  const bool diag = S.diag(0);
+  for (int i = lane; i < grows * 64; i += 64) {
+    L.gf[i] = reinterpret_cast<const double2*>(P.g_f)[i];
+    L.gw[i] = P.g_word[i];
+  }
+  for (int i = lane; i < P.n; i += 64) {
+    double w = 0.0;
+    for (int q = 0; q < C.n_weights; ++q)
+      if (C.weight_index[q] == i) w += C.weight_value[q];
+    L.wts[i] = w;
+  }
+  wave_sync();
+  const double my_w = (S.valid(0) && diag) ? L.wts[S.state(0)] : 0.0;
+
+  // per-half trajectory state
+  bool active = false, done = false;
+  int64_t traj = 0;
+  double t1 = 0.0, x = 0.0, h = 0.0, hmax = 0.0, fac_old = 1.0e-4;
+  double integral = 0.0, last_t = 0.0, last_v = 0.0, fail_x = 0.0;
+  int64_t accepted = 0, rejected = 0, rhs_calls = 0, ns = 0, nst = 0, ia = 0;
+  int status = 0;
+  bool reject = false, last = false;
+  double2 selfv = double2{0.0, 0.0};
+  double y_r[1] = {0.0}, y_i[1] = {0.0};
+  double k_r[8][1], k_i[8][1];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    k_r[j][0] = 0.0;
+    k_i[j][0] = 0.0;
+  }
+  const double beta = 0.04, alpha = 0.2 - beta * 0.75, facc1 = 1.0 / 0.2, facc2 = 1.0 / 10.0;
+  const double safety = 0.9, nanv = __builtin_nan("");
+
+  for (;;) {
+    // ---- halves without a trajectory take the next one
+    const bool need = !active && !done;
+    if (need && sl == 0) L.misc[half] = atomicAdd(C.queue, 1);
+    wave_sync();
+    if (need) {
+      const int64_t local = L.misc[half];
+      if (local >= C.n_traj) {
+        done = true;
+      } else {
+        traj = C.queue_order ? (int64_t)C.queue_order[local + C.traj_offset] : local + C.traj_offset;
+        active = traj >= 0 && traj < C.n_total;  // entries outside the batch are skipped
+      }
+    }
+    if (__all(done)) break;
+    // ---- initialise a new trajectory (rust/src/ode/dopri5.rs:160-220)
+    const bool starting = need && active;
+    if (starting) {
+      t1 = C.t1_per_traj ? C.t1_per_traj[traj] : C.t1;
+      for (int i = sl; i < P.n_slots; i += 32)
+        slots[i] = i < P.n_base ? mk(P.base[2 * i], P.base[2 * i + 1]) : mk(0.0, 0.0);
+      wave_sync();
+      if (sl == 0) {
+        apply_overrides(C, traj, slots);
+        eval_coefficients(P, slots, coef, C.t0, false, L.vm + half * 8 * P.vm_depth);
+      }
+      wave_sync();
+      for (int i = sl; i < 7 * P.n_slots; i += 32) slots[P.n_slots + i] = slots[i % P.n_slots];
+      for (int i = sl; i < 7 * KK; i += 32) coef[KK + i] = coef[i % KK];
+      wave_sync();  // the stage rows are read by the expression-machine lanes
+      {  // folded self coefficient of this element (fold_self arithmetic)
+        dcpx acc = mk(0.0, 0.0);
+        for (int j = 0; j < P.max_self; ++j) {
+          const size_t e = (size_t)sl * P.max_self + j;
+          const int id = P.self_coef[e];
+          if (id == -2) break;
+          const dcpx f = mk(P.self_f[2 * e], P.self_f[2 * e + 1]);
+          dcpx c = id < 0 ? mk(1.0, 0.0) : coef[id];
+          if (P.self_conj[e]) c.im = -c.im;
+          acc = cadd(acc, cmul(f, c));
+        }
+        selfv = double2{acc.re, acc.im};
+      }
+      const double* y0 = C.y0 + (C.y0_shared ? 0 : traj * (int64_t)P.dim);
+      y_r[0] = S.valid(0) ? y0[P.slot_re[sl]] : 0.0;
+      y_i[0] = (S.valid(0) && !diag) ? y0[P.slot_im[sl]] : 0.0;
+      bool bad_support = false;
+      if (P.restricted) {
+        if (sl == 0) L.misc[2 + half] = 0;
+        wave_sync();
+        for (int k = sl; k < P.dim; k += 32)
+          if (!P.upper_active[P.packed_upper[k]] && y0[k] != 0.0) L.misc[2 + half] = 1;
+        wave_sync();
+        bad_support = L.misc[2 + half] != 0;
+      }
+      accepted = rejected = rhs_calls = 0;
+      status = 0;
+      fail_x = 0.0;
+      integral = 0.0;
+      last_t = __builtin_nan("");
+      last_v = 0.0;
+      x = C.t0;
+      reject = false;
+      last = false;
+      fac_old = 1.0e-4;
+      ns = nst = ia = 0;
+      hmax = fabs(t1 - C.t0);
+      if (bad_support) {
+        status = LB_TRAJ_BAD_SUPPORT;
+        fail_x = C.t0;
+        last = true;
+      } else if (t1 < C.t0) {
+        status = LB_TRAJ_BAD_SPAN;
+        fail_x = C.t0;
+        last = true;
+      } else {
+        // pushed point t0 (save_start): the trapezoid's first node
+        if (C.save_start && C.want_integral) {
+          last_v = half_total(S.valid(0) && diag ? my_w * y_r[0] : 0.0, lane);
+          last_t = C.t0;
+        }
+        if (t1 == C.t0) {
+          last = true;
+        } else {
+          rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], coef);
+          h = fmin(C.dt, t1 - C.t0);
+          rhs_calls = 1;
+        }
+      }
+    }
+    // ---- one step attempt of every half that is integrating
+    const bool stepping = active && !last;
+    if (stepping) {
+      if (ns > C.maxiters) {
+        status = LB_TRAJ_MAXITERS;
+        fail_x = x;
+        last = true;
+      } else if (0.1 * fabs(h) <= DBL_EPSILON * fabs(x)) {
+        status = LB_TRAJ_UNDERFLOW;
+        fail_x = x;
+        last = true;
+      }
+    }
+    const bool attempt = stepping && !last;
+    bool finishing = false;
+    if (attempt) {
+      if (x + 1.01 * h - t1 > 0.0) {
+        h = t1 - x;
+        finishing = true;
+      }
+      ++ns;
+      L.park[lane] = double2{y_r[0], y_i[0]};
+      L.park[64 + lane] = double2{k_r[1][0], k_i[1][0]};
+      if (P.time_dependent && sl < 6) {
+        const int st = sl + 2;
+        eval_coefficients(P, slots + st * P.n_slots, coef + st * KK, x + h * T::c(st), true,
+                          L.vm + (half * 8 + sl) * P.vm_depth);
+      }
+      wave_sync();  // stage coefficients are read by every lane of the half
+      {
+        const double2 py = L.park[lane], pk = L.park[64 + lane];
+        y_r[0] = py.x;
+        y_i[0] = py.y;
+        k_r[1][0] = pk.x;
+        k_i[1][0] = pk.y;
+      }
+      const bool need_stiff = ((accepted + 1) % 1000 == 0) || ia > 0;
+      double yt_r[1], yt_i[1], yn_r[1], yn_i[1];
+#define LB_PAIR_STAGE(st)                                                                            \
+  stage_combo<1, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                                    \
+  rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0],   \
+              coef + (st) * KK);
+      LB_PAIR_STAGE(2)
+      LB_PAIR_STAGE(3)
+      LB_PAIR_STAGE(4)
+      LB_PAIR_STAGE(5)
+      LB_PAIR_STAGE(6)
+#undef LB_PAIR_STAGE
+      stage_combo<1, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yn_r, yn_i);
+      double stiff_d = 0.0;
+      if (need_stiff) {
+        const double a0 = yn_r[0] - yt_r[0], a1 = yn_i[0] - yt_i[0];
+        stiff_d = a0 * a0 + a1 * a1;
+      }
+      rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, yn_r[0], yn_i[0], k_r[7][0], k_i[7][0], coef + 7 * KK);
+      rhs_calls += 6;
+      // ---- error norm, integrand, stiffness (as the single-wave form)
+      double er = T::e(1) * k_r[1][0], ei = T::e(1) * k_i[1][0];
+#pragma unroll
+      for (int j = 2; j <= 7; ++j) {
+        if (T::e(j) != 0.0) {
+          er = fma(T::e(j), k_r[j][0], er);
+          ei = fma(T::e(j), k_i[j][0], ei);
+        }
+      }
+      er *= h;
+      ei *= h;
+      const double sr = C.abstol + fmax(fabs(y_r[0]), fabs(yn_r[0])) * C.reltol;
+      const double si = C.abstol + fmax(fabs(y_i[0]), fabs(yn_i[0])) * C.reltol;
+      const double qr = er / sr, qi = ei / si;
+      const double e2 = half_total(S.valid(0) ? qr * qr + (diag ? 0.0 : qi * qi) : 0.0, lane);
+      const double err = sqrt(e2 / (double)P.dim);
+      const double fac11 = pow(err, alpha);
+      double fac = fac11 * pow(fac_old, -beta);
+      fac = fmax(facc2, fmin(facc1, fac / safety));
+      double hn = h / fac;
+      if (err <= 1.0) {
+        fac_old = fmax(err, 1.0e-4);
+        if (fabs(hn) > hmax) hn = hmax;
+        if (reject) hn = fmin(fabs(hn), fabs(h));
+        reject = false;
+        ++accepted;
+        bool stiff = false;
+        if (accepted % 1000 == 0 || ia > 0) {
+          double a0 = k_r[7][0] - k_r[6][0], a1 = k_i[7][0] - k_i[6][0];
+          const double n2 = half_total(a0 * a0 + a1 * a1, lane), dd2 = half_total(stiff_d, lane);
+          const double hl = dd2 > 0.0 ? h * sqrt(n2 / dd2) : 0.0;
+          if (hl > 3.25) {
+            ++ia;
+            nst = 0;
+            if (ia == 15) {
+              status = LB_TRAJ_STIFF;
+              fail_x = x;
+              stiff = true;
+            }
+          } else {
+            ++nst;
+            if (nst == 6) ia = 0;
+          }
+        }
+        if (stiff) {
+          last = true;
+        } else {
+          x += h;
+          if (C.want_integral) {
+            const double v = half_total(S.valid(0) && diag ? my_w * yn_r[0] : 0.0, lane);
+            if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (x - last_t);
+            last_t = x;
+            last_v = v;
+          }
+          y_r[0] = yn_r[0];
+          y_i[0] = yn_i[0];
+          k_r[1][0] = k_r[7][0];
+          k_i[1][0] = k_i[7][0];
+          if (finishing) last = true;
+          h = hn;
+        }
+      } else {
+        hn = h / fmin(facc1, fac11 / safety);
+        reject = true;
+        if (accepted >= 1) ++rejected;
+        h = hn;
+      }
+    }
+    // ---- a finished trajectory writes its outputs (the single-wave form's)
+    if (active && last) {
+      const bool ok = status == 0;
+      if (P.restricted) {
+        if (C.output_kind == LB_OUT_POPULATIONS) {
+          for (int i = sl; i < P.n; i += 32)
+            if (!P.upper_active[P.packed_upper[i]]) C.out_values[traj * P.n + i] = ok ? 0.0 : nanv;
+        } else if (C.output_kind == LB_OUT_FULL) {
+          for (int k = sl; k < P.dim; k += 32)
+            if (!P.upper_active[P.packed_upper[k]]) C.out_values[traj * P.dim + k] = ok ? 0.0 : nanv;
+        } else if (C.output_kind == LB_OUT_SELECTED) {
+          for (int q = sl; q < C.n_selected; q += 32)
+            if (!P.upper_active[C.sel_upper[q]]) {
+              C.out_values[2 * (traj * C.n_selected + q)] = ok ? 0.0 : nanv;
+              C.out_values[2 * (traj * C.n_selected + q) + 1] = ok ? 0.0 : nanv;
+            }
+        }
+      }
+      if (S.valid(0)) {
+        if (C.output_kind == LB_OUT_POPULATIONS) {
+          if (diag) C.out_values[traj * P.n + S.state(0)] = ok ? y_r[0] : nanv;
+        } else if (C.output_kind == LB_OUT_FULL) {
+          C.out_values[traj * P.dim + P.slot_re[sl]] = ok ? y_r[0] : nanv;
+          if (!diag) C.out_values[traj * P.dim + P.slot_im[sl]] = ok ? y_i[0] : nanv;
+        } else if (C.output_kind == LB_OUT_SELECTED) {
+          const int up = P.slot_up[sl];
+          for (int q = 0; q < C.n_selected; ++q) {
+            if (C.sel_upper[q] != up) continue;
+            const double im = (C.sel_flags[q] & 1) ? 0.0 : ((C.sel_flags[q] & 2) ? -y_i[0] : y_i[0]);
+            C.out_values[2 * (traj * C.n_selected + q)] = ok ? y_r[0] : nanv;
+            C.out_values[2 * (traj * C.n_selected + q) + 1] = ok ? im : nanv;
+          }
+        }
+      }
+      if (sl == 0) {
+        if (C.output_kind == LB_OUT_INTEGRAL) C.out_values[traj] = ok ? integral : nanv;
+        if (C.out_integral) C.out_integral[traj] = ok ? integral : nanv;
+        C.out_stats[4 * traj + 0] = accepted;
+        C.out_stats[4 * traj + 1] = rejected;
+        C.out_stats[4 * traj + 2] = rhs_calls;
+        C.out_stats[4 * traj + 3] = status;
+        C.out_fail_t[traj] = fail_x;
+      }
+      active = false;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Discrete adjoint of the accepted Dopri5/Tsit5 steps.
 //
 // For the weighted trapezoid I over the pushed points (t0 when saving the
@@ -1926,6 +2340,13 @@ void* adjoint_ptr(int solver) {
     return nullptr;
 }
 template <int I, int GG, int VV>
+void* pair_solve_ptr(int solver) {
+  if constexpr (I % LB_NPART == LB_PART)
+    return solver == 1 ? (void*)&lb_solve_pair_kernel<1, GG, VV> : (void*)&lb_solve_pair_kernel<0, GG, VV>;
+  else
+    return nullptr;
+}
+template <int I, int GG, int VV>
 void* adjoint_wave_ptr(int solver) {
   if constexpr (I % LB_NPART == LB_PART)
     return solver == 1 ? (void*)&lb_adjoint_wave_kernel<1, GG, VV> : (void*)&lb_adjoint_wave_kernel<0, GG, VV>;
@@ -1964,6 +2385,10 @@ extern "C" void* LB_PART_FN(lb_dev_solve_kernel)(int id, int solver) {
   if (id == i) return wave_solve_ptr<i, g, v>(solver);
   LB_WAVE_CONFIGS(Y)
 #undef Y
+#define Q(i, g, v) \
+  if (id == i) return pair_solve_ptr<i, g, v>(solver);
+  LB_PAIR_CONFIGS(Q)
+#undef Q
   return nullptr;
 }
 

@@ -25,6 +25,9 @@ extern "C" int lb_dev_variant_count(void) {
 #define Y(i, g, v) ++n;
   LB_WAVE_CONFIGS(Y)
 #undef Y
+#define Q(i, g, v) ++n;
+  LB_PAIR_CONFIGS(Q)
+#undef Q
   return n;
 }
 
@@ -37,6 +40,19 @@ extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves) {
   if (id == i) { *R = 1; *maxw = 64; *waves = v; return 1; }
   LB_WAVE_CONFIGS(Y)
 #undef Y
+#define Q(i, g, v)                                                                              \
+  if (id == i) { *R = 1; *maxw = 64; *waves = v; return 1; }
+  LB_PAIR_CONFIGS(Q)
+#undef Q
+  return 0;
+}
+
+// 1 for the paired (two trajectories per wave) variants.
+extern "C" int lb_dev_variant_pair(int id) {
+#define Q(i, g, v) \
+  if (id == i) return 1;
+  LB_PAIR_CONFIGS(Q)
+#undef Q
   return 0;
 }
 
@@ -46,6 +62,10 @@ extern "C" int lb_dev_variant_rows(int id) {
   if (id == i) return g;
   LB_WAVE_CONFIGS(Y)
 #undef Y
+#define Q(i, g, v) \
+  if (id == i) return g;
+  LB_PAIR_CONFIGS(Q)
+#undef Q
   return 0;
 }
 

@@ -13,6 +13,12 @@
 // held in registers, waves-per-SIMD hint); one slot per lane, 64 threads.
 #define LB_WAVE_CONFIGS(Y) Y(11, 8, 4) Y(12, 8, 2) Y(13, 16, 2) Y(14, 12, 2)
 
+// Paired variants for plans of at most 32 elements: two trajectories per wave
+// (one per 32-lane half); Q(id, gather rows in registers, waves-per-SIMD
+// hint).  Only for persistent calls that push at every accepted step with a
+// given initial step (no trace, save plan, tape or replay).
+#define LB_PAIR_CONFIGS(Q) Q(15, 8, 2) Q(16, 12, 2)
+
 // Adjoint kernel variants: Z(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
 #define LB_ADJ_CONFIGS(Z) Z(0, 1, 1024, 2) Z(1, 2, 512, 2) Z(2, 4, 256, 2) Z(3, 8, 256, 1)
 

@@ -262,3 +262,39 @@ def test_hip_graph_capture_replays_match_eager():
         torch.cuda.synchronize()
         for a, b in zip(eager, captured):
             assert torch.equal(a, b), replay
+
+
+@pytest.mark.parametrize("solver", ["dopri5", "tsit5"])
+def test_paired_variants_are_bitwise_identical(solver):
+    """Two trajectories per wave (the default for plans of at most 32 elements)
+    against the single-wave (13) and multi-wave (1) forms: an odd batch, end
+    times per trajectory including t1 = t0 and t1 < t0, and a queue order."""
+    torch = require_gpu()
+    from centrex_tlf_amd.lindblad.native import NativeError
+
+    system, params, rho0, t_span, weights = tlf_n39()
+    prepared = prepare_lindblad_problem(system, params, backend="host")
+    plan = _device_plan(prepared)
+    rng = np.random.default_rng(29)
+    B = 257
+    slots = [prepared.slot_names.index("delta0"), prepared.slot_names.index("Omega0")]
+    pb = np.stack([rng.uniform(-5, 5, B) * GAMMA, rng.uniform(0.1, 10, B) * GAMMA], axis=1).astype(complex)
+    t1 = rng.uniform(0.3, 1.0, B) * t_span[1]
+    t1[5], t1[17] = t_span[0], -1.0e-9  # an empty span and a backward one
+    order = rng.permutation(B).astype(np.int32)
+    y0 = prepared.layout.pack(rho0)
+    kw = dict(abstol=1e-10, reltol=1e-8, dt=1e-8, solver=solver, output_kind="populations", weights=weights,
+              with_integral=True, override_slots=slots, override_values=pb, queue_order=order)
+    info = plan.launch_info(support=plan.support_of(y0))
+    assert info["variant"] in (16, 17), info  # the paired form is the default here
+    ref = plan.solve(y0, 0.0, t1, **kw)
+    stats = ref[1].cpu().numpy()
+    assert stats[17, 3] == 4 and stats[5, 3] == 0 and stats[5, 2] == 0
+    assert np.all(stats[np.arange(B) != 17, 3] == 0)
+    for variant in (13, 1, 16, 17):
+        other = plan.solve(y0, 0.0, t1, variant=variant, **kw)
+        for a, b, what in zip(ref, other, ("populations", "stats", "fail_t", "integral")):
+            assert torch.equal(a.nan_to_num(7.0), b.nan_to_num(7.0)), (variant, what)
+    with pytest.raises(NativeError, match="paired variant"):
+        plan.solve(y0, 0.0, t_span[1], variant=16, trace=True, capacity=4, abstol=1e-10, reltol=1e-8,
+                   dt=1e-8, output_kind="populations")
