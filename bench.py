@@ -45,7 +45,7 @@ for _p in (str(ROOT / "centrex-tlf_amd"), str(ROOT)):
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 F_RK = {"dopri5": 65, "tsit5": 67}  # algorithmic FLOPs per packed element per attempted step
-TRAFFIC_FILE = ROOT / "profiles" / "traffic_lb_solve_kernel_n39_grid4096.json"
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_lb_solve_pair_kernel_n39_grid4096.json"
 
 
 def log(msg: str) -> None:
