@@ -34,19 +34,32 @@ class DifferentiablePhotonCount:
 
     ``parameters`` names real base parameters (e.g. ``["delta0", "Omega0",
     "theta", "sigma"]``); each trajectory b has its own values ``p[b]``.
+    ``t_span`` is (t0, t1) or (t0, per-trajectory end times); ``fixed`` maps
+    further base parameters to per-trajectory values that are overridden but
+    not differentiated (e.g. a beam width per trajectory).
     """
 
     def __init__(self, prepared: Any, parameters: Sequence[Any], integral_weights: Sequence[tuple[int, float]],
                  t_span: Sequence[float], *, solver: str = "dopri5", abstol: float = 1e-7,
                  reltol: float = 1e-4, dt: float = 1e-8, maxiters: int = 100_000, save_start: bool = True,
-                 ckpt_every: int = 16, tape_capacity: int = 1024):
+                 ckpt_every: int = 16, tape_capacity: int = 1024, fixed: dict[str, Any] | None = None):
         if solver not in D.GPU_SOLVERS:
             raise NotImplementedError("the adjoint supports 'dopri5' and 'tsit5'")
         self.prepared = prepared
         self.names = [D.parameter_slot_name(p) for p in parameters]
         self.slots = D.parameter_slot_indices(prepared, parameters)
         self.weights = [(int(i), float(w)) for i, w in integral_weights]
-        self.t_span = D.normalize_t_span(t_span)
+        if np.ndim(t_span[1]) > 0:
+            t1 = np.ascontiguousarray(t_span[1], dtype=np.float64).reshape(-1)
+            if not np.all(np.isfinite(t1)):
+                raise ValueError("end times must be finite")
+            self.t_span = (float(t_span[0]), t1)
+        else:
+            self.t_span = D.normalize_t_span(t_span)
+        fixed = fixed or {}
+        self.fixed_slots = D.parameter_slot_indices(prepared, list(fixed.keys())) if fixed else []
+        self.fixed_values = (np.stack([np.asarray(v, dtype=np.complex128).reshape(-1) for v in fixed.values()], axis=1)
+                             if fixed else None)
         self.kw = dict(solver=solver, abstol=float(abstol), reltol=float(reltol), dt=float(dt),
                        maxiters=int(maxiters), save_start=bool(save_start))
         self.every = int(ckpt_every)
@@ -65,6 +78,18 @@ class DifferentiablePhotonCount:
         slots = self.plan.support_info(support)["slot_count"]
         return p, y, support, slots
 
+    def _overrides(self, p):
+        """Override slots and (B, P + F) complex values: the parameters, then the fixed ones."""
+        import torch
+
+        vals = p.to(torch.complex128)
+        if self.fixed_values is None:
+            return self.slots, vals
+        fv = torch.as_tensor(self.fixed_values, device=vals.device)
+        if fv.shape[0] != vals.shape[0]:
+            raise ValueError("fixed parameter values need one row per trajectory")
+        return self.slots + self.fixed_slots, torch.cat([vals, fv], dim=1)
+
     def forward(self, params, y0, events=None):
         """Photon counts (B,) and the tape of the solve (``events``: a (start, end)
         pair of torch events recorded around the solver launch)."""
@@ -75,8 +100,8 @@ class DifferentiablePhotonCount:
         while True:
             tape = Tape(p.shape[0], slots, cap, self.every)
             vals, stats, fail_t = self.plan.solve(
-                y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self.slots,
-                override_values=p.to(torch.complex128), support=support, tape=tape, events=events, **self.kw)
+                y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self._overrides(p)[0],
+                override_values=self._overrides(p)[1], support=support, tape=tape, events=events, **self.kw)
             st = stats[:, 3].cpu().numpy()
             if np.any(st == 7):
                 cap = int(tape.n.max().item()) + 1
@@ -95,8 +120,8 @@ class DifferentiablePhotonCount:
 
         p, y, support, _ = self._common(params, y0)
         vals, stats, _ = self.plan.solve(
-            y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self.slots,
-            override_values=p.to(torch.complex128), support=support, replay=tape, **self.kw)
+            y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self._overrides(p)[0],
+            override_values=self._overrides(p)[1], support=support, replay=tape, **self.kw)
         if int(stats[:, 3].abs().max().item()) != 0:
             raise ValueError("replay of the recorded step sizes failed")
         return vals[:, 0]
@@ -111,8 +136,8 @@ class DifferentiablePhotonCount:
         p, y, support, _ = self._common(params, y0)
         grad = torch.empty((p.shape[0], len(self.slots)), dtype=torch.float64, device="cuda")
         return self.plan.solve(
-            y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self.slots,
-            override_values=p.to(torch.complex128), support=support, tape=tape, adjoint_grad=grad,
+            y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self._overrides(p)[0],
+            override_values=self._overrides(p)[1], support=support, tape=tape, adjoint_grad=grad,
             variant=int(variant), events=events, **self.kw)
 
 

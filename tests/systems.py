@@ -68,6 +68,50 @@ def tlf_n39_xy(theta: float = 0.25 * math.pi, sigma: float = 3.0 / GAMMA):
     return system, params, rho0, (0.0, 20.0 / GAMMA), weights
 
 
+def tlf_n72_theta(polarization_frequency: float = 2.0 * math.pi * 1e6):
+    """The n=72 system with a polarisation angle theta rotating the switched pair:
+    PX = cos(theta) sw + sin(theta) (1 - sw), PY = sin(theta) sw + cos(theta) (1 - sw),
+    sw the 1 MHz square wave (theta = 0 is tlf_n72).  Base parameters delta0,
+    Omega0, theta, sigma, t_center (the fit workload)."""
+    system = load_obe_system(GOLDEN / "tlf_r0_f1_3o2_n72.json")
+    params = LindbladParameters()
+    omega0 = params.real("Omega0", GAMMA)
+    delta0 = params.real("delta0", 0.0)
+    th = params.real("theta", 0.0)
+    sigma = params.real("sigma", 5.0 / GAMMA)
+    tc = params.real("t_center", 5.0 / GAMMA)
+    envelope = gaussian(Time(), center=tc, sigma=sigma, amplitude=omega0)
+    the = _to_runtime_expression(th)
+    names = set(str(s) for s in system.H_symbolic.free_symbols)
+    for k in (0, 1):
+        if f"Ω{k}" in names:
+            params.bind(f"Ω{k}", envelope, finalize=False)
+        if f"δ{k}" in names:
+            params.bind(f"δ{k}", delta0, finalize=False)
+        sw = _to_runtime_expression(square_wave(Time(), polarization_frequency, 0.0))
+        c, s_ = smp.cos(the.expr), smp.sin(the.expr)
+        deps = {**dict(the.parameters), **dict(sw.parameters)}
+        if f"PX{k}" in names:
+            params.bind(f"PX{k}", RuntimeExpression(c * sw.expr + s_ * (1 - sw.expr), deps), finalize=False)
+        if f"PY{k}" in names:
+            params.bind(f"PY{k}", RuntimeExpression(s_ * sw.expr + c * (1 - sw.expr), deps), finalize=False)
+    if "Ω2" in names:
+        params.bind("Ω2", 0.5 * GAMMA, finalize=False)
+    if "δ2" in names:
+        params.bind("δ2", 0.0, finalize=False)
+    if "PZ2" in names:
+        params.bind("PZ2", 1.0, finalize=False)
+    params._finalize()
+    n = system.n_states
+    rho0 = np.zeros((n, n), dtype=np.complex128)
+    ground = system.indices(electronic="X", J=0)
+    for i in ground:
+        rho0[i, i] = 1.0 / len(ground)
+    excited = system.indices(electronic="B")
+    weights = [(int(i), GAMMA) for i in excited]
+    return system, params, rho0, (0.0, 2.0 * 5.0 / GAMMA), weights
+
+
 def tlf_n72(polarization_frequency: float = 2.0 * math.pi * 1e6):
     """X(J=0..3) + B(J=1,F1'=3/2,F'=1,2); two lasers with x<->y polarisation switching, microwave J1-J2."""
     system = load_obe_system(GOLDEN / "tlf_r0_f1_3o2_n72.json")
