@@ -164,6 +164,74 @@ def uneven_leg(steps: int, warmup: int, cpu_seconds: float) -> dict:
     return out
 
 
+def fit_leg(per_group: int, iterations: int, t_scale: float) -> dict:
+    """Batched gradient-based fit: per_group n=39 (x/y) + per_group n=72 (theta)
+    trajectories, sigma ~ N(5, 1.5)/Gamma clipped above 1/Gamma (times t_scale),
+    t_center = sigma, t_end = 2 sigma; fit per-trajectory (delta0, Omega0,
+    theta) by Adam (lr 0.05 in units of Gamma, Gamma, 1) to photon counts of
+    seeded true parameters, starting from perturbed ones; Dopri5, rtol 1e-8,
+    checkpoints every 16 accepted steps.  Reports gradients/s, the peak tape
+    memory of one iteration, and the final populations of a sample against the
+    CPU oracle at rtol 1e-8.
+    """
+    import numpy as np
+    import torch
+
+    from centrex_tlf_amd.lindblad import prepare_lindblad_problem
+    from centrex_tlf_amd.lindblad.adjoint import DifferentiablePhotonCount
+    from centrex_tlf_amd.lindblad.fit import FitGroup, fit_photon_counts
+    from oracle.lb_oracle import Oracle
+    from tests.systems import GAMMA, tlf_n39_xy, tlf_n72_theta
+
+    names = ["delta0", "Omega0", "theta"]
+    tol = dict(abstol=1e-10, reltol=1e-8, dt=1e-8)
+    rng = np.random.default_rng(20261202 + 5)
+    groups = []
+    for build in (tlf_n39_xy, tlf_n72_theta):
+        system, params, rho0, _, weights = build()
+        prepared = prepare_lindblad_problem(system, params, backend="gpu")
+        sigma = np.clip(rng.normal(5.0, 1.5, per_group), 1.0, None) / GAMMA * t_scale
+        t_end = 2.0 * sigma
+        true = np.stack([rng.uniform(-3, 3, per_group) * GAMMA, rng.uniform(0.3, 3, per_group) * GAMMA,
+                         rng.uniform(0, np.pi, per_group)], axis=1)
+        fixed = {"sigma": sigma.astype(complex), "t_center": sigma.astype(complex)}
+        eng = DifferentiablePhotonCount(prepared, names, weights, (0.0, t_end), fixed=fixed, **tol)
+        targets, _ = eng.forward(true, prepared.layout.pack(rho0))
+        initial = true * np.asarray([1.0, 1.2, 1.0]) + np.asarray([0.3 * GAMMA, 0.0, 0.2])
+        groups.append(FitGroup(prepared, rho0, t_end, targets.cpu().numpy(), initial, weights, fixed))
+    log(f"bench: fit of {2 * per_group} trajectories x {iterations} iterations")
+    res = fit_photon_counts(groups, names, scales=[GAMMA, GAMMA, 1.0], iterations=iterations, lr=0.05, **tol)
+    # final populations of a sample against the oracle
+    errs = []
+    for grp, p in zip(groups, res.parameters):
+        plan = grp.prepared.device_plan()
+        y0 = grp.prepared.layout.pack(grp.rho0)
+        slots = [grp.prepared.slot_names.index(m) for m in names + ["sigma", "t_center"]]
+        vals = np.concatenate([p, np.stack([grp.fixed["sigma"].real, grp.fixed["t_center"].real], axis=1)], axis=1)
+        pops = plan.solve(y0, 0.0, grp.t_end, output_kind="populations", override_slots=slots,
+                          override_values=vals.astype(complex), **tol)[0].cpu().numpy()
+        oracle = Oracle(grp.prepared)
+        for b in np.argsort(grp.t_end)[:2]:
+            r = oracle.solve_batch(y0, (0.0, float(grp.t_end[b])), output="populations", output_when="final",
+                                   parameter_slots=names + ["sigma", "t_center"],
+                                   parameter_batch=vals[b:b + 1].astype(complex), threads=1, **tol)
+            errs.append(float(np.abs(r["final"][0] - pops[b]).max()))
+    return {
+        "metric": "fit gradients/s",
+        "value": res.gradients_per_second,
+        "unit": "trajectory gradients (delta0, Omega0, theta) per second, forward + adjoint + Adam",
+        "trajectories": 2 * per_group,
+        "iterations": iterations,
+        "seconds": res.seconds,
+        "forward_seconds": res.forward_seconds,
+        "adjoint_seconds": res.adjoint_seconds,
+        "loss_first_last": [res.loss[0], res.loss[-1]],
+        "tape_bytes_peak": res.tape_bytes,
+        "final_population_error_vs_oracle": max(errs),
+        "config": f"{per_group} tlf_n39_xy + {per_group} tlf_n72_theta, sigma scale {t_scale}, rtol 1e-8",
+    }
+
+
 def adjoint_leg(steps: int, warmup: int) -> dict:
     """Gradients of the photon-count loss on 256 n=39 trajectories.
 
@@ -300,6 +368,10 @@ def main() -> int:
                     help="skip the adjoint-gradient leg")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint-gradient leg")
     ap.add_argument("--no-uneven", dest="uneven", action="store_false", help="skip the uneven-batch leg")
+    ap.add_argument("--fit", action="store_true", help="run only the batched fit leg")
+    ap.add_argument("--fit-trajectories", type=int, default=2048, help="trajectories per system in the fit")
+    ap.add_argument("--fit-iterations", type=int, default=50)
+    ap.add_argument("--fit-time-scale", type=float, default=1.0, help="scale of the transit times in the fit")
     ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
@@ -331,6 +403,9 @@ def main() -> int:
         return 0
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(args.steps, args.warmup)))
+        return 0
+    if args.fit:
+        print(json.dumps(fit_leg(args.fit_trajectories, args.fit_iterations, args.fit_time_scale)))
         return 0
     if args.uneven_only:
         print(json.dumps(uneven_leg(min(args.steps, 3), 1, args.cpu_seconds)))

@@ -104,7 +104,9 @@ class DifferentiablePhotonCount:
                 override_values=self._overrides(p)[1], support=support, tape=tape, events=events, **self.kw)
             st = stats[:, 3].cpu().numpy()
             if np.any(st == 7):
-                cap = int(tape.n.max().item()) + 1
+                # the tape was too short: grow it (with a margin the next calls keep)
+                cap = int(int(tape.n.max().item()) * 1.25) + 16
+                self.capacity = cap
                 continue
             bad = np.flatnonzero(st != 0)
             if bad.size:

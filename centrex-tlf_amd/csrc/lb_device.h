@@ -48,6 +48,15 @@ struct DevInstr {
   double re, im, aux;
 };
 
+// The device record of one instruction: 32 bytes, read by the expression
+// machine with one scalar load (code = op | argc << 8 | function << 16).
+struct alignas(32) DevOp {
+  int32_t code;
+  int32_t index;
+  double re, im, aux;
+};
+static_assert(sizeof(DevOp) == 32, "one 32-byte record per instruction");
+
 struct DevPlan {
   int32_t n, dim, upper_len;
   int32_t S;        // padded slot count (multiple of 64)
@@ -87,7 +96,7 @@ struct DevPlan {
   const uint32_t* t_word;
   const double* t_f;
   // expression machine
-  const DevInstr* instr;
+  const DevOp* instr;
   const int32_t* expr_start;    // n_compounds + K + 1
   const int32_t* compound_slot; // n_compounds
   const int32_t* expr_dyn;      // n_compounds + K: 1 if time-dependent

@@ -417,7 +417,18 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   up(blk_rows, &P.blk_rows);
   up(g_word, &P.g_word);
   up(g_f, &P.g_f);
-  up(src.prog.instr, &P.instr);
+  {
+    std::vector<DevOp> ops(src.prog.instr.size());
+    for (size_t i = 0; i < ops.size(); ++i) {
+      const DevInstr& q = src.prog.instr[i];
+      ops[i].code = (q.op & 0xFF) | ((q.argc & 0xFF) << 8) | ((q.function & 0xFFFF) << 16);
+      ops[i].index = q.index;
+      ops[i].re = q.re;
+      ops[i].im = q.im;
+      ops[i].aux = q.aux;
+    }
+    up(ops, &P.instr);
+  }
   up(src.prog.expr_start, &P.expr_start);
   up(src.compound_slot, &P.compound_slot);
   up(src.expr_dyn, &P.expr_dyn);

@@ -175,21 +175,40 @@ __device__ __forceinline__ dcpx dev_powc(dcpx a, double c, double cs, double sn)
 // Stack machine of rust/src/lindblad/eval.rs:791-911 (scalar form).  The plan
 // builder has checked ops, helper ids and the stack depth.  The top of the
 // stack lives in registers and the entries below it in this lane's LDS
-// window `stk` (nothing goes to scratch memory).
-__device__ __forceinline__ dcpx vm_eval(const __attribute__((address_space(4))) DevInstr* ins, int start,
+// window `stk` (nothing goes to scratch memory).  Each 32-byte instruction
+// record comes in with one scalar load, issued one instruction ahead.
+__device__ __forceinline__ DevOp fetch_op(const __attribute__((address_space(4))) DevOp* ins, int p) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(4))) u32x4 cu4;
+  const cu4* w = reinterpret_cast<cu4*>(ins + p);
+  const u32x4 a = w[0], b = w[1];
+  DevOp q;
+  q.code = (int32_t)a.x;
+  q.index = (int32_t)a.y;
+  q.re = __hiloint2double((int)a.w, (int)a.z);
+  q.im = __hiloint2double((int)b.y, (int)b.x);
+  q.aux = __hiloint2double((int)b.w, (int)b.z);
+  return q;
+}
+
+__device__ __forceinline__ dcpx vm_eval(const __attribute__((address_space(4))) DevOp* ins, int start,
                                         int end, const dcpx* slots,
                                         double t, dcpx* stk) {
   dcpx top = mk(0.0, 0.0);
+  if (start >= end) return top;
   int d = 0;  // entries on the stack; entry d-1 is `top`
+  DevOp next = fetch_op(ins, start);
   for (int p = start; p < end; ++p) {
-    const auto& q = ins[p];
-    const int op = q.op;
+    const DevOp q = next;
+    if (p + 1 < end) next = fetch_op(ins, p + 1);
+    const int op = q.code & 0xFF;
     if (op <= 4) {
       if (d > 0) stk[d - 1] = top;
       ++d;
       top = op == 1 ? mk(q.re, q.im) : op == 2 ? slots[q.index] : mk(t, 0.0);
       continue;
     }
+    const int argc = (q.code >> 8) & 0xFF, function = q.code >> 16;
     switch (op) {
       case 5: top = cadd(stk[d - 2], top); --d; break;
       case 6: top = csub(stk[d - 2], top); --d; break;
@@ -199,16 +218,16 @@ __device__ __forceinline__ dcpx vm_eval(const __attribute__((address_space(4))) 
       case 10: top = mk(-top.re, -top.im); break;
       case 11: top.im = -top.im; break;
       case 12: {
-        const dcpx v = q.argc == 1 ? top : stk[d - q.argc];
-        if (q.function == 4 && v.im == 0.0) top = mk(exp(v.re), 0.0);
-        else top = dev_builtin(q.function, v);
-        d -= q.argc - 1;
+        const dcpx v = argc == 1 ? top : stk[d - argc];
+        if (function == 4 && v.im == 0.0) top = mk(exp(v.re), 0.0);
+        else top = dev_builtin(function, v);
+        d -= argc - 1;
         break;
       }
       case 13:
         stk[d - 1] = top;
-        top = dev_helper(q.function, &stk[d - q.argc], q.argc);
-        d -= q.argc - 1;
+        top = dev_helper(function, &stk[d - argc], argc);
+        d -= argc - 1;
         break;
       case LB_OP_POWC: top = dev_powc(top, q.re, q.im, q.aux); break;
       default: {
