@@ -998,9 +998,12 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         }
         __syncthreads();
         const double err = C.replay_h ? 0.0 : sqrt(block_total(P, L.red, 0) / (double)P.dim);
-        // ---- controller (common.rs:48-69)
-        const double fac11 = pow(err, alpha);
-        double fac = fac11 * pow(fac_old, -beta);
+        // ---- controller (common.rs:48-69); the two powers at once in each
+        // wave (lanes 0-31: err^alpha, lanes 32-63: fac_old^-beta)
+        const int wl = tid & 63;
+        const double pw = pow(wl < 32 ? err : fac_old, wl < 32 ? alpha : -beta);
+        const double fac11 = __shfl(pw, 0, 64), fo = __shfl(pw, 32, 64);
+        double fac = fac11 * fo;
         fac = fmax(facc2, fmin(facc1, fac / safety));
         double hn = h / fac;
         if (err <= 1.0) {
@@ -1477,8 +1480,11 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
       const double qr = er / sr, qi = ei / si;
       const double e2 = half_total(S.valid(0) ? qr * qr + (diag ? 0.0 : qi * qi) : 0.0, lane);
       const double err = sqrt(e2 / (double)P.dim);
-      const double fac11 = pow(err, alpha);
-      double fac = fac11 * pow(fac_old, -beta);
+      // the controller's two powers at once: lanes 0-15 of the half take
+      // err^alpha, lanes 16-31 fac_old^-beta (the same pow on the same inputs)
+      const double pw = pow(sl < 16 ? err : fac_old, sl < 16 ? alpha : -beta);
+      const double fac11 = __shfl(pw, base, 64), fo = __shfl(pw, base + 16, 64);
+      double fac = fac11 * fo;
       fac = fmax(facc2, fmin(facc1, fac / safety));
       double hn = h / fac;
       if (err <= 1.0) {
