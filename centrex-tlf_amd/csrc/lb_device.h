@@ -57,6 +57,12 @@ struct alignas(32) DevOp {
 };
 static_assert(sizeof(DevOp) == 32, "one 32-byte record per instruction");
 
+// One expression of the program: instructions [start, end) and where the
+// value goes: dst >= 0 a compound's slot, dst < 0 coefficient row entry -dst-1.
+struct alignas(16) DevExpr {
+  int32_t start, end, dst, pad;
+};
+
 struct DevPlan {
   int32_t n, dim, upper_len;
   int32_t S;        // padded slot count (multiple of 64)
@@ -100,6 +106,12 @@ struct DevPlan {
   const int32_t* expr_start;    // n_compounds + K + 1
   const int32_t* compound_slot; // n_compounds
   const int32_t* expr_dyn;      // n_compounds + K: 1 if time-dependent
+  // expression records in evaluation order: every expression (the solver's
+  // n_compounds + K, then the derivative programs), and the time-dependent
+  // ones only, with the solver's count first (prefixes of each list)
+  const DevExpr* ex_all;
+  const DevExpr* ex_dyn;
+  int32_t n_all_fwd, n_all, n_dyn_fwd, n_dyn;
   int32_t vm_depth;             // deepest expression stack
   const double* base;           // n_base*2
 };

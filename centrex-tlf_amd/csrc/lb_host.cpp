@@ -432,6 +432,35 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   up(src.prog.expr_start, &P.expr_start);
   up(src.compound_slot, &P.compound_slot);
   up(src.expr_dyn, &P.expr_dyn);
+  {
+    const int nc = src.prog.n_compounds, K = src.K, KD = src.K * (1 + src.n_diff);
+    std::vector<DevExpr> all, dyn;
+    int n_dyn_fwd = 0;
+    for (int e = 0; e < nc + KD; ++e) {
+      DevExpr r{};
+      r.start = src.prog.expr_start[e];
+      r.end = src.prog.expr_start[e + 1];
+      if (e < nc) {
+        r.dst = src.prog.compound_slot[e];
+      } else {
+        const int j = e - nc;
+        r.dst = -((j < K ? j : j + 1) + 1);
+      }
+      all.push_back(r);
+      if (src.expr_dyn[e]) {
+        dyn.push_back(r);
+        if (e < nc + K) ++n_dyn_fwd;
+      }
+    }
+    P.n_all_fwd = nc + K;
+    P.n_all = nc + KD;
+    P.n_dyn_fwd = n_dyn_fwd;
+    P.n_dyn = (int32_t)dyn.size();
+    if (all.empty()) all.push_back(DevExpr{});
+    if (dyn.empty()) dyn.push_back(DevExpr{});
+    up(all, &P.ex_all);
+    up(dyn, &P.ex_dyn);
+  }
   up(src.base, &P.base);
   up(act, &P.upper_active);
   up(packed_upper, &P.packed_upper);

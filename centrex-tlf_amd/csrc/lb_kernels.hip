@@ -542,21 +542,35 @@ __device__ __forceinline__ double time_tol(double t) { return 10.0 * DBL_EPSILON
 
 // Evaluate the parameter graph + coefficients for time t into slot copy `sl`
 // and coefficient row `cf` (compounds first, in dependency order, then the
-// coefficients); only_dynamic skips the time-independent expressions.
-// n_prog coefficient programs are evaluated (P.K for the solver, P.KD with
-// the derivative programs for the adjoint); program j lands in cf[j] for
-// j < K and in cf[j + 1] after the constant entry cf[K].
+// coefficients); only_dynamic runs the time-dependent expressions only.
+// n_prog < 0 evaluates the solver's K coefficient programs, n_prog = P.KD
+// also the derivative programs of a differentiable plan; program j lands in
+// cf[j] for j < K and in cf[j + 1] after the constant entry cf[K].  Each
+// expression record comes in with one scalar load, issued one ahead.
+__device__ __forceinline__ DevExpr fetch_expr(const __attribute__((address_space(4))) DevExpr* ex, int i) {
+  typedef int i32x4 __attribute__((ext_vector_type(4)));
+  const i32x4 a = reinterpret_cast<const __attribute__((address_space(4))) i32x4*>(ex)[i];
+  DevExpr r;
+  r.start = a.x;
+  r.end = a.y;
+  r.dst = a.z;
+  r.pad = 0;
+  return r;
+}
+
 __device__ __forceinline__ void eval_coefficients(const DevPlan& P, dcpx* sl, dcpx* cf, double t,
                                                   bool only_dynamic, dcpx* stk, int n_prog = -1) {
-  const int n_expr = P.n_compounds + (n_prog < 0 ? P.K : n_prog);
-  for (int e = 0; e < n_expr; ++e) {
-    if (only_dynamic && !cst(P.expr_dyn)[e]) continue;
-    const dcpx v = vm_eval(cst(P.instr), cst(P.expr_start)[e], cst(P.expr_start)[e + 1], sl, t, stk);
-    if (e < P.n_compounds) {
-      sl[cst(P.compound_slot)[e]] = v;
-    } else {
-      const int j = e - P.n_compounds;
-      cf[j < P.K ? j : j + 1] = v;
+  const bool all_programs = n_prog >= 0 && n_prog > P.K;
+  const __attribute__((address_space(4))) DevExpr* ex = cst(only_dynamic ? P.ex_dyn : P.ex_all);
+  const int n = only_dynamic ? (all_programs ? P.n_dyn : P.n_dyn_fwd) : (all_programs ? P.n_all : P.n_all_fwd);
+  if (n > 0) {
+    DevExpr next = fetch_expr(ex, 0);
+    for (int i = 0; i < n; ++i) {
+      const DevExpr r = next;
+      if (i + 1 < n) next = fetch_expr(ex, i + 1);
+      const dcpx v = vm_eval(cst(P.instr), r.start, r.end, sl, t, stk);
+      if (r.dst >= 0) sl[r.dst] = v;
+      else cf[-r.dst - 1] = v;
     }
   }
   cf[P.K] = mk(1.0, 0.0);
