@@ -40,10 +40,17 @@ extern "C" {
 #define LB_TRAJ_CAPACITY 7    /* more trace rows than lb_solve_args.capacity (the integration
                                  completed; counts and stats are exact, rows past capacity
                                  were dropped).  No reference counterpart. */
+#define LB_TRAJ_BAD_DT 8      /* fixed-step solver without a positive finite dt
+                                 ("fixed-step solvers require positive finite dt", fixed.rs:154-156) */
 
 /* Solvers (rust/src/ode/batch.rs:7-27). */
 #define LB_SOLVER_DOPRI5 0
 #define LB_SOLVER_TSIT5 1
+/* fixed-step solvers of rust/src/ode/fixed.rs:133-249: steps of dt, shortened
+ * to land on t1 and on save-plan points, no error control */
+#define LB_SOLVER_FIXED_DOPRI5 2
+#define LB_SOLVER_FIXED_RK2 3
+#define LB_SOLVER_FIXED_RK4 4
 
 /* Output kinds (rust/src/ode/output.rs). */
 #define LB_OUT_POPULATIONS 0 /* final diagonal, width n                          */

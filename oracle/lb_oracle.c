@@ -926,10 +926,135 @@ static void fill_stage(const tableau* tb, int s, int dim, const double* y, doubl
   }
 }
 
+/* ---------------- fixed-step solvers (rust/src/ode/fixed.rs:133-249) ------ */
+/* Steps of dt, shortened to land on t1 and on the next save-plan point, with
+ * the stage sums written in the reference's operation order; no error
+ * control.  Pushes after every step, or at the save-plan points (which the
+ * steps land on). */
+static int solve_fixed(const lbo_plan* p, workspace* w, const lbo_solve_args* a, const double* y0,
+                       double t0, double t1, collector* out, ode_stats* st) {
+  int dim = p->dim;
+  memset(st, 0, sizeof(*st));
+  if (a->maxiters <= 0) return fail("maxiters must be positive");
+  if (t1 < t0) { st->status = LB_TRAJ_BAD_SPAN; st->fail_x = t0; return fail("only forward integration"); }
+  if (!isfinite(a->dt) || a->dt <= 0.0) {
+    st->status = LB_TRAJ_BAD_DT; st->fail_x = t0;
+    return fail("fixed-step solvers require positive finite dt");
+  }
+  if (t1 == t0) {
+    if (a->save_start) coll_push(out, t0, y0);
+    return 0;
+  }
+  double* sp = NULL;
+  int64_t nsp = 0;
+  int has_sp = a->saveat != NULL;
+  if (has_sp) {
+    sp = malloc(sizeof(double) * (a->n_saveat + 1));
+    if (a->save_start) sp[nsp++] = t0;
+    for (int64_t q = 0; q < a->n_saveat; ++q) {
+      double tt = a->saveat[q];
+      if (a->save_start && close_to_start(tt, t0)) continue;
+      if (!isfinite(tt)) { free(sp); return fail("saveat contains non-finite value"); }
+      if (tt < t0 - time_tol(t0) || tt > t1 + time_tol(t1)) {
+        free(sp);
+        return fail("saveat value %g is outside [%g, %g]", tt, t0, t1);
+      }
+      sp[nsp++] = tt;
+    }
+  }
+  double* buf = calloc((size_t)dim * 9, sizeof(double));
+  double *y = buf, *yn = buf + dim, *yt = buf + 2 * dim;
+  double *k1 = buf + 3 * dim, *k2 = buf + 4 * dim, *k3 = buf + 5 * dim, *k4 = buf + 6 * dim,
+         *k5 = buf + 7 * dim, *k6 = buf + 8 * dim;
+  memcpy(y, y0, sizeof(double) * dim);
+  int rc = 0;
+  int64_t si = 0, steps = 0;
+  double x = t0;
+  if (has_sp) {
+    while (si < nsp && close_to_start(sp[si], t0)) coll_push(out, sp[si++], y);
+  } else if (a->save_start) {
+    coll_push(out, x, y);
+  }
+  while (x < t1 - time_tol(t1)) {
+    if (steps >= a->maxiters) {
+      st->status = LB_TRAJ_MAXITERS; st->fail_x = x;
+      rc = fail("Stopped at x=%.17g. Need more than %lld steps.", x, (long long)steps);
+      goto done;
+    }
+    double target = fmin(x + a->dt, t1);
+    if (has_sp && si < nsp && sp[si] > x + time_tol(x)) target = fmin(target, sp[si]);
+    const double h = target - x;
+    if (h <= 0.0 || 0.1 * h <= DBL_EPSILON * fmax(fabs(x), 1.0)) {
+      st->status = LB_TRAJ_UNDERFLOW; st->fail_x = x;
+      rc = fail("Stopped at x=%.17g. Step size underflow.", x);
+      goto done;
+    }
+    int calls = 0;
+    if (a->solver == LB_SOLVER_FIXED_RK2) {
+      if ((rc = rhs_eval(w, y, x, k1))) goto done;
+      for (int i = 0; i < dim; ++i) yt[i] = y[i] + 0.5 * h * k1[i];
+      if ((rc = rhs_eval(w, yt, x + 0.5 * h, k2))) goto done;
+      for (int i = 0; i < dim; ++i) yn[i] = y[i] + h * k2[i];
+      calls = 2;
+    } else if (a->solver == LB_SOLVER_FIXED_RK4) {
+      if ((rc = rhs_eval(w, y, x, k1))) goto done;
+      for (int i = 0; i < dim; ++i) yt[i] = y[i] + 0.5 * h * k1[i];
+      if ((rc = rhs_eval(w, yt, x + 0.5 * h, k2))) goto done;
+      for (int i = 0; i < dim; ++i) yt[i] = y[i] + 0.5 * h * k2[i];
+      if ((rc = rhs_eval(w, yt, x + 0.5 * h, k3))) goto done;
+      for (int i = 0; i < dim; ++i) yt[i] = y[i] + h * k3[i];
+      if ((rc = rhs_eval(w, yt, x + h, k4))) goto done;
+      for (int i = 0; i < dim; ++i) yn[i] = y[i] + h * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) / 6.0;
+      calls = 4;
+    } else {
+      if ((rc = rhs_eval(w, y, x, k1))) goto done;
+      for (int i = 0; i < dim; ++i) yt[i] = y[i] + h * (1.0 / 5.0) * k1[i];
+      if ((rc = rhs_eval(w, yt, x + h * (1.0 / 5.0), k2))) goto done;
+      for (int i = 0; i < dim; ++i) yt[i] = y[i] + h * ((3.0 / 40.0) * k1[i] + (9.0 / 40.0) * k2[i]);
+      if ((rc = rhs_eval(w, yt, x + h * (3.0 / 10.0), k3))) goto done;
+      for (int i = 0; i < dim; ++i)
+        yt[i] = y[i] + h * ((44.0 / 45.0) * k1[i] - (56.0 / 15.0) * k2[i] + (32.0 / 9.0) * k3[i]);
+      if ((rc = rhs_eval(w, yt, x + h * (4.0 / 5.0), k4))) goto done;
+      for (int i = 0; i < dim; ++i)
+        yt[i] = y[i] + h * ((19372.0 / 6561.0) * k1[i] - (25360.0 / 2187.0) * k2[i] +
+                            (64448.0 / 6561.0) * k3[i] - (212.0 / 729.0) * k4[i]);
+      if ((rc = rhs_eval(w, yt, x + h * (8.0 / 9.0), k5))) goto done;
+      for (int i = 0; i < dim; ++i)
+        yt[i] = y[i] + h * ((9017.0 / 3168.0) * k1[i] - (355.0 / 33.0) * k2[i] + (46732.0 / 5247.0) * k3[i] +
+                            (49.0 / 176.0) * k4[i] - (5103.0 / 18656.0) * k5[i]);
+      if ((rc = rhs_eval(w, yt, x + h, k6))) goto done;
+      for (int i = 0; i < dim; ++i)
+        yn[i] = y[i] + h * ((35.0 / 384.0) * k1[i] + (500.0 / 1113.0) * k3[i] + (125.0 / 192.0) * k4[i] -
+                            (2187.0 / 6784.0) * k5[i] + (11.0 / 84.0) * k6[i]);
+      calls = 6;
+    }
+    st->rhs_calls += calls;
+    st->accepted += 1;
+    steps += 1;
+    x += h;
+    if (has_sp) {
+      while (si < nsp && sp[si] <= x + time_tol(x)) coll_push(out, sp[si++], yn);
+    } else {
+      coll_push(out, x, yn);
+    }
+    double* tmp = y; y = yn; yn = tmp;
+  }
+  if (has_sp && si != nsp) {
+    st->status = LB_TRAJ_SAVEAT; st->fail_x = (double)si;
+    rc = fail("ended before all saveat: %lld of %lld", (long long)si, (long long)nsp);
+  }
+done:
+  if (rc && st->status == LB_TRAJ_OK) st->status = 5;
+  free(buf);
+  free(sp);
+  return rc;
+}
+
 static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, const double* y0,
                      double t0, double t1, collector* out, ode_stats* st) {
   int dim = p->dim;
   memset(st, 0, sizeof(*st));
+  if (a->solver >= LB_SOLVER_FIXED_DOPRI5) return solve_fixed(p, w, a, y0, t0, t1, out, st);
   if (a->maxiters <= 0) return fail("maxiters must be positive");
   if (t1 < t0) { st->status = LB_TRAJ_BAD_SPAN; st->fail_x = t0; return fail("only forward integration"); }
   if (t1 == t0) {

@@ -165,7 +165,7 @@ class Oracle:
         kind = ("integral" if output in INTEGRAL_OUTPUTS else "rate" if output in RATE_OUTPUTS
                 else output)
         a = LboSolveArgs()
-        a.solver = 1 if solver == "tsit5" else 0
+        a.solver = {"dopri5": 0, "tsit5": 1, "fixed_dopri5": 2, "fixed_rk2": 3, "fixed_rk4": 4}[solver]
         a.t0, a.t1 = float(t_span[0]), float(t_span[1])
         a.abstol, a.reltol, a.dt = float(abstol), float(reltol), float(dt)
         a.maxiters = int(maxiters)
