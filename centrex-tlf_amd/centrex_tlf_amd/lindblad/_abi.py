@@ -25,12 +25,12 @@ __all__ = [
 ]
 
 LB_OK = 0
-LB_SOLVER = {"dopri5": 0, "tsit5": 1}
+LB_SOLVER = {"dopri5": 0, "tsit5": 1, "fixed_dopri5": 2, "fixed_rk2": 3, "fixed_rk4": 4}
 LB_OUT = {"populations": 0, "selected": 1, "integral": 2, "full": 3, "rate": 4}
 LB_MODE_PERSISTENT = 0
 LB_MODE_PLAIN = 1
 LB_TRAJ = {0: "ok", 1: "maxiters", 2: "underflow", 3: "stiff", 4: "bad_span", 5: "bad_support",
-           6: "saveat", 7: "capacity"}
+           6: "saveat", 7: "capacity", 8: "bad_dt"}
 
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)

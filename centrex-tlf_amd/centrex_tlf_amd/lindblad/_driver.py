@@ -21,14 +21,16 @@ from .parameters import Parameter
 # reference's expanded-sparse mode; "gpu_plain" launches one trajectory at a time
 GPU_MODES = {"expanded_sparse": "persistent", "gpu": "persistent", "gpu_plain": "plain"}
 REFERENCE_HOST_MODES = {"reference", "structured", "structured_upper"}
-GPU_SOLVERS = {"dopri5", "tsit5"}
+GPU_SOLVERS = {"dopri5", "tsit5", "fixed_dopri5", "fixed_rk2", "fixed_rk4"}
+ADAPTIVE_SOLVERS = {"dopri5", "tsit5"}
+FIXED_SOLVERS = {"fixed_dopri5", "fixed_rk2", "fixed_rk4"}
 REFERENCE_SOLVERS = {"dopri5", "tsit5", "fixed_dopri5", "fixed_rk2", "fixed_rk4", "scipy_rk45", "scipy_bdf",
                      "scipy_radau", "python_rk45"}
 INTEGRAL_OUTPUTS = {"weighted_integral", "photon_integral", "excited_population"}
 RATE_OUTPUTS = {"weighted_rate", "photon_rate", "excited_population_rate"}
 WEIGHTED_OUTPUTS = INTEGRAL_OUTPUTS | RATE_OUTPUTS
 STATUS_MESSAGES = {1: "maxiters", 2: "underflow", 3: "stiff", 4: "bad_span", 5: "bad_support",
-                   6: "saveat", 7: "capacity"}
+                   6: "saveat", 7: "capacity", 8: "bad_dt"}
 
 
 def rust_float(x: float) -> str:
@@ -55,9 +57,11 @@ def check_mode_and_solver(execution_mode: str, solver: str) -> str:
     if solver not in GPU_SOLVERS:
         if solver in REFERENCE_SOLVERS:
             raise NotImplementedError(
-                f"solver={solver!r} is not available on the GPU; use 'dopri5' or 'tsit5'"
+                f"solver={solver!r} is not available on the GPU; use 'dopri5', 'tsit5', 'fixed_dopri5', "
+                "'fixed_rk2' or 'fixed_rk4'"
             )
-        raise NotImplementedError("supported solvers are 'dopri5' and 'tsit5'")
+        raise NotImplementedError(
+            "supported solvers are 'dopri5', 'tsit5', 'fixed_dopri5', 'fixed_rk2' and 'fixed_rk4'")
     return GPU_MODES[execution_mode]
 
 
@@ -165,10 +169,12 @@ def check_outputs(output: str, output_indices: Any, output_when: str, integral_w
         raise ValueError(f"output={output!r} requires explicit saveat values")
 
 
-def trajectory_message(status: int, fail_t: float, *, maxiters: int, n_plan: int) -> str:
-    """The reference's error text for a failed trajectory."""
+def trajectory_message(status: int, fail_t: float, *, maxiters: int, n_plan: int, solver: str = "dopri5") -> str:
+    """The reference's error text for a failed trajectory (dopri5.rs:226-231,
+    fixed.rs:154-156, 185-190: the fixed-step loop counts steps from 0)."""
     if status == 1:
-        return f"Stopped at x={rust_float(fail_t)}. Need more than {maxiters + 1} steps."
+        steps = maxiters if solver in FIXED_SOLVERS else maxiters + 1
+        return f"Stopped at x={rust_float(fail_t)}. Need more than {steps} steps."
     if status == 2:
         return f"Stopped at x={rust_float(fail_t)}. Step size underflow."
     if status == 3:
@@ -179,6 +185,8 @@ def trajectory_message(status: int, fail_t: float, *, maxiters: int, n_plan: int
         return f"ended before all saveat: {int(fail_t)} of {n_plan}"
     if status == 5:
         return "initial state is nonzero outside the declared support"
+    if status == 8:
+        return "fixed-step solvers require positive finite dt"
     return f"solver failed with status {status}"
 
 

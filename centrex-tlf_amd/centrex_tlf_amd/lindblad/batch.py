@@ -132,7 +132,7 @@ def _run(prepared: PreparedLindbladProblem, y0: np.ndarray, t_span: tuple[float,
     if bad.size:
         i = int(bad[0])
         msg = D.trajectory_message(int(run.stats[i, 3]), float(run.fail_t[i]), maxiters=maxiters,
-                                   n_plan=0 if save_plan is None else save_plan.size)
+                                   n_plan=0 if save_plan is None else save_plan.size, solver=solver)
         raise ValueError(f"trajectory {i}: {msg}")
     if trace:
         time_count = int(run.counts[0])

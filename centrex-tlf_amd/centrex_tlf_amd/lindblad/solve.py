@@ -140,7 +140,7 @@ def solve_lindblad(
     elapsed = time.perf_counter() - start
     if status != 0:
         raise ValueError(D.trajectory_message(status, float(run.fail_t[0]), maxiters=maxiters,
-                                              n_plan=0 if save_plan is None else save_plan.size))
+                                              n_plan=0 if save_plan is None else save_plan.size, solver=solver))
     if trace:
         count = int(run.counts[0])
         rows = run.values[0, :count]

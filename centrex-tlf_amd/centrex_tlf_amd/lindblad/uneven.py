@@ -258,7 +258,8 @@ def solve_uneven_batch(groups: Sequence[UnevenGroup], *, t0: float = 0.0, solver
         bad = np.flatnonzero(stats[:, 3] != 0)
         if bad.size:
             i = int(bad[0])
-            msg = D.trajectory_message(int(stats[i, 3]), float(fail_t[i]), maxiters=maxiters, n_plan=0)
+            msg = D.trajectory_message(int(stats[i, 3]), float(fail_t[i]), maxiters=maxiters, n_plan=0,
+                                       solver=solver)
             raise ValueError(f"group {g}, trajectory {i}: {msg}")
         photons = out[3].cpu().numpy() if grp.integral_weights is not None else None
         results.append(UnevenGroupResult(values, photons, stats))

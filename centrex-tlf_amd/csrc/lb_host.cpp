@@ -26,6 +26,8 @@ extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
 extern "C" int lb_dev_variant_rows(int id);
 extern "C" int lb_dev_variant_pair(int id);
 extern "C" int lb_dev_adjoint_variant(int id, int* R, int* maxw, int* rows);
+extern "C" int lb_dev_fixed_variant(int id, int* R, int* maxw);
+extern "C" void* lb_dev_fixed_kernel(int id, int solver);
 extern "C" void* lb_dev_adjoint_kernel(int id, int solver);
 extern "C" void* lb_dev_solve_kernel(int id, int solver);
 extern "C" void* lb_dev_rhs_kernel(int id);
@@ -789,8 +791,8 @@ int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out,
   if (!a->y0 || !a->out_values || !a->out_stats || !a->out_fail_t)
     return fail(LB_ERR_INVALID, "y0 and output buffers are required");
   if (a->maxiters <= 0) return fail(LB_ERR_INVALID, "maxiters must be positive");
-  if (a->solver != LB_SOLVER_DOPRI5 && a->solver != LB_SOLVER_TSIT5)
-    return fail(LB_ERR_INVALID, "solver must be dopri5 or tsit5");
+  if (a->solver < LB_SOLVER_DOPRI5 || a->solver > LB_SOLVER_FIXED_RK4)
+    return fail(LB_ERR_INVALID, "solver must be dopri5, tsit5, fixed_dopri5, fixed_rk2 or fixed_rk4");
   if (!std::isfinite(a->t0) || (!a->t1_per_traj && !std::isfinite(a->t1)))
     return fail(LB_ERR_INVALID, "t_span must be finite");
   if (a->n_traj > INT32_MAX) return fail(LB_ERR_UNSUPPORTED, "at most 2^31-1 trajectories per call");
@@ -900,6 +902,58 @@ int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out,
 
 }  // namespace
 
+namespace {
+
+// Fixed-step solvers: the multi-wave fixed kernel with the fewest slots per
+// thread that covers the plan (or the given threads per trajectory).
+int launch_fixed(const lb_plan* p, const lb_solve_args* a, const DevImage* im, DevCall& c, void* stream) {
+  if (a->tape_h || a->replay_h)
+    return fail(LB_ERR_UNSUPPORTED, "adjoint tapes and replays need an adaptive solver (dopri5 or tsit5)");
+  const int S = im->dev.S;
+  int id = -1, W = 0, R = 0, maxw = 0;
+  for (int v = 0; lb_dev_fixed_variant(v, &R, &maxw); ++v) {
+    const int w = a->threads_per_traj > 0 ? a->threads_per_traj : std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
+    if (w <= maxw && (int64_t)R * w >= S) { id = v; W = w; break; }
+  }
+  if (id < 0) return fail(LB_ERR_UNSUPPORTED, "no fixed-step kernel variant covers the plan");
+  void* fn = lb_dev_fixed_kernel(id, a->solver);
+  if (!fn) return fail(LB_ERR_UNSUPPORTED, "fixed-step kernel variant is not built");
+  void* reset = lb_dev_queue_reset_kernel();
+  hipStream_t s = (hipStream_t)stream;
+  DevPlan P = im->dev;
+  const size_t lds = (size_t)im->lds_bytes;
+  int rc = set_lds_limit(fn, lds);
+  if (rc) return rc;
+  if (a->mode == LB_MODE_PLAIN) {
+    for (int64_t i = 0; i < a->n_traj; ++i) {
+      c.n_traj = 1;
+      c.traj_offset = i;
+      void* rargs[] = {&c.queue};
+      HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
+      void* args[] = {&P, &c};
+      HIP_TRY(hipLaunchKernel(fn, dim3(1), dim3(W), args, lds, s));
+    }
+    HIP_TRY(hipGetLastError());
+    return LB_OK;
+  }
+  int grid = a->workgroups;
+  if (grid <= 0) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, W, lds));
+    grid = std::max(1, per_cu) * std::max(1, p->cus);
+  }
+  grid = (int)std::min<int64_t>(grid, a->n_traj);
+  c.traj_offset = 0;
+  void* rargs[] = {&c.queue};
+  HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
+  void* args[] = {&P, &c};
+  HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
+  HIP_TRY(hipGetLastError());
+  return LB_OK;
+}
+
+}  // namespace
+
 extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) {
   if (!p || !a) return fail(LB_ERR_INVALID, "null argument");
   if (a->n_traj <= 0) return LB_OK;
@@ -907,6 +961,7 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   DevCall c{};
   int rc = fill_call(p, a, &im, &c);
   if (rc) return rc;
+  if (a->solver >= LB_SOLVER_FIXED_DOPRI5) return launch_fixed(p, a, im, c, stream);
   int id, W;
   const bool pair_ok = a->mode == LB_MODE_PERSISTENT && !a->trace && a->n_saveat == 0 && !a->tape_h &&
                        !a->replay_h && std::isfinite(a->dt) && a->dt > 0.0 && a->output_kind != LB_OUT_RATE;
@@ -984,6 +1039,8 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   if (!p || !a) return fail(LB_ERR_INVALID, "null argument");
   if (a->n_traj <= 0) return LB_OK;
   if (p->src.n_diff <= 0) return fail(LB_ERR_INVALID, "the plan has no differentiated parameters");
+  if (a->solver != LB_SOLVER_DOPRI5 && a->solver != LB_SOLVER_TSIT5)
+    return fail(LB_ERR_UNSUPPORTED, "the adjoint differentiates the adaptive solvers (dopri5, tsit5)");
   if (!a->tape_h || !a->out_grad) return fail(LB_ERR_INVALID, "lb_adjoint needs the recorded tape and out_grad");
   if (a->output_kind != LB_OUT_INTEGRAL || a->n_weights <= 0)
     return fail(LB_ERR_INVALID, "the adjoint differentiates the weighted integral output");

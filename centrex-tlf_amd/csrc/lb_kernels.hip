@@ -1181,6 +1181,334 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
 }
 
 // ---------------------------------------------------------------------------
+// Fixed-step integrators (rust/src/ode/fixed.rs:133-249): steps of dt,
+// shortened to land on t1 and on the next save-plan point, Dopri5 (6 RHS),
+// midpoint RK2 (2) or classical RK4 (4) stages with the stage sums written in
+// the reference's operation order, no error control; a push after every step
+// or at the save-plan points.  Persistent multi-wave form (any plan size).
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int ST>  // 0 Dopri5, 1 RK2, 2 RK4
+struct FixedStep {
+  static constexpr int NS = ST == 0 ? 6 : ST == 1 ? 2 : 4;
+  // time of stage s (1-based), as the reference writes it
+  __device__ static double time(double x, double h, int s) {
+    if constexpr (ST == 1) return s == 1 ? x : x + 0.5 * h;
+    else if constexpr (ST == 2) return s == 1 ? x : s == 4 ? x + h : x + 0.5 * h;
+    else
+      return s == 1 ? x : s == 2 ? x + h * (1.0 / 5.0) : s == 3 ? x + h * (3.0 / 10.0)
+           : s == 4 ? x + h * (4.0 / 5.0) : s == 5 ? x + h * (8.0 / 9.0) : x + h;
+  }
+};
+
+// Outputs of a finished trajectory (final values, integral, stats), as the
+// adaptive kernel writes them.
+template <int R>
+__device__ void write_final(const DevPlan& P, const DevCall& C, const Slots<R>& S, int tid, int W, int64_t traj,
+                            const double (&y_r)[R], const double (&y_i)[R], int status, double fail_x,
+                            double integral, int64_t accepted, int64_t rhs_calls, int64_t n_pushed) {
+  const bool ok = status == 0;
+  const double nanv = __builtin_nan("");
+  if (C.trace) {
+    if (tid == 0) {
+      C.out_counts[traj] = (int32_t)n_pushed;
+      if (C.out_integral) C.out_integral[traj] = ok ? integral : nanv;
+      C.out_stats[4 * traj + 0] = accepted;
+      C.out_stats[4 * traj + 1] = 0;
+      C.out_stats[4 * traj + 2] = rhs_calls;
+      C.out_stats[4 * traj + 3] = status;
+      C.out_fail_t[traj] = fail_x;
+    }
+    return;
+  }
+  if (P.restricted) {
+    if (C.output_kind == LB_OUT_POPULATIONS) {
+      for (int i = tid; i < P.n; i += W)
+        if (!P.upper_active[P.packed_upper[i]]) C.out_values[traj * P.n + i] = ok ? 0.0 : nanv;
+    } else if (C.output_kind == LB_OUT_FULL) {
+      for (int k = tid; k < P.dim; k += W)
+        if (!P.upper_active[P.packed_upper[k]]) C.out_values[traj * P.dim + k] = ok ? 0.0 : nanv;
+    } else if (C.output_kind == LB_OUT_SELECTED) {
+      for (int q = tid; q < C.n_selected; q += W)
+        if (!P.upper_active[C.sel_upper[q]]) {
+          C.out_values[2 * (traj * C.n_selected + q)] = ok ? 0.0 : nanv;
+          C.out_values[2 * (traj * C.n_selected + q) + 1] = ok ? 0.0 : nanv;
+        }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!S.valid(r)) continue;
+    const int s = tid + W * r;
+    if (C.output_kind == LB_OUT_POPULATIONS) {
+      if (S.diag(r)) C.out_values[traj * P.n + S.state(r)] = ok ? y_r[r] : nanv;
+    } else if (C.output_kind == LB_OUT_FULL) {
+      C.out_values[traj * P.dim + P.slot_re[s]] = ok ? y_r[r] : nanv;
+      if (!S.diag(r)) C.out_values[traj * P.dim + P.slot_im[s]] = ok ? y_i[r] : nanv;
+    } else if (C.output_kind == LB_OUT_SELECTED) {
+      const int up = P.slot_up[s];
+      for (int q = 0; q < C.n_selected; ++q) {
+        if (C.sel_upper[q] != up) continue;
+        const double im = (C.sel_flags[q] & 1) ? 0.0 : ((C.sel_flags[q] & 2) ? -y_i[r] : y_i[r]);
+        C.out_values[2 * (traj * C.n_selected + q)] = ok ? y_r[r] : nanv;
+        C.out_values[2 * (traj * C.n_selected + q) + 1] = ok ? im : nanv;
+      }
+    }
+  }
+  if (tid == 0) {
+    if (C.output_kind == LB_OUT_INTEGRAL) C.out_values[traj] = ok ? integral : nanv;
+    if (C.out_integral) C.out_integral[traj] = ok ? integral : nanv;
+    C.out_stats[4 * traj + 0] = accepted;
+    C.out_stats[4 * traj + 1] = 0;
+    C.out_stats[4 * traj + 2] = rhs_calls;
+    C.out_stats[4 * traj + 3] = status;
+    C.out_fail_t[traj] = fail_x;
+  }
+}
+
+}  // namespace
+
+template <int R, int ST, int MAXW, int WAVES>
+__global__ __launch_bounds__(MAXW, WAVES) void lb_fixed_kernel(const DevPlan P, const DevCall C) {
+  using F = FixedStep<ST>;
+  constexpr int NS = F::NS;
+  const int tid = threadIdx.x, W = blockDim.x;
+  const Lds L = carve(P);
+  const int KK = P.K + 1;
+  Slots<R> S;
+  load_slots<R>(P, S, tid, W);
+  stage_tables(P, L, tid, W);
+
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) L.misc[0] = atomicAdd(C.queue, 1);
+    __syncthreads();
+    const int64_t local = L.misc[0];
+    if (local >= C.n_traj) break;
+    const int64_t traj = C.queue_order ? (int64_t)C.queue_order[local + C.traj_offset] : local + C.traj_offset;
+    if (traj < 0 || traj >= C.n_total) continue;
+    const double t0 = C.t0;
+    const double t1 = C.t1_per_traj ? C.t1_per_traj[traj] : C.t1;
+
+    // ---- parameters and coefficients at t0, self-term folds, weights
+    init_slots(P, L.slots, tid, W);
+    if (tid == 0) {
+      apply_overrides(C, traj, L.slots);
+      eval_coefficients(P, L.slots, L.coef, t0, false, L.vm);
+    }
+    for (int i = tid; i < P.n; i += W) {
+      double w = 0.0;
+      for (int q = 0; q < C.n_weights; ++q)
+        if (C.weight_index[q] == i) w += C.weight_value[q];
+      L.wts[i] = w;
+    }
+    __syncthreads();
+    for (int i = tid; i < 7 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
+    for (int i = tid; i < 7 * KK; i += W) L.coef[KK + i] = L.coef[i % KK];
+    fold_self(P, tid, W, L.coef, L.selfc);
+
+    double y_r[R], y_i[R];
+    const double* y0 = C.y0 + (C.y0_shared ? 0 : traj * (int64_t)P.dim);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int s = tid + W * r;
+      y_r[r] = S.valid(r) ? y0[P.slot_re[s]] : 0.0;
+      y_i[r] = (S.valid(r) && !S.diag(r)) ? y0[P.slot_im[s]] : 0.0;
+    }
+    bool bad_support = false;
+    if (P.restricted) {
+      if (tid == 0) L.misc[1] = 0;
+      __syncthreads();
+      for (int k = tid; k < P.dim; k += W)
+        if (!P.upper_active[P.packed_upper[k]] && y0[k] != 0.0) L.misc[1] = 1;
+      __syncthreads();
+      bad_support = L.misc[1] != 0;
+    }
+
+    int64_t accepted = 0, rhs_calls = 0, n_pushed = 0, si = 0;
+    int status = 0;
+    double fail_x = 0.0, integral = 0.0, last_t = __builtin_nan(""), last_v = 0.0;
+    const bool step_push = C.n_saveat == 0;
+    const bool rate = C.output_kind == LB_OUT_RATE;
+    auto push = [&](double tp, const double (&vr)[R], const double (&vi)[R]) {
+      double v = last_v;
+      if (C.want_integral || rate) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          block_store(P, tid, W, r, L.red, 1, S.diag(r) && S.valid(r) ? L.wts[S.state(r)] * vr[r] : 0.0);
+        __syncthreads();
+        v = block_total(P, L.red, 1);
+        __syncthreads();
+        if (C.want_integral) {
+          if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (tp - last_t);
+          last_t = tp;
+          last_v = v;
+        }
+      }
+      if (C.trace) {
+        if (n_pushed < C.capacity) {
+          const int64_t row = traj * C.capacity + n_pushed;
+          if (C.output_kind == LB_OUT_INTEGRAL) {
+            if (tid == 0) C.out_values[row] = integral;
+          } else if (rate) {
+            if (tid == 0) C.out_values[row] = v;
+          } else {
+            write_row<R>(P, C, S, tid, W, row, vr, vi);
+          }
+          if (tid == 0 && C.out_times) C.out_times[row] = tp;
+        }
+        ++n_pushed;
+      }
+    };
+    double k_r[7][R], k_i[7][R];
+    int buf = 0;
+#define LB_FRHS(xr, xi, kr, ki, cf)                                         \
+  do {                                                                      \
+    rhs_eval<R>(P, L, S, xr, xi, kr, ki, L.pub + buf * 2 * P.n_pub, cf, tid, W); \
+    buf ^= 1;                                                               \
+  } while (0)
+
+    if (bad_support) {
+      status = LB_TRAJ_BAD_SUPPORT;
+      fail_x = t0;
+    } else if (t1 < t0) {
+      status = LB_TRAJ_BAD_SPAN;
+      fail_x = t0;
+    } else if (!(isfinite(C.dt) && C.dt > 0.0)) {
+      status = LB_TRAJ_BAD_DT;
+      fail_x = t0;
+    } else if (t1 == t0) {
+      if (C.save_start) push(t0, y_r, y_i);
+    } else {
+      if (!step_push) {
+        while (si < C.n_saveat && fabs(C.saveat[si] - t0) <= time_tol(t0)) {
+          push(C.saveat[si], y_r, y_i);
+          ++si;
+        }
+      } else if (C.save_start) {
+        push(t0, y_r, y_i);
+      }
+      double x = t0;
+      int64_t steps = 0;
+      while (x < t1 - time_tol(t1)) {
+        if (steps >= C.maxiters) { status = LB_TRAJ_MAXITERS; fail_x = x; break; }
+        double target = fmin(x + C.dt, t1);
+        if (!step_push && si < C.n_saveat && C.saveat[si] > x + time_tol(x)) target = fmin(target, C.saveat[si]);
+        const double h = target - x;
+        if (h <= 0.0 || 0.1 * h <= DBL_EPSILON * fmax(fabs(x), 1.0)) { status = LB_TRAJ_UNDERFLOW; fail_x = x; break; }
+        __syncthreads();  // the previous step's RHS have read their coefficient rows
+        if (P.time_dependent && tid < NS)
+          eval_coefficients(P, L.slots + (tid + 1) * P.n_slots, L.coef + (tid + 1) * KK, F::time(x, h, tid + 1),
+                            true, L.vm + tid * P.vm_depth);
+        // (the barrier inside the first RHS orders the rows before the gathers)
+        double yt_r[R], yt_i[R], yn_r[R], yn_i[R];
+        LB_FRHS(y_r, y_i, k_r[1], k_i[1], L.coef + KK);
+        if constexpr (ST == 1) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            yt_r[r] = y_r[r] + 0.5 * h * k_r[1][r];
+            yt_i[r] = y_i[r] + 0.5 * h * k_i[1][r];
+          }
+          LB_FRHS(yt_r, yt_i, k_r[2], k_i[2], L.coef + 2 * KK);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            yn_r[r] = y_r[r] + h * k_r[2][r];
+            yn_i[r] = y_i[r] + h * k_i[2][r];
+          }
+        } else if constexpr (ST == 2) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            yt_r[r] = y_r[r] + 0.5 * h * k_r[1][r];
+            yt_i[r] = y_i[r] + 0.5 * h * k_i[1][r];
+          }
+          LB_FRHS(yt_r, yt_i, k_r[2], k_i[2], L.coef + 2 * KK);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            yt_r[r] = y_r[r] + 0.5 * h * k_r[2][r];
+            yt_i[r] = y_i[r] + 0.5 * h * k_i[2][r];
+          }
+          LB_FRHS(yt_r, yt_i, k_r[3], k_i[3], L.coef + 3 * KK);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            yt_r[r] = y_r[r] + h * k_r[3][r];
+            yt_i[r] = y_i[r] + h * k_i[3][r];
+          }
+          LB_FRHS(yt_r, yt_i, k_r[4], k_i[4], L.coef + 4 * KK);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            yn_r[r] = y_r[r] + h * (k_r[1][r] + 2.0 * k_r[2][r] + 2.0 * k_r[3][r] + k_r[4][r]) / 6.0;
+            yn_i[r] = y_i[r] + h * (k_i[1][r] + 2.0 * k_i[2][r] + 2.0 * k_i[3][r] + k_i[4][r]) / 6.0;
+          }
+        } else {
+#define LB_FSUM(expr_r, expr_i)            \
+  _Pragma("unroll") for (int r = 0; r < R; ++r) { \
+    yt_r[r] = y_r[r] + h * (expr_r);       \
+    yt_i[r] = y_i[r] + h * (expr_i);       \
+  }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            yt_r[r] = y_r[r] + h * (1.0 / 5.0) * k_r[1][r];
+            yt_i[r] = y_i[r] + h * (1.0 / 5.0) * k_i[1][r];
+          }
+          LB_FRHS(yt_r, yt_i, k_r[2], k_i[2], L.coef + 2 * KK);
+          LB_FSUM((3.0 / 40.0) * k_r[1][r] + (9.0 / 40.0) * k_r[2][r],
+                  (3.0 / 40.0) * k_i[1][r] + (9.0 / 40.0) * k_i[2][r])
+          LB_FRHS(yt_r, yt_i, k_r[3], k_i[3], L.coef + 3 * KK);
+          LB_FSUM((44.0 / 45.0) * k_r[1][r] - (56.0 / 15.0) * k_r[2][r] + (32.0 / 9.0) * k_r[3][r],
+                  (44.0 / 45.0) * k_i[1][r] - (56.0 / 15.0) * k_i[2][r] + (32.0 / 9.0) * k_i[3][r])
+          LB_FRHS(yt_r, yt_i, k_r[4], k_i[4], L.coef + 4 * KK);
+          LB_FSUM((19372.0 / 6561.0) * k_r[1][r] - (25360.0 / 2187.0) * k_r[2][r] + (64448.0 / 6561.0) * k_r[3][r] -
+                      (212.0 / 729.0) * k_r[4][r],
+                  (19372.0 / 6561.0) * k_i[1][r] - (25360.0 / 2187.0) * k_i[2][r] + (64448.0 / 6561.0) * k_i[3][r] -
+                      (212.0 / 729.0) * k_i[4][r])
+          LB_FRHS(yt_r, yt_i, k_r[5], k_i[5], L.coef + 5 * KK);
+          LB_FSUM((9017.0 / 3168.0) * k_r[1][r] - (355.0 / 33.0) * k_r[2][r] + (46732.0 / 5247.0) * k_r[3][r] +
+                      (49.0 / 176.0) * k_r[4][r] - (5103.0 / 18656.0) * k_r[5][r],
+                  (9017.0 / 3168.0) * k_i[1][r] - (355.0 / 33.0) * k_i[2][r] + (46732.0 / 5247.0) * k_i[3][r] +
+                      (49.0 / 176.0) * k_i[4][r] - (5103.0 / 18656.0) * k_i[5][r])
+          LB_FRHS(yt_r, yt_i, k_r[6], k_i[6], L.coef + 6 * KK);
+#undef LB_FSUM
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            yn_r[r] = y_r[r] + h * ((35.0 / 384.0) * k_r[1][r] + (500.0 / 1113.0) * k_r[3][r] +
+                                    (125.0 / 192.0) * k_r[4][r] - (2187.0 / 6784.0) * k_r[5][r] +
+                                    (11.0 / 84.0) * k_r[6][r]);
+            yn_i[r] = y_i[r] + h * ((35.0 / 384.0) * k_i[1][r] + (500.0 / 1113.0) * k_i[3][r] +
+                                    (125.0 / 192.0) * k_i[4][r] - (2187.0 / 6784.0) * k_i[5][r] +
+                                    (11.0 / 84.0) * k_i[6][r]);
+          }
+        }
+        rhs_calls += NS;
+        ++accepted;
+        ++steps;
+        x += h;
+        if (step_push) {
+          push(x, yn_r, yn_i);
+        } else {
+          while (si < C.n_saveat && C.saveat[si] <= x + time_tol(x)) {
+            push(C.saveat[si], yn_r, yn_i);
+            ++si;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          y_r[r] = yn_r[r];
+          y_i[r] = yn_i[r];
+        }
+      }
+      if (status == 0 && !step_push && si != C.n_saveat) {
+        status = LB_TRAJ_SAVEAT;
+        fail_x = (double)si;
+      }
+    }
+#undef LB_FRHS
+    if (status == 0 && C.trace && n_pushed > C.capacity) status = LB_TRAJ_CAPACITY;
+    write_final<R>(P, C, S, tid, W, traj, y_r, y_i, status, fail_x, integral, accepted, rhs_calls, n_pushed);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Paired integrator: two trajectories per wave.
 //
 // For plans of at most 32 elements each half-wave (32 lanes) integrates its
@@ -2378,6 +2706,15 @@ void* adjoint_ptr(int solver) {
   else
     return nullptr;
 }
+template <int I, int RR, int WW, int VV>
+void* fixed_ptr(int solver) {
+  if constexpr (I % LB_NPART == LB_PART)
+    return solver == 2 ? (void*)&lb_fixed_kernel<RR, 0, WW, VV>
+         : solver == 3 ? (void*)&lb_fixed_kernel<RR, 1, WW, VV>
+                       : (void*)&lb_fixed_kernel<RR, 2, WW, VV>;
+  else
+    return nullptr;
+}
 template <int I, int GG, int VV>
 void* pair_solve_ptr(int solver) {
   if constexpr (I % LB_NPART == LB_PART)
@@ -2428,6 +2765,14 @@ extern "C" void* LB_PART_FN(lb_dev_solve_kernel)(int id, int solver) {
   if (id == i) return pair_solve_ptr<i, g, v>(solver);
   LB_PAIR_CONFIGS(Q)
 #undef Q
+  return nullptr;
+}
+
+extern "C" void* LB_PART_FN(lb_dev_fixed_kernel)(int id, int solver) {
+#define F(i, r, w, v) \
+  if (id == i) return fixed_ptr<i, r, w, v>(solver);
+  LB_FIXED_CONFIGS(F)
+#undef F
   return nullptr;
 }
 

@@ -19,6 +19,10 @@
 // given initial step (no trace, save plan, tape or replay).
 #define LB_PAIR_CONFIGS(Q) Q(15, 8, 2) Q(16, 12, 2)
 
+// Fixed-step kernel variants (fixed_dopri5 / fixed_rk2 / fixed_rk4):
+// F(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
+#define LB_FIXED_CONFIGS(F) F(0, 1, 1024, 2) F(1, 2, 512, 2) F(2, 4, 256, 2) F(3, 8, 256, 1)
+
 // Adjoint kernel variants: Z(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
 #define LB_ADJ_CONFIGS(Z) Z(0, 1, 1024, 2) Z(1, 2, 512, 2) Z(2, 4, 256, 2) Z(3, 8, 256, 1)
 

@@ -73,7 +73,7 @@ def test_execution_modes_and_solvers(prepared, rho0):
     for mode in ("reference", "structured", "structured_upper", "nonsense"):
         with pytest.raises(NotImplementedError):
             solve_lindblad(prepared, rho0, (0.0, 1.0), execution_mode=mode)
-    for solver in ("fixed_rk4", "scipy_bdf", "python_rk45", "rk99"):
+    for solver in ("scipy_bdf", "python_rk45", "rk99"):
         with pytest.raises(NotImplementedError):
             solve_lindblad(prepared, rho0, (0.0, 1.0), solver=solver)
     with pytest.raises(NotImplementedError):
