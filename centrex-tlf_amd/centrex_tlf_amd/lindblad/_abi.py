@@ -15,6 +15,7 @@ __all__ = [
     "LB_OK",
     "LB_MODE_PERSISTENT",
     "LB_MODE_PLAIN",
+    "LB_EVENT",
     "LB_OUT",
     "LB_SOLVER",
     "LB_TRAJ",
@@ -30,7 +31,8 @@ LB_OUT = {"populations": 0, "selected": 1, "integral": 2, "full": 3, "rate": 4}
 LB_MODE_PERSISTENT = 0
 LB_MODE_PLAIN = 1
 LB_TRAJ = {0: "ok", 1: "maxiters", 2: "underflow", 3: "stiff", 4: "bad_span", 5: "bad_support",
-           6: "saveat", 7: "capacity", 8: "bad_dt"}
+           6: "saveat", 7: "capacity", 8: "bad_dt", 9: "event_complex"}
+LB_EVENT = {"none": 0, "population": 1, "runtime_expression": 2}
 
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
@@ -120,6 +122,18 @@ class LbSolveArgs(C.Structure):
         ("replay_h", C.c_void_p),
         ("replay_n", C.c_void_p),
         ("queue_order", C.c_void_p),
+        ("event_kind", C.c_int32),
+        ("n_event_indices", C.c_int32),
+        ("event_indices", _i32p),
+        ("event_threshold", C.c_double),
+        ("n_event_instr", C.c_int32),
+        ("event_op", _i32p),
+        ("event_index", _i32p),
+        ("event_argc", _i32p),
+        ("event_function", _i32p),
+        ("event_re", _f64p),
+        ("event_im", _f64p),
+        ("out_event_t", C.c_void_p),
     ]
 
 

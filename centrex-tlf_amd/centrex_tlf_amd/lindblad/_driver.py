@@ -30,7 +30,12 @@ INTEGRAL_OUTPUTS = {"weighted_integral", "photon_integral", "excited_population"
 RATE_OUTPUTS = {"weighted_rate", "photon_rate", "excited_population_rate"}
 WEIGHTED_OUTPUTS = INTEGRAL_OUTPUTS | RATE_OUTPUTS
 STATUS_MESSAGES = {1: "maxiters", 2: "underflow", 3: "stiff", 4: "bad_span", 5: "bad_support",
-                   6: "saveat", 7: "capacity", 8: "bad_dt"}
+                   6: "saveat", 7: "capacity", 8: "bad_dt", 9: "event_complex"}
+
+
+def time_tol(t: float) -> float:
+    """``rust/src/ode/common.rs:11-13``."""
+    return 10.0 * np.finfo(np.float64).eps * max(abs(float(t)), 1.0)
 
 
 def rust_float(x: float) -> str:
@@ -187,6 +192,8 @@ def trajectory_message(status: int, fail_t: float, *, maxiters: int, n_plan: int
         return "initial state is nonzero outside the declared support"
     if status == 8:
         return "fixed-step solvers require positive finite dt"
+    if status == 9:
+        return "event expression evaluated to a complex value"
     return f"solver failed with status {status}"
 
 
@@ -199,6 +206,7 @@ class DeviceRun:
     fail_t: np.ndarray          # (B,)
     counts: np.ndarray | None   # (B,) rows pushed, trace mode
     times: np.ndarray | None    # (B, capacity), trace mode with times
+    event_t: np.ndarray | None = None  # (B,) stop-event root times, NaN where none fired
 
 
 def run_device(prepared: Any, y0: np.ndarray, t0: float, t1: float, *, solver: str, mode: str,
@@ -206,13 +214,15 @@ def run_device(prepared: Any, y0: np.ndarray, t0: float, t1: float, *, solver: s
                kernel_output: str, selected: Sequence[tuple[int, int]] | None,
                weights: Sequence[tuple[int, float]] | None, slot_indices: Sequence[int],
                parameter_values: np.ndarray | None, grid_axes: Sequence[np.ndarray] | None,
-               save_plan: np.ndarray | None, trace: bool, capacity: int, with_times: bool) -> DeviceRun:
+               save_plan: np.ndarray | None, trace: bool, capacity: int, with_times: bool,
+               stop_event: dict[str, Any] | None = None) -> DeviceRun:
     """One ``lb_solve`` launch; y0 is (dim,) shared or (B, dim)."""
     plan = prepared.device_plan()
     kw: dict[str, Any] = dict(
         solver=solver, abstol=float(abstol), reltol=float(reltol), dt=float(dt), maxiters=int(maxiters),
         save_start=bool(save_start), output_kind=kernel_output, selected=selected, weights=weights,
         mode=mode, save_plan=save_plan, trace=trace, capacity=int(capacity), with_times=with_times,
+        stop_event=stop_event,
     )
     if grid_axes is not None:
         kw.update(override_slots=list(slot_indices), grid_axes=grid_axes)
@@ -221,12 +231,13 @@ def run_device(prepared: Any, y0: np.ndarray, t0: float, t1: float, *, solver: s
     elif np.ndim(y0) == 1:
         kw.update(n_traj=1)
     out = plan.solve(np.ascontiguousarray(y0, dtype=np.float64), t0, t1, **kw)
+    event_t = out[-1].cpu().numpy() if stop_event is not None else None
     if trace:
-        values, stats, fail_t, _, counts, times = out
+        values, stats, fail_t, _, counts, times = out[:6]
         return DeviceRun(values.cpu().numpy(), stats.cpu().numpy(), fail_t.cpu().numpy(),
-                         counts.cpu().numpy(), None if times is None else times.cpu().numpy())
+                         counts.cpu().numpy(), None if times is None else times.cpu().numpy(), event_t)
     values, stats, fail_t = out[:3]
-    return DeviceRun(values.cpu().numpy(), stats.cpu().numpy(), fail_t.cpu().numpy(), None, None)
+    return DeviceRun(values.cpu().numpy(), stats.cpu().numpy(), fail_t.cpu().numpy(), None, None, event_t)
 
 
 def kernel_output_kind(output: str) -> str:
@@ -248,19 +259,28 @@ def width_of(prepared: Any, output: str, output_indices: Any) -> int:
 
 
 def stats_dict(solver: str, stats: np.ndarray, *, saved_points: int, elapsed: float,
-               batch: bool, trajectory_count: int) -> dict[str, Any]:
-    """Solver stats with the reference's keys (``python_api.rs:654-667``, ``batch.py:127-140``)."""
+               batch: bool, trajectory_count: int, event_t: np.ndarray | None = None,
+               end_times: np.ndarray | None = None) -> dict[str, Any]:
+    """Solver stats with the reference's keys (``python_api.rs:516-527, 654-667``,
+    ``batch.py:127-140``): per-trajectory event flags and times (the root, or
+    the last output time ``end_times`` where no event fired)."""
+    fired = np.zeros(trajectory_count, dtype=bool) if event_t is None else np.isfinite(event_t)
     d: dict[str, Any] = {
         "solver": solver,
         "accepted_steps": int(stats[:, 0].sum()),
         "rejected_steps": int(stats[:, 1].sum()),
         "rhs_calls": int(stats[:, 2].sum()),
-        "event_triggered": False,
+        "event_triggered": bool(fired.any()),
     }
     if batch:
-        d["event_triggered_by_trajectory"] = [False] * trajectory_count
-        d["event_times"] = []
-        d["event_count"] = 0
+        d["event_triggered_by_trajectory"] = [bool(f) for f in fired]
+        ends = np.zeros(trajectory_count) if end_times is None else np.broadcast_to(end_times, (trajectory_count,))
+        d["event_times"] = [float(e) if f else float(x) for f, e, x in
+                            zip(fired, event_t if event_t is not None else ends, ends)]
+        d["event_count"] = int(fired.sum())
+    elif fired.any():
+        d["event_time"] = float(event_t[0])
+        d["event_index"] = 0
     d["function_evaluations"] = d["rhs_calls"]
     d["saved_points"] = saved_points
     d["rhs_seconds"] = elapsed

@@ -22,6 +22,7 @@ import numpy as np
 import numpy.typing as npt
 
 from . import _driver as D
+from .events import normalize_stop_event
 from .plan_static import PreparedLindbladProblem
 
 __all__ = [
@@ -97,8 +98,6 @@ def _common_checks(prepared: PreparedLindbladProblem, *, solver: str, execution_
         raise ValueError(f"{name} output_when='saveat' requires explicit saveat values")
     if stop_event is not None and output_when != "final":
         raise ValueError(f"{name} stop_event is only supported with output_when='final'")
-    if stop_event is not None:
-        raise NotImplementedError("stop_event is not implemented by the GPU solver")
     if not dense_output and output_when == "saveat" and saveat_values is not None:
         raise ValueError("dense_output=False is only supported with output_when='final'")
     if threads is not None and threads <= 0:
@@ -110,7 +109,8 @@ def _run(prepared: PreparedLindbladProblem, y0: np.ndarray, t_span: tuple[float,
          solver: str, abstol: float, reltol: float, dt: float, saveat_values: np.ndarray | None,
          save_start: bool, maxiters: int, output: str, output_indices: Any, output_when: str,
          integral_weights: Any, slot_indices: list[int], parameter_values: np.ndarray | None,
-         grid_axes: list[np.ndarray] | None, trajectory_count: int, collect_stats: bool):
+         grid_axes: list[np.ndarray] | None, trajectory_count: int, collect_stats: bool,
+         event_spec: dict[str, Any] | None = None):
     t0, t1 = t_span
     integral = output in D.INTEGRAL_OUTPUTS
     final = output_when == "final"
@@ -126,7 +126,7 @@ def _run(prepared: PreparedLindbladProblem, y0: np.ndarray, t_span: tuple[float,
                        maxiters=maxiters, save_start=eff_save_start, kernel_output=D.kernel_output_kind(output),
                        selected=selected, weights=integral_weights, slot_indices=slot_indices,
                        parameter_values=parameter_values, grid_axes=grid_axes, save_plan=save_plan,
-                       trace=trace, capacity=capacity, with_times=False)
+                       trace=trace, capacity=capacity, with_times=False, stop_event=event_spec)
     elapsed = time.perf_counter() - start
     bad = np.flatnonzero(run.stats[:, 3] != 0)
     if bad.size:
@@ -149,12 +149,19 @@ def _run(prepared: PreparedLindbladProblem, y0: np.ndarray, t_span: tuple[float,
     else:
         values = np.asarray(raw, dtype=np.float64)
     values = values.reshape((trajectory_count, width) if final else (trajectory_count, time_count, width))
+    end_time = float(times[-1]) if len(times) else t1
+    events = None
+    if event_spec is not None:
+        # ode_batch.rs:305-340: t holds each trajectory's root, or its last output time
+        fired = np.isfinite(run.event_t)
+        times = np.where(fired, run.event_t, end_time)
+        events = {"event_triggered": fired, "event_times": np.asarray(times, dtype=np.float64)}
     stats = None
     if collect_stats:
         stats = D.stats_dict(solver, run.stats, saved_points=time_count, elapsed=elapsed, batch=True,
-                             trajectory_count=trajectory_count)
-        stats["event_times"] = [t1] * trajectory_count
-    return np.asarray(times, dtype=np.float64), values, stats
+                             trajectory_count=trajectory_count, event_t=run.event_t,
+                             end_times=np.full(trajectory_count, end_time))
+    return np.asarray(times, dtype=np.float64), values, stats, events
 
 
 def solve_lindblad_batch(
@@ -210,12 +217,13 @@ def solve_lindblad_batch(
             )
     elif parameter_slots is not None:
         raise ValueError("parameter_slots were provided without parameter_batch")
-    times, values, stats = _run(
+    event_spec = normalize_stop_event(stop_event, prepared)
+    times, values, stats, events = _run(
         prepared, packed_batch, t_span_tuple, mode=mode, solver=solver, abstol=abstol, reltol=reltol, dt=dt,
         saveat_values=saveat_values, save_start=save_start, maxiters=maxiters, output=output,
         output_indices=output_indices, output_when=output_when, integral_weights=integral_weights,
         slot_indices=slot_indices, parameter_values=parameter_values, grid_axes=None,
-        trajectory_count=trajectory_count, collect_stats=collect_stats)
+        trajectory_count=trajectory_count, collect_stats=collect_stats, event_spec=event_spec)
     return LindbladBatchResult(
         t=times,
         values=values,
@@ -225,7 +233,7 @@ def solve_lindblad_batch(
         parameter_slots=None if parameter_slots is None else [D.parameter_slot_name(s) for s in parameter_slots],
         parameter_values=parameter_values,
         solver_stats=stats,
-        metadata={} if metadata is None else dict(metadata),
+        metadata={**({} if metadata is None else dict(metadata)), **(events or {})},
     )
 
 
@@ -306,12 +314,13 @@ def grid_scan(
                           stop_event=stop_event, threads=threads, dense_output=dense_output,
                           name="grid_scan")
     trajectory_count = int(np.prod([a.size for a in axes], dtype=np.int64))
-    times, values, stats = _run(
+    event_spec = normalize_stop_event(stop_event, prepared)
+    times, values, stats, events = _run(
         prepared, packed, t_span_tuple, mode=mode, solver=solver, abstol=abstol, reltol=reltol, dt=dt,
         saveat_values=saveat_values, save_start=save_start, maxiters=maxiters, output=output,
         output_indices=output_indices, output_when=output_when, integral_weights=integral_weights,
         slot_indices=slot_indices, parameter_values=None, grid_axes=axes,
-        trajectory_count=trajectory_count, collect_stats=collect_stats)
+        trajectory_count=trajectory_count, collect_stats=collect_stats, event_spec=event_spec)
     result = LindbladBatchResult(
         t=times,
         values=values,
@@ -321,7 +330,7 @@ def grid_scan(
         parameter_slots=parameter_slot_names,
         parameter_values=None,
         solver_stats=stats,
-        metadata={} if metadata is None else dict(metadata),
+        metadata={**({} if metadata is None else dict(metadata)), **(events or {})},
     )
     result.metadata.update({
         "scan_kind": "grid",

@@ -13,7 +13,7 @@ from typing import Any, Sequence
 
 import numpy as np
 
-from ._abi import LB_MODE_PERSISTENT, LB_MODE_PLAIN, LB_OUT, LB_SOLVER, LbSolveArgs, make_plan_desc, ptr
+from ._abi import LB_EVENT, LB_MODE_PERSISTENT, LB_MODE_PLAIN, LB_OUT, LB_SOLVER, LbSolveArgs, make_plan_desc, ptr
 
 __all__ = ["DevicePlan", "NativeError", "Tape", "library", "library_path", "torch_device"]
 
@@ -234,7 +234,8 @@ class DevicePlan:
               stream=None, with_integral: bool = False, variant: int = 0,
               restrict_support: bool = True, support=None, events=None,
               save_plan=None, trace: bool = False, capacity: int = 0, with_times: bool = False,
-              tape=None, adjoint_grad=None, replay=None, queue_order=None, _launch: bool = True):
+              tape=None, adjoint_grad=None, replay=None, queue_order=None, stop_event=None,
+              _launch: bool = True):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
@@ -256,6 +257,10 @@ class DevicePlan:
         ``tape`` (a :class:`Tape`) records the accepted steps for the adjoint;
         with ``adjoint_grad`` (a (B, P) float64 tensor) the call runs the
         adjoint of that recorded tape instead of a solve (``lb_adjoint``).
+
+        ``stop_event`` (a spec from ``events.normalize_stop_event``) ends each
+        trajectory at the first root of its event function; the root times
+        (NaN where it did not fire) are appended to the result tuple.
         """
         import torch
 
@@ -400,9 +405,30 @@ class DevicePlan:
             result = (values, stats, fail_t, integral)
         else:
             result = (values, stats, fail_t)
+        if stop_event is not None:
+            event_t = torch.full((B,), float("nan"), dtype=torch.float64, device=dev)
+            a.out_event_t = _dptr(event_t)
+            if stop_event["kind"] == "population":
+                idx = np.ascontiguousarray(stop_event["indices"], dtype=np.int32)
+                keep.append(idx)
+                a.event_kind = LB_EVENT["population"]
+                a.n_event_indices = idx.size
+                a.event_indices = ptr(idx, C.c_int32)
+                a.event_threshold = float(stop_event["threshold"])
+            else:
+                ins = stop_event["expression"].instructions
+                cols = [np.ascontiguousarray([q[k] for q in ins], dtype=np.int32) for k in range(4)]
+                re = np.ascontiguousarray([q[4] for q in ins], dtype=np.float64)
+                im = np.ascontiguousarray([q[5] for q in ins], dtype=np.float64)
+                keep += cols + [re, im]
+                a.event_kind = LB_EVENT["runtime_expression"]
+                a.n_event_instr = len(ins)
+                a.event_op, a.event_index, a.event_argc, a.event_function = (ptr(c, C.c_int32) for c in cols)
+                a.event_re, a.event_im = ptr(re, C.c_double), ptr(im, C.c_double)
+            result = result + (event_t,)
         if not _launch:
-            if adjoint_grad is not None or events is not None:
-                raise ValueError("grouped solves take no adjoint or events")
+            if adjoint_grad is not None or events is not None or stop_event is not None:
+                raise ValueError("grouped solves take no adjoint, timing events or stop event")
             return _Deferred(a, keep + [y0, t1_arr, ov, plan_t, order_t], result)
         sp = _stream_ptr() if stream is None else C.c_void_p(stream.cuda_stream)
         if events is not None:  # (start, end) torch events recorded right around the launch

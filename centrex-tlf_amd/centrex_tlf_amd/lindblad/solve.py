@@ -19,6 +19,7 @@ import numpy as np
 import numpy.typing as npt
 
 from . import _driver as D
+from .events import normalize_stop_event
 from .plan_static import PreparedLindbladProblem, prepare_lindblad_problem
 
 __all__ = ["LindbladObservableResult", "LindbladResult", "solve_lindblad"]
@@ -89,8 +90,6 @@ def solve_lindblad(
         solver = "dopri5"
     if backend not in {"gpu", "rust"}:
         raise NotImplementedError(f"backend={backend!r} is not available; this solver runs on the GPU")
-    if stop_event is not None:
-        raise NotImplementedError("stop_event is not implemented by the GPU solver")
     launch_mode = D.check_mode_and_solver(execution_mode, solver)
     t_span_tuple = D.normalize_t_span(t_span)
     saveat_values = D.normalize_saveat(saveat, t_span_tuple, save_start)
@@ -110,6 +109,7 @@ def solve_lindblad(
     prepared.check_execution_mode(execution_mode)
     packed_rho0 = prepared.layout.pack(np.asarray(rho0, dtype=np.complex128))
     t0, t1 = t_span_tuple
+    event_spec = normalize_stop_event(stop_event, prepared)
 
     # option translation of python_api.rs:422-528 / solve.py:421-429
     integral = output in D.INTEGRAL_OUTPUTS
@@ -131,7 +131,7 @@ def solve_lindblad(
             dt=dt, maxiters=maxiters, save_start=eff_save_start, kernel_output=kernel_output,
             selected=selected, weights=integral_weights, slot_indices=[], parameter_values=None,
             grid_axes=None, save_plan=save_plan, trace=trace, capacity=capacity,
-            with_times=trace and save_plan is None)
+            with_times=trace and (save_plan is None or event_spec is not None), stop_event=event_spec)
         status = int(run.stats[0, 3])
         if status == 7:  # more accepted steps than rows: the counts are exact now
             capacity = int(run.counts[0])
@@ -150,8 +150,16 @@ def solve_lindblad(
         times = np.asarray([t1], dtype=np.float64)
     if integral and final:
         times = np.asarray([t1 if save_plan is None else (save_plan[-1] if save_plan.size else t1)])
+    fired = event_spec is not None and bool(np.isfinite(run.event_t[0]))
+    if fired and final:
+        # the root is the last pushed point -- unless a state output's save
+        # point t1 lies within time_tol of it (dopri5.rs:318-352)
+        te = float(run.event_t[0])
+        times = np.asarray([t1 if (not integral and t1 <= te + D.time_tol(te)) else te])
     stats = (D.stats_dict(solver, run.stats, saved_points=int(times.size), elapsed=elapsed, batch=False,
-                          trajectory_count=1) if collect_stats else None)
+                          trajectory_count=1, event_t=run.event_t) if collect_stats else None)
+    if stats is not None and fired:
+        stats["event_name"] = event_spec.get("name", "event")
     if output == "full":
         packed = np.asarray(rows, dtype=np.float64).reshape(times.size, width)
         return LindbladResult(t=np.asarray(times, dtype=np.float64), packed_y=packed, layout=prepared.layout,

@@ -172,6 +172,13 @@ struct DevCall {
   // instead of the step-size controller; every step is accepted
   const double* replay_h;
   const int32_t* replay_n;
+  // terminal stop event: population sum (event_idx) or expression (event_ops)
+  int32_t event_kind;       // LB_EVENT_*
+  int32_t n_event;          // indices or instructions
+  const DevOp* event_ops;   // device
+  const int32_t* event_idx; // device, diagonal indices in summation order
+  double event_threshold;
+  double* out_event_t;      // device n_traj: root time, NaN when none fired; or null
 };
 
 struct DevRhsCall {

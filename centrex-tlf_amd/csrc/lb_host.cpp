@@ -28,6 +28,8 @@ extern "C" int lb_dev_variant_pair(int id);
 extern "C" int lb_dev_adjoint_variant(int id, int* R, int* maxw, int* rows);
 extern "C" int lb_dev_fixed_variant(int id, int* R, int* maxw);
 extern "C" void* lb_dev_fixed_kernel(int id, int solver);
+extern "C" int lb_dev_event_variant(int id, int* R, int* maxw);
+extern "C" void* lb_dev_event_kernel(int id, int solver);
 extern "C" void* lb_dev_adjoint_kernel(int id, int solver);
 extern "C" void* lb_dev_solve_kernel(int id, int solver);
 extern "C" void* lb_dev_rhs_kernel(int id);
@@ -54,6 +56,7 @@ int fail(int code, const std::string& msg) {
 // Validated copy of a plan description plus the lowered expression program.
 struct PlanSource {
   int n = 0, U = 0, K = 0, n_base = 0;
+  int n_slots = 0;  // slots of the description (the program adds hidden ones after them)
   int n_diff = 0;  // differentiated parameters (derivative programs follow the K coefficients)
   int64_t T = 0;
   std::vector<int64_t> output_ptrs, input_indices, coeff_indices;
@@ -91,8 +94,14 @@ struct lb_plan {
   int32_t* queue = nullptr;
   std::unique_ptr<DevImage> full;
   std::map<std::vector<uint8_t>, std::unique_ptr<DevImage>> restricted;
-  std::mutex mu;  // guards `restricted` and the occupancy caches
+  std::mutex mu;  // guards `restricted`, the occupancy caches and the event buffers
   int64_t info[16] = {0};
+  // stop-event program of the latest call: device copy, pinned staging, and
+  // an event marking the end of the staging copy
+  void* ev_dev = nullptr;
+  void* ev_pinned = nullptr;
+  size_t ev_cap = 0;
+  hipEvent_t ev_copied = nullptr;
   ~lb_plan() {
     auto release = [](DevImage* im) {
       if (!im) return;
@@ -103,6 +112,10 @@ struct lb_plan {
     release(full.get());
     for (auto& kv : restricted) release(kv.second.get());
     if (queue) (void)hipFree(queue);
+    if (ev_copied) (void)hipEventSynchronize(ev_copied);
+    if (ev_dev) (void)hipFree(ev_dev);
+    if (ev_pinned) (void)hipHostFree(ev_pinned);
+    if (ev_copied) (void)hipEventDestroy(ev_copied);
   }
 };
 
@@ -121,7 +134,8 @@ int upload(DevImage* im, const std::vector<T>& host, const T** out) {
 
 // Simulated stack depth of one expression; rejects what the device machine
 // does not evaluate (temps, tuple helpers, unknown ids).
-int check_expression(const lb_plan_desc* d, int start, int end, int n_slots, std::string& why) {
+int check_expression(const lb_plan_desc* d, int start, int end, int n_slots, std::string& why,
+                     int* depth = nullptr) {
   int sp = 0, peak = 0;
   for (int p = start; p < end; ++p) {
     int op = d->instr_op[p];
@@ -159,6 +173,7 @@ int check_expression(const lb_plan_desc* d, int start, int end, int n_slots, std
   }
   if (sp != 1) { why = "expression does not end with a single value"; return -1; }
   if (peak > LB_MAX_STACK) { why = "expression stack deeper than the device machine"; return -2; }
+  if (depth) *depth = peak;
   return 0;
 }
 
@@ -207,7 +222,8 @@ int64_t lds_bytes_for(const DevPlan& P) {
   const int64_t cells = P.lds_tables ? (int64_t)P.g_rows * 64 : 0;
   return 8 * (int64_t)(4 * P.n_pub) + 8 * 2 * (int64_t)P.S + 8 * (int64_t)((P.n + 1) & ~1) +
          16 * 8 * (int64_t)(P.K + 1) + 16 * 8 * (int64_t)P.n_slots + 8 * 4 * (int64_t)P.nblk +
-         16 * 8 * (int64_t)P.vm_depth + 16 * 128 + 16 * cells + 4 * ((cells + 3) & ~3) + 16;
+         16 * 8 * (int64_t)P.vm_depth + 16 * 128 + 16 * cells + 4 * ((cells + 3) & ~3) + 16 +
+         8 * (int64_t)(P.n + 2);
 }
 
 // Must match carve_pair() in lb_kernels.hip.
@@ -542,6 +558,7 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   src.K = d->n_coefficients;
   src.n_diff = d->n_diff_params;
   src.n_base = d->n_base;
+  src.n_slots = d->n_slots;
   src.T = T;
   src.output_ptrs.assign(d->output_ptrs, d->output_ptrs + U + 1);
   src.input_indices.assign(d->input_indices, d->input_indices + T);
@@ -895,8 +912,84 @@ int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out,
     c.replay_n = a->replay_n;
     c.tape_cap = a->tape_cap;
   }
+  c.out_event_t = a->out_event_t;
   c.queue = p->queue;
   *im_out = im;
+  return LB_OK;
+}
+
+// The stop event of a call: checks it and copies its program (population:
+// the diagonal indices; expression: the instructions as device records) into
+// the plan's event buffer on `stream`, through pinned staging whose previous
+// copy is waited for first.  *depth: the expression's stack depth.
+int prepare_event(lb_plan* p, const lb_solve_args* a, hipStream_t stream, DevCall& c, int* depth) {
+  *depth = 0;
+  if (a->event_kind == LB_EVENT_NONE) return LB_OK;
+  if (a->event_kind != LB_EVENT_POPULATION && a->event_kind != LB_EVENT_EXPRESSION)
+    return fail(LB_ERR_INVALID, "unknown stop event kind " + std::to_string(a->event_kind));
+  if (a->tape_h || a->replay_h)
+    return fail(LB_ERR_UNSUPPORTED, "stop events are not supported with adjoint tapes or replays");
+  std::vector<unsigned char> bytes;
+  int n = 0;
+  if (a->event_kind == LB_EVENT_POPULATION) {
+    n = a->n_event_indices;
+    if (n <= 0 || !a->event_indices) return fail(LB_ERR_INVALID, "population event requires at least one index");
+    for (int k = 0; k < n; ++k)
+      if (a->event_indices[k] < 0 || a->event_indices[k] >= p->src.n)
+        return fail(LB_ERR_INVALID, "population event index " + std::to_string(a->event_indices[k]) +
+                                        " out of bounds");
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(a->event_indices);
+    bytes.assign(b, b + (size_t)n * sizeof(int32_t));
+  } else {
+    n = a->n_event_instr;
+    if (n <= 0 || !a->event_op || !a->event_index || !a->event_argc || !a->event_function || !a->event_re ||
+        !a->event_im)
+      return fail(LB_ERR_INVALID, "the stop event expression is empty");
+    lb_plan_desc d{};
+    d.instr_op = a->event_op;
+    d.instr_index = a->event_index;
+    d.instr_argc = a->event_argc;
+    d.instr_function = a->event_function;
+    d.instr_re = a->event_re;
+    d.instr_im = a->event_im;
+    std::string why;
+    const int rc = check_expression(&d, 0, n, p->src.n_slots, why, depth);
+    if (rc == -1) return fail(LB_ERR_INVALID, "stop event expression: " + why);
+    if (rc == -2) return fail(LB_ERR_UNSUPPORTED, "stop event expression: " + why);
+    std::vector<DevOp> ops(n);
+    for (int i = 0; i < n; ++i) {
+      ops[i].code = (a->event_op[i] & 0xFF) | ((a->event_argc[i] & 0xFF) << 8) | ((a->event_function[i] & 0xFFFF) << 16);
+      ops[i].index = a->event_index[i];
+      ops[i].re = a->event_re[i];
+      ops[i].im = a->event_im[i];
+      ops[i].aux = 0.0;
+    }
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(ops.data());
+    bytes.assign(b, b + ops.size() * sizeof(DevOp));
+  }
+  std::lock_guard<std::mutex> lock(p->mu);
+  if (!p->ev_copied) HIP_TRY(hipEventCreateWithFlags(&p->ev_copied, hipEventDisableTiming));
+  else HIP_TRY(hipEventSynchronize(p->ev_copied));
+  if (bytes.size() > p->ev_cap) {
+    // earlier calls on this stream may still read the old buffer
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (p->ev_dev) HIP_TRY(hipFree(p->ev_dev));
+    if (p->ev_pinned) HIP_TRY(hipHostFree(p->ev_pinned));
+    p->ev_dev = p->ev_pinned = nullptr;
+    p->ev_cap = 0;
+    const size_t cap = std::max<size_t>(bytes.size(), 4096);
+    HIP_TRY(hipMalloc(&p->ev_dev, cap));
+    HIP_TRY(hipHostMalloc(&p->ev_pinned, cap, hipHostMallocDefault));
+    p->ev_cap = cap;
+  }
+  std::memcpy(p->ev_pinned, bytes.data(), bytes.size());
+  HIP_TRY(hipMemcpyAsync(p->ev_dev, p->ev_pinned, bytes.size(), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipEventRecord(p->ev_copied, stream));
+  c.event_kind = a->event_kind;
+  c.n_event = n;
+  c.event_threshold = a->event_threshold;
+  if (a->event_kind == LB_EVENT_POPULATION) c.event_idx = static_cast<const int32_t*>(p->ev_dev);
+  else c.event_ops = static_cast<const DevOp*>(p->ev_dev);
   return LB_OK;
 }
 
@@ -904,24 +997,28 @@ int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out,
 
 namespace {
 
-// Fixed-step solvers: the multi-wave fixed kernel with the fewest slots per
-// thread that covers the plan (or the given threads per trajectory).
-int launch_fixed(const lb_plan* p, const lb_solve_args* a, const DevImage* im, DevCall& c, void* stream) {
+// Fixed-step solvers and calls with a stop event: the multi-wave kernel of
+// that family with the fewest slots per thread that covers the plan (or the
+// given threads per trajectory; `variant` is not used).
+int launch_table(const lb_plan* p, const lb_solve_args* a, const DevImage* im, DevCall& c, void* stream,
+                 bool event, int ev_depth) {
   if (a->tape_h || a->replay_h)
     return fail(LB_ERR_UNSUPPORTED, "adjoint tapes and replays need an adaptive solver (dopri5 or tsit5)");
   const int S = im->dev.S;
   int id = -1, W = 0, R = 0, maxw = 0;
-  for (int v = 0; lb_dev_fixed_variant(v, &R, &maxw); ++v) {
+  for (int v = 0; event ? lb_dev_event_variant(v, &R, &maxw) : lb_dev_fixed_variant(v, &R, &maxw); ++v) {
     const int w = a->threads_per_traj > 0 ? a->threads_per_traj : std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
     if (w <= maxw && (int64_t)R * w >= S) { id = v; W = w; break; }
   }
-  if (id < 0) return fail(LB_ERR_UNSUPPORTED, "no fixed-step kernel variant covers the plan");
-  void* fn = lb_dev_fixed_kernel(id, a->solver);
-  if (!fn) return fail(LB_ERR_UNSUPPORTED, "fixed-step kernel variant is not built");
+  if (id < 0) return fail(LB_ERR_UNSUPPORTED, event ? "no stop-event kernel variant covers the plan"
+                                                    : "no fixed-step kernel variant covers the plan");
+  void* fn = event ? lb_dev_event_kernel(id, a->solver) : lb_dev_fixed_kernel(id, a->solver);
+  if (!fn) return fail(LB_ERR_UNSUPPORTED, "kernel variant is not built");
   void* reset = lb_dev_queue_reset_kernel();
   hipStream_t s = (hipStream_t)stream;
   DevPlan P = im->dev;
-  const size_t lds = (size_t)im->lds_bytes;
+  P.vm_depth = std::max(P.vm_depth, ev_depth);  // the stop-event expression runs on lane 0's stack
+  const size_t lds = (size_t)lds_bytes_for(P);
   int rc = set_lds_limit(fn, lds);
   if (rc) return rc;
   if (a->mode == LB_MODE_PLAIN) {
@@ -961,7 +1058,10 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   DevCall c{};
   int rc = fill_call(p, a, &im, &c);
   if (rc) return rc;
-  if (a->solver >= LB_SOLVER_FIXED_DOPRI5) return launch_fixed(p, a, im, c, stream);
+  int ev_depth = 0;
+  if ((rc = prepare_event(const_cast<lb_plan*>(p), a, (hipStream_t)stream, c, &ev_depth))) return rc;
+  const bool event = a->event_kind != LB_EVENT_NONE;
+  if (event || a->solver >= LB_SOLVER_FIXED_DOPRI5) return launch_table(p, a, im, c, stream, event, ev_depth);
   int id, W;
   const bool pair_ok = a->mode == LB_MODE_PERSISTENT && !a->trace && a->n_saveat == 0 && !a->tape_h &&
                        !a->replay_h && std::isfinite(a->dt) && a->dt > 0.0 && a->output_kind != LB_OUT_RATE;
@@ -1045,6 +1145,7 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   if (a->output_kind != LB_OUT_INTEGRAL || a->n_weights <= 0)
     return fail(LB_ERR_INVALID, "the adjoint differentiates the weighted integral output");
   if (a->n_saveat > 0) return fail(LB_ERR_UNSUPPORTED, "the adjoint supports pushes at every accepted step only");
+  if (a->event_kind != LB_EVENT_NONE) return fail(LB_ERR_UNSUPPORTED, "the adjoint takes no stop event");
   const DevImage* im = nullptr;
   DevCall c{};
   int rc = fill_call(p, a, &im, &c);
@@ -1147,6 +1248,8 @@ extern "C" int lb_solve_group(const lb_plan* const* plans, const lb_solve_args* 
     if (!plans[g]) return fail(LB_ERR_INVALID, "null plan in group " + std::to_string(g));
     if (args[g].mode != LB_MODE_PERSISTENT)
       return fail(LB_ERR_INVALID, "grouped solves use the persistent mode");
+    if (args[g].event_kind != LB_EVENT_NONE)
+      return fail(LB_ERR_UNSUPPORTED, "grouped solves take no stop events (solve each plan with lb_solve)");
     for (int h = 0; h < g; ++h)
       if (plans[h] == plans[g])
         return fail(LB_ERR_INVALID, "a plan may appear in one group only (it holds one work queue)");

@@ -348,6 +348,7 @@ struct Lds {
   double2* gf;    // gather factors (when the plan stages its tables in LDS)
   uint32_t* gw;   // gather words
   int* misc;      // [0] queue value
+  double* evv;    // n + 2: stop-event populations, [n] expression value, [n + 1] complex flag
 };
 
 __device__ __forceinline__ Lds carve(const DevPlan& P) {
@@ -365,6 +366,7 @@ __device__ __forceinline__ Lds carve(const DevPlan& P) {
   const size_t cells = P.lds_tables ? (size_t)P.g_rows * 64 : 0;
   L.gw = reinterpret_cast<uint32_t*>(L.gf + cells);
   L.misc = reinterpret_cast<int*>(L.gw + ((cells + 3) & ~(size_t)3));
+  L.evv = reinterpret_cast<double*>(L.misc + 4);
   return L;
 }
 
@@ -637,6 +639,54 @@ __device__ __forceinline__ void fold_self(const DevPlan& P, int tid, int W, cons
   }
 }
 
+// ---- terminal stop events (rust/src/lindblad/ode_impl.rs:134-153)
+
+// The time-dependent compounds (parameter-graph slots) at time t into slot
+// copy `sl`; coefficient records are skipped.  One lane.
+__device__ __forceinline__ void eval_compounds(const DevPlan& P, dcpx* sl, double t, dcpx* stk) {
+  const __attribute__((address_space(4))) DevExpr* ex = cst(P.ex_dyn);
+  for (int i = 0; i < P.n_dyn_fwd; ++i) {
+    const DevExpr r = fetch_expr(ex, i);
+    if (r.dst >= 0) sl[r.dst] = vm_eval(cst(P.instr), r.start, r.end, sl, t, stk);
+  }
+}
+
+// g(t, y): the listed populations summed in list order minus the threshold,
+// or the real part of the event expression over the slots `sl` (which hold
+// the parameter values at t).  Every lane returns the same value; `complex_`
+// reports an expression with |Im| > 1e-12 (rhs.rs:1230-1236).  Called by
+// every lane of the workgroup (it synchronises).
+template <int R>
+__device__ __forceinline__ double event_value(const DevPlan& P, const DevCall& C, const Lds& L, const Slots<R>& S, int tid, int W,
+                              double t, const dcpx* sl, const double (&vr)[R], bool& complex_) {
+  double g;
+  if (C.event_kind == LB_EVENT_POPULATION) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (S.valid(r) && S.diag(r)) L.evv[S.state(r)] = vr[r];
+    __syncthreads();
+    double v = 0.0;
+    for (int k = 0; k < C.n_event; ++k) v += L.evv[C.event_idx[k]];
+    g = v - C.event_threshold;
+    complex_ = false;
+  } else {
+    if (tid == 0) {
+      const dcpx v = vm_eval(cst(C.event_ops), 0, C.n_event, sl, t, L.vm);
+      L.evv[P.n] = v.re;
+      L.evv[P.n + 1] = fabs(v.im) > 1e-12 ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    g = L.evv[P.n];
+    complex_ = L.evv[P.n + 1] != 0.0;
+  }
+  __syncthreads();  // the next evaluation rewrites evv
+  return g;
+}
+
+__device__ __forceinline__ bool crossed_zero(double g0, double g1) {
+  return g0 == 0.0 || g1 == 0.0 || (g0 < 0.0 && g1 > 0.0) || (g0 > 0.0 && g1 < 0.0);
+}
+
 // Dense output at theta in [0, 1] of the accepted step [x, x + h]
 // (rust/src/ode/dopri5.rs:38-43, 104-143: Hairer's continuous extension;
 // rust/src/ode/tsit5.rs:39-60, 121-142: Tsitouras' interpolant).
@@ -714,9 +764,10 @@ __device__ __forceinline__ void write_row(const DevPlan& P, const DevCall& C, co
 // ---------------------------------------------------------------------------
 // G > 0 selects the single-wave form (R = 1, 64 threads, G gather rows in
 // registers); G = 0 is the general multi-wave form.
-template <int R, int SOLVER, int MAXW, int WAVES, int G>
+template <int R, int SOLVER, int MAXW, int WAVES, int G, bool EV = false>
 __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, const DevCall C) {
   static_assert(G == 0 || (R == 1 && MAXW == 64), "the single-wave form holds one slot per lane");
+  static_assert(!EV || G == 0, "stop events run in the multi-wave form");
   using T = Tab<SOLVER>;
   const int tid = threadIdx.x, W = blockDim.x;
   const Lds L = carve(P);
@@ -751,6 +802,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
       for (int q = 0; q < C.n_weights; ++q)
         if (C.weight_index[q] == i) w += C.weight_value[q];
       L.wts[i] = w;
+      if constexpr (EV) L.evv[i] = 0.0;  // stop-event populations outside a restricted plan stay zero
     }
     __syncthreads();
     for (int i = tid; i < 7 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
@@ -784,6 +836,11 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     int64_t accepted = 0, rejected = 0, rhs_calls = 0;
     int status = 0;
     double fail_x = 0.0, integral = 0.0, last_t = __builtin_nan(""), last_v = 0.0;
+    // stop event: g at the last accepted point, the root's time once it fired,
+    // and the time of the last pushed point (the root is pushed unless it is
+    // that point, dopri5.rs:149-158)
+    double ev_old = 0.0, ev_t = __builtin_nan(""), last_pt = __builtin_nan("");
+    bool ev_fired = false;
     // pushed points (rust/src/ode/output.rs): the trapezoid integral and the
     // trace rows.  Pushing at every accepted step folds the integrand into
     // the error-norm reduction; a save plan (saveat) reduces per point.
@@ -792,6 +849,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
     const bool rate = C.output_kind == LB_OUT_RATE;
     auto push = [&](double tp, const double (&vr)[R], const double (&vi)[R], bool reduce_integrand) {
       double v = last_v;
+      if constexpr (EV) last_pt = tp;
       if ((C.want_integral || rate) && reduce_integrand) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -883,6 +941,13 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         h = fmin(100.0 * fabs(h0), fmin(h1, hmax));
       }
       rhs_calls = 1;
+      if constexpr (EV) {
+        if (C.event_kind) {
+          bool cpx = false;
+          ev_old = event_value<R>(P, C, L, S, tid, W, t0, L.slots, y_r, cpx);
+          if (cpx) { status = LB_TRAJ_EVENT_COMPLEX; fail_x = t0; }
+        }
+      }
       // ---- points at t0 (dopri5.rs:212-220)
       if (!step_push) {
         while (si < C.n_saveat && fabs(C.saveat[si] - t0) <= time_tol(t0)) {
@@ -899,7 +964,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
       bool reject = false, last = false;
       int64_t ns = 0, nst = 0, ia = 0;
 
-      while (!last) {
+      while (!last && (!EV || status == 0)) {
         if (ns > C.maxiters) { status = LB_TRAJ_MAXITERS; fail_x = x; break; }
         if (C.replay_h) {
           // recorded schedule: step sizes of a tape, every step accepted
@@ -1063,17 +1128,66 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
               }
             }
           }
-          if (step_push) {
-            if (C.want_integral) {
-              const double v = block_total(P, L.red, 1);
-              if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (x - last_t);
-              last_t = x;
-              last_v = v;
+          // ---- terminal stop event (dopri5.rs:280-317, tsit5.rs:274-309): g at
+          // the step end (the slots of the last stage are those at x), a root
+          // by 60 bisections on the dense output, its state left in yt
+          bool ev_hit = false;
+          double te = x;
+          if constexpr (EV) {
+            if (C.event_kind) {
+              bool cpx = false;
+              const double g1 = event_value<R>(P, C, L, S, tid, W, x, L.slots + 7 * P.n_slots, yn_r, cpx);
+              if (cpx) { status = LB_TRAJ_EVENT_COMPLEX; fail_x = x; break; }
+              if (crossed_zero(ev_old, g1)) {
+                ev_hit = true;
+                if (ev_old == 0.0 || g1 == 0.0) {
+                  te = ev_old == 0.0 ? xo : x;
+#pragma unroll
+                  for (int r = 0; r < R; ++r) {
+                    yt_r[r] = ev_old == 0.0 ? y_r[r] : yn_r[r];
+                    yt_i[r] = ev_old == 0.0 ? y_i[r] : yn_i[r];
+                  }
+                } else {
+                  double lo = 0.0, hi = 1.0, glo = ev_old;
+                  for (int it = 0; it < 60; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    dense_point<R, SOLVER>(mid, h, y_r, y_i, yn_r, yn_i, k_r, k_i, yt_r, yt_i);
+                    // slot copy 0 is free after the trajectory's start
+                    if (C.event_kind == LB_EVENT_EXPRESSION && tid == 0) eval_compounds(P, L.slots, xo + mid * h, L.vm);
+                    const double gm = event_value<R>(P, C, L, S, tid, W, xo + mid * h, L.slots, yt_r, cpx);
+                    if (cpx) { fail_x = xo + mid * h; break; }
+                    if (gm == 0.0) { lo = mid; hi = mid; break; }
+                    if ((glo < 0.0 && gm > 0.0) || (glo > 0.0 && gm < 0.0)) {
+                      hi = mid;
+                    } else {
+                      lo = mid;
+                      glo = gm;
+                    }
+                  }
+                  if (cpx) { status = LB_TRAJ_EVENT_COMPLEX; break; }
+                  const double th = 0.5 * (lo + hi);
+                  dense_point<R, SOLVER>(th, h, y_r, y_i, yn_r, yn_i, k_r, k_i, yt_r, yt_i);
+                  te = xo + th * h;
+                }
+              }
+              ev_old = g1;
             }
-            if (C.trace) push(x, yn_r, yn_i, rate);
+          }
+          if (step_push) {
+            if (!ev_hit) {
+              if (C.want_integral) {
+                const double v = block_total(P, L.red, 1);
+                if (isfinite(last_t)) integral += 0.5 * (last_v + v) * (x - last_t);
+                last_t = x;
+                last_v = v;
+              }
+              if constexpr (EV) last_pt = x;
+              if (C.trace) push(x, yn_r, yn_i, rate);
+            }
           } else {
-            // save plan points inside (xo, x] (dopri5.rs:318-344)
-            while (si < C.n_saveat && C.saveat[si] <= x + time_tol(x)) {
+            // save plan points inside (xo, x], or up to the root (dopri5.rs:318-344)
+            const double tb = ev_hit ? te : x;
+            while (si < C.n_saveat && C.saveat[si] <= tb + time_tol(tb)) {
               const double ts = C.saveat[si];
               if (ts >= xo - time_tol(xo)) {
                 if (fabs(ts - xo) <= time_tol(xo)) {
@@ -1088,6 +1202,31 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
               }
               ++si;
             }
+          }
+          if (EV && ev_hit) {
+            if (!(isfinite(last_pt) && fabs(last_pt - te) <= time_tol(te))) push(te, yt_r, yt_i, true);
+            // A final state output is the root's state; the reference's save
+            // plan for it is {t1}, so a root within time_tol of t1 leaves the
+            // state pushed at t1 there instead (dopri5.rs:318-352).
+            if (!C.trace && C.output_kind != LB_OUT_INTEGRAL && step_push && t1 <= te + time_tol(te)) {
+              if (fabs(t1 - xo) <= time_tol(xo)) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) { yt_r[r] = y_r[r]; yt_i[r] = y_i[r]; }
+              } else if (fabs(t1 - x) <= time_tol(x)) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) { yt_r[r] = yn_r[r]; yt_i[r] = yn_i[r]; }
+              } else {
+                dense_point<R, SOLVER>((t1 - xo) / h, h, y_r, y_i, yn_r, yn_i, k_r, k_i, yt_r, yt_i);
+              }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              y_r[r] = yt_r[r];
+              y_i[r] = yt_i[r];
+            }
+            ev_fired = true;
+            ev_t = te;
+            break;
           }
 #pragma unroll
           for (int r = 0; r < R; ++r) {
@@ -1110,7 +1249,10 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
 #undef PUB
 
     // ---- outputs
-    if (status == 0 && !step_push && si != C.n_saveat) {
+    if constexpr (EV) {
+      if (C.out_event_t && tid == 0) C.out_event_t[traj] = ev_t;
+    }
+    if (status == 0 && !step_push && si != C.n_saveat && (!EV || !ev_fired)) {
       status = LB_TRAJ_SAVEAT;
       fail_x = (double)si;  // plan points reached
     }
@@ -1269,7 +1411,7 @@ __device__ void write_final(const DevPlan& P, const DevCall& C, const Slots<R>& 
 
 }  // namespace
 
-template <int R, int ST, int MAXW, int WAVES>
+template <int R, int ST, int MAXW, int WAVES, bool EV = false>
 __global__ __launch_bounds__(MAXW, WAVES) void lb_fixed_kernel(const DevPlan P, const DevCall C) {
   using F = FixedStep<ST>;
   constexpr int NS = F::NS;
@@ -1302,6 +1444,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_fixed_kernel(const DevPlan P, 
       for (int q = 0; q < C.n_weights; ++q)
         if (C.weight_index[q] == i) w += C.weight_value[q];
       L.wts[i] = w;
+      if constexpr (EV) L.evv[i] = 0.0;
     }
     __syncthreads();
     for (int i = tid; i < 7 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
@@ -1329,6 +1472,8 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_fixed_kernel(const DevPlan P, 
     int64_t accepted = 0, rhs_calls = 0, n_pushed = 0, si = 0;
     int status = 0;
     double fail_x = 0.0, integral = 0.0, last_t = __builtin_nan(""), last_v = 0.0;
+    double ev_old = 0.0, ev_t = __builtin_nan("");  // stop event: g at the last step end, root time
+    bool ev_fired = false;
     const bool step_push = C.n_saveat == 0;
     const bool rate = C.output_kind == LB_OUT_RATE;
     auto push = [&](double tp, const double (&vr)[R], const double (&vi)[R]) {
@@ -1389,9 +1534,14 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_fixed_kernel(const DevPlan P, 
       } else if (C.save_start) {
         push(t0, y_r, y_i);
       }
+      if constexpr (EV) {
+        bool cpx = false;
+        ev_old = event_value<R>(P, C, L, S, tid, W, t0, L.slots, y_r, cpx);
+        if (cpx) { status = LB_TRAJ_EVENT_COMPLEX; fail_x = t0; }
+      }
       double x = t0;
       int64_t steps = 0;
-      while (x < t1 - time_tol(t1)) {
+      while ((!EV || status == 0) && x < t1 - time_tol(t1)) {
         if (steps >= C.maxiters) { status = LB_TRAJ_MAXITERS; fail_x = x; break; }
         double target = fmin(x + C.dt, t1);
         if (!step_push && si < C.n_saveat && C.saveat[si] > x + time_tol(x)) target = fmin(target, C.saveat[si]);
@@ -1482,14 +1632,41 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_fixed_kernel(const DevPlan P, 
         rhs_calls += NS;
         ++accepted;
         ++steps;
+        const double xo = x;
         x += h;
+        // terminal stop event (fixed.rs:213-238): the root is the step's end
+        // (its start when g was zero there), with the end state
+        bool ev_hit = false;
+        if constexpr (EV) {
+          // slot copy 0 is free after the trajectory's start
+          if (C.event_kind == LB_EVENT_EXPRESSION && tid == 0) eval_compounds(P, L.slots, x, L.vm);
+          bool cpx = false;
+          const double g1 = event_value<R>(P, C, L, S, tid, W, x, L.slots, yn_r, cpx);
+          if (cpx) { status = LB_TRAJ_EVENT_COMPLEX; fail_x = x; break; }
+          if (crossed_zero(ev_old, g1)) {
+            ev_hit = true;
+            ev_t = ev_old == 0.0 ? xo : x;
+          }
+          ev_old = g1;
+        }
         if (step_push) {
-          push(x, yn_r, yn_i);
+          if (!ev_hit) push(x, yn_r, yn_i);
         } else {
-          while (si < C.n_saveat && C.saveat[si] <= x + time_tol(x)) {
+          const double tb = ev_hit ? ev_t : x;
+          while (si < C.n_saveat && C.saveat[si] <= tb + time_tol(tb)) {
             push(C.saveat[si], yn_r, yn_i);
             ++si;
           }
+        }
+        if (EV && ev_hit) {
+          push(ev_t, yn_r, yn_i);
+          ev_fired = true;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            y_r[r] = yn_r[r];
+            y_i[r] = yn_i[r];
+          }
+          break;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -1497,13 +1674,16 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_fixed_kernel(const DevPlan P, 
           y_i[r] = yn_i[r];
         }
       }
-      if (status == 0 && !step_push && si != C.n_saveat) {
+      if (status == 0 && !step_push && si != C.n_saveat && (!EV || !ev_fired)) {
         status = LB_TRAJ_SAVEAT;
         fail_x = (double)si;
       }
     }
 #undef LB_FRHS
     if (status == 0 && C.trace && n_pushed > C.capacity) status = LB_TRAJ_CAPACITY;
+    if constexpr (EV) {
+      if (C.out_event_t && tid == 0) C.out_event_t[traj] = ev_t;
+    }
     write_final<R>(P, C, S, tid, W, traj, y_r, y_i, status, fail_x, integral, accepted, rhs_calls, n_pushed);
   }
 }
@@ -2715,6 +2895,18 @@ void* fixed_ptr(int solver) {
   else
     return nullptr;
 }
+// Stop-event forms: adaptive (solvers 0, 1) and fixed-step (2, 3, 4).
+template <int I, int RR, int WW, int VV>
+void* event_ptr(int solver) {
+  if constexpr (I % LB_NPART == LB_PART)
+    return solver == 0   ? (void*)&lb_solve_kernel<RR, 0, WW, VV, 0, true>
+         : solver == 1 ? (void*)&lb_solve_kernel<RR, 1, WW, VV, 0, true>
+         : solver == 2 ? (void*)&lb_fixed_kernel<RR, 0, WW, VV, true>
+         : solver == 3 ? (void*)&lb_fixed_kernel<RR, 1, WW, VV, true>
+                       : (void*)&lb_fixed_kernel<RR, 2, WW, VV, true>;
+  else
+    return nullptr;
+}
 template <int I, int GG, int VV>
 void* pair_solve_ptr(int solver) {
   if constexpr (I % LB_NPART == LB_PART)
@@ -2773,6 +2965,14 @@ extern "C" void* LB_PART_FN(lb_dev_fixed_kernel)(int id, int solver) {
   if (id == i) return fixed_ptr<i, r, w, v>(solver);
   LB_FIXED_CONFIGS(F)
 #undef F
+  return nullptr;
+}
+
+extern "C" void* LB_PART_FN(lb_dev_event_kernel)(int id, int solver) {
+#define E(i, r, w, v) \
+  if (id == i) return event_ptr<i, r, w, v>(solver);
+  LB_EVENT_CONFIGS(E)
+#undef E
   return nullptr;
 }
 

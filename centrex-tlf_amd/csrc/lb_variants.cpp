@@ -10,6 +10,10 @@ extern "C" void* lb_dev_fixed_kernel_p0(int, int);
 extern "C" void* lb_dev_fixed_kernel_p1(int, int);
 extern "C" void* lb_dev_fixed_kernel_p2(int, int);
 extern "C" void* lb_dev_fixed_kernel_p3(int, int);
+extern "C" void* lb_dev_event_kernel_p0(int, int);
+extern "C" void* lb_dev_event_kernel_p1(int, int);
+extern "C" void* lb_dev_event_kernel_p2(int, int);
+extern "C" void* lb_dev_event_kernel_p3(int, int);
 extern "C" void* lb_dev_adjoint_kernel_p0(int, int);
 extern "C" void* lb_dev_adjoint_kernel_p1(int, int);
 extern "C" void* lb_dev_adjoint_kernel_p2(int, int);
@@ -113,6 +117,20 @@ extern "C" int lb_dev_fixed_variant(int id, int* R, int* maxw) {
   LB_FIXED_CONFIGS(F)
 #undef F
   return 0;
+}
+
+extern "C" int lb_dev_event_variant(int id, int* R, int* maxw) {
+#define E(i, r, w, v)                                                                           \
+  if (id == i) { *R = r; *maxw = w; return 1; }
+  LB_EVENT_CONFIGS(E)
+#undef E
+  return 0;
+}
+
+extern "C" void* lb_dev_event_kernel(int id, int solver) {
+  void* (*parts[])(int, int) = {lb_dev_event_kernel_p0, lb_dev_event_kernel_p1, lb_dev_event_kernel_p2,
+                                lb_dev_event_kernel_p3};
+  return id < 0 ? nullptr : parts[id % LB_NPARTS_BUILT](id, solver);
 }
 
 extern "C" void* lb_dev_fixed_kernel(int id, int solver) {

@@ -22,6 +22,10 @@
 // Fixed-step kernel variants (fixed_dopri5 / fixed_rk2 / fixed_rk4):
 // F(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
 #define LB_FIXED_CONFIGS(F) F(0, 1, 1024, 2) F(1, 2, 512, 2) F(2, 4, 256, 2) F(3, 8, 256, 1)
+// Stop-event forms (adaptive and fixed-step, compiled apart so the event code
+// leaves the other kernels' register allocation alone): one slot per lane up
+// to 1024 slots, six per lane up to 3072.
+#define LB_EVENT_CONFIGS(E) E(0, 1, 1024, 4) E(1, 6, 512, 1)
 
 // Adjoint kernel variants: Z(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
 #define LB_ADJ_CONFIGS(Z) Z(0, 1, 1024, 2) Z(1, 2, 512, 2) Z(2, 4, 256, 2) Z(3, 8, 256, 1)

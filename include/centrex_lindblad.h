@@ -42,6 +42,14 @@ extern "C" {
                                  were dropped).  No reference counterpart. */
 #define LB_TRAJ_BAD_DT 8      /* fixed-step solver without a positive finite dt
                                  ("fixed-step solvers require positive finite dt", fixed.rs:154-156) */
+#define LB_TRAJ_EVENT_COMPLEX 9 /* the stop-event expression evaluated to a complex value
+                                   ("event expression evaluated to a complex value",
+                                   rust/src/lindblad/rhs.rs:1230-1236); fail_t = its time */
+
+/* Terminal stop events (rust/src/lindblad/ode_impl.rs:8-16, 134-153). */
+#define LB_EVENT_NONE 0
+#define LB_EVENT_POPULATION 1 /* g = sum of rho_kk over event_indices - event_threshold */
+#define LB_EVENT_EXPRESSION 2 /* g = Re of the event bytecode over the parameter slots at t */
 
 /* Solvers (rust/src/ode/batch.rs:7-27). */
 #define LB_SOLVER_DOPRI5 0
@@ -248,6 +256,28 @@ typedef struct lb_solve_args {
                                      trajectories in this order (a permutation of
                                      0..n_traj-1; e.g. longest first for uneven batches).
                                      Entries outside [0, n_traj) are skipped. */
+  /* Terminal stop event (SolverOptions stop_event, rust/src/lindblad/ode_impl.rs:8-16).
+   * The event function g(t, y) is checked at the end of every accepted step; a
+   * sign change or zero since the previous accepted step ends the trajectory at
+   * the root: 60 bisections on the dense output for the adaptive solvers
+   * (dopri5.rs:280-317, tsit5.rs:274-309), the step end for the fixed-step ones
+   * (fixed.rs:213-238).  The root is pushed once as the last point; save-plan
+   * points past it are not pushed and are not an error.  Final outputs hold the
+   * state at the root.  Runs on the multi-wave kernels (the single-wave and
+   * paired forms are not used for such a call); no tapes or replays. */
+  int32_t event_kind;             /* LB_EVENT_* */
+  int32_t n_event_indices;        /* population: diagonal indices summed in this order */
+  const int32_t* event_indices;   /* host */
+  double event_threshold;
+  int32_t n_event_instr;          /* expression: instructions in the lb_plan_desc format */
+  const int32_t* event_op;        /* host, n_event_instr each */
+  const int32_t* event_index;
+  const int32_t* event_argc;
+  const int32_t* event_function;
+  const double* event_re;
+  const double* event_im;
+  double* out_event_t;            /* device, n_traj, or NULL: the root's time, NaN when the
+                                     event did not fire */
 } lb_solve_args;
 
 /*

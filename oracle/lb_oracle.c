@@ -580,7 +580,8 @@ static void ws_set_overrides(workspace* w, int n, const int32_t* slots, const cp
   w->cache = CACHE_NONE;
 }
 
-static int eval_graph_and_coefficients(workspace* w, double t) {
+/* evaluate_parameter_graph (rhs.rs:1176-1212): base values, overrides, compounds. */
+static int eval_graph(workspace* w, double t) {
   const lbo_plan* p = w->p;
   for (int i = 0; i < p->n_slots; ++i) w->slots[i] = i < p->n_base ? p->base[i] : C(0.0, 0.0);
   for (int i = 0; i < w->n_override; ++i) {
@@ -594,6 +595,13 @@ static int eval_graph_and_coefficients(workspace* w, double t) {
     if (rc) return rc;
     w->slots[p->compound_slots[c]] = v;
   }
+  return 0;
+}
+
+static int eval_graph_and_coefficients(workspace* w, double t) {
+  const lbo_plan* p = w->p;
+  int rc0 = eval_graph(w, t);
+  if (rc0) return rc0;
   for (int k = 0; k < p->n_coeff; ++k) {
     int rc = eval_expr(p->ins, &p->exprs[p->n_compounds + k], w->slots, t, &w->coeff[k]);
     if (rc) return rc;
@@ -724,6 +732,7 @@ typedef struct {
   double* values;
   /* integral state */
   double integral, last_t, last_value;
+  double last_push_t; /* time of the latest push (any output kind), NaN before the first */
 } collector;
 
 static double weighted(const collector* c, const double* y) {
@@ -741,6 +750,7 @@ static int64_t packed_upper_real_index(int n, int i, int j) {
 static void coll_push(collector* c, double t, const double* y) {
   const lbo_solve_args* a = c->a;
   int n = c->p->n;
+  c->last_push_t = t;
   if (a->output_kind == LBO_OUT_INTEGRAL) {
     double v = weighted(c, y);
     if (isfinite(c->last_t)) c->integral += 0.5 * (c->last_value + v) * (t - c->last_t);
@@ -798,6 +808,50 @@ int64_t lbo_output_width(const lbo_plan* p, const lbo_solve_args* a) {
 static inline double sign_of(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
 static inline double time_tol(double t) { return 10.0 * DBL_EPSILON * fmax(fabs(t), 1.0); }
 static inline int close_to_start(double t, double t0) { return fabs(t - t0) <= time_tol(t0); }
+
+/* ---------------- terminal stop events ------------------------------------ */
+
+#define EVENT_COMPLEX (-9)
+
+/* event_value (rust/src/lindblad/ode_impl.rs:134-153): the listed packed
+   entries (the populations) summed in list order minus the threshold, or the
+   real part of the event expression over the parameter graph at t
+   (rhs.rs:1214-1240, an error beyond |Im| 1e-12). */
+static int event_value(workspace* w, const lbo_solve_args* a, const instr_t* ev_ins, double t, const double* y,
+                       double* g) {
+  if (a->event_kind == LB_EVENT_POPULATION) {
+    double v = 0.0;
+    for (int k = 0; k < a->n_event_indices; ++k) v += y[a->event_indices[k]];
+    *g = v - a->event_threshold;
+    return 0;
+  }
+  int rc = eval_graph(w, t);
+  if (rc) return rc;
+  expr_t e = {0, a->n_event_instr, 1};
+  cpx v;
+  if ((rc = eval_expr(ev_ins, &e, w->slots, t, &v))) return rc;
+  if (fabs(v.im) > 1e-12) {
+    fail("event expression evaluated to a complex value");
+    return EVENT_COMPLEX;
+  }
+  *g = v.re;
+  return 0;
+}
+
+/* crossed_zero (dopri5.rs:145-147, fixed.rs:12-14) */
+static inline int crossed_zero(double g0, double g1) {
+  return g0 == 0.0 || g1 == 0.0 || (g0 < 0.0 && g1 > 0.0) || (g0 > 0.0 && g1 < 0.0);
+}
+
+/* The event program as oracle instructions (NULL for population events). */
+static instr_t* event_program(const lbo_solve_args* a) {
+  if (a->event_kind != LB_EVENT_EXPRESSION) return NULL;
+  instr_t* ins = calloc((size_t)a->n_event_instr + 1, sizeof(instr_t));
+  for (int i = 0; i < a->n_event_instr; ++i)
+    ins[i] = (instr_t){a->event_op[i], a->event_index[i], a->event_argc[i], a->event_function[i], a->event_re[i],
+                       a->event_im[i]};
+  return ins;
+}
 
 typedef struct {
   double alpha, beta, facc1, facc2, fac_old, h_max, safety, posneg;
@@ -909,6 +963,8 @@ typedef struct {
   int64_t accepted, rejected, rhs_calls;
   int status;
   double fail_x;
+  int event_fired;
+  double event_t;
 } ode_stats;
 
 /* Stage input (dopri5.rs:45-86, tsit5.rs:62-102): the first stage is
@@ -963,6 +1019,7 @@ static int solve_fixed(const lbo_plan* p, workspace* w, const lbo_solve_args* a,
     }
   }
   double* buf = calloc((size_t)dim * 9, sizeof(double));
+  instr_t* ev_ins = event_program(a);
   double *y = buf, *yn = buf + dim, *yt = buf + 2 * dim;
   double *k1 = buf + 3 * dim, *k2 = buf + 4 * dim, *k3 = buf + 5 * dim, *k4 = buf + 6 * dim,
          *k5 = buf + 7 * dim, *k6 = buf + 8 * dim;
@@ -974,6 +1031,11 @@ static int solve_fixed(const lbo_plan* p, workspace* w, const lbo_solve_args* a,
     while (si < nsp && close_to_start(sp[si], t0)) coll_push(out, sp[si++], y);
   } else if (a->save_start) {
     coll_push(out, x, y);
+  }
+  double ev_old = 0.0;
+  if (a->event_kind && (rc = event_value(w, a, ev_ins, x, y, &ev_old))) {
+    if (rc == EVENT_COMPLEX) { st->status = LB_TRAJ_EVENT_COMPLEX; st->fail_x = x; }
+    goto done;
   }
   while (x < t1 - time_tol(t1)) {
     if (steps >= a->maxiters) {
@@ -1031,15 +1093,37 @@ static int solve_fixed(const lbo_plan* p, workspace* w, const lbo_solve_args* a,
     st->rhs_calls += calls;
     st->accepted += 1;
     steps += 1;
+    const double xo = x;
     x += h;
+    /* stop event (fixed.rs:213-238): the root is the step end, or its start
+       when g was zero there, with the end state */
+    int ev_hit = 0;
+    if (a->event_kind) {
+      double g1;
+      if ((rc = event_value(w, a, ev_ins, x, yn, &g1))) {
+        if (rc == EVENT_COMPLEX) { st->status = LB_TRAJ_EVENT_COMPLEX; st->fail_x = x; }
+        goto done;
+      }
+      if (crossed_zero(ev_old, g1)) {
+        ev_hit = 1;
+        st->event_fired = 1;
+        st->event_t = ev_old == 0.0 ? xo : x;
+      }
+      ev_old = g1;
+    }
     if (has_sp) {
-      while (si < nsp && sp[si] <= x + time_tol(x)) coll_push(out, sp[si++], yn);
-    } else {
+      const double stop = ev_hit ? st->event_t : x;
+      while (si < nsp && sp[si] <= stop + time_tol(stop)) coll_push(out, sp[si++], yn);
+    } else if (!ev_hit) {
       coll_push(out, x, yn);
+    }
+    if (ev_hit) {
+      coll_push(out, st->event_t, yn);
+      break;
     }
     double* tmp = y; y = yn; yn = tmp;
   }
-  if (has_sp && si != nsp) {
+  if (has_sp && si != nsp && !st->event_fired) {
     st->status = LB_TRAJ_SAVEAT; st->fail_x = (double)si;
     rc = fail("ended before all saveat: %lld of %lld", (long long)si, (long long)nsp);
   }
@@ -1047,6 +1131,7 @@ done:
   if (rc && st->status == LB_TRAJ_OK) st->status = 5;
   free(buf);
   free(sp);
+  free(ev_ins);
   return rc;
 }
 
@@ -1080,7 +1165,9 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
     }
   }
   tableau tb = a->solver == LB_SOLVER_TSIT5 ? tsit5_tab() : dopri5_tab();
-  double* buf = calloc((size_t)dim * 16, sizeof(double));
+  double* buf = calloc((size_t)dim * 17, sizeof(double));
+  instr_t* ev_ins = event_program(a);
+  double* ye_buf = buf + 16 * dim; /* the state at an event's root */
   double* y = buf;
   double* yn = buf + dim;
   double* yt = buf + 2 * dim;
@@ -1123,6 +1210,11 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
   double pn = sign_of(1.0, t1 - t0);
   if ((rc = rhs_eval(w, y, x, k[1]))) goto done;
   st->rhs_calls += 1;
+  double ev_old = 0.0;
+  if (a->event_kind && (rc = event_value(w, a, ev_ins, x, y, &ev_old))) {
+    if (rc == EVENT_COMPLEX) { st->status = LB_TRAJ_EVENT_COMPLEX; st->fail_x = x; }
+    goto done;
+  }
   int64_t si = 0;
   if (has_sp) {
     while (si < nsp && close_to_start(sp[si], t0)) coll_push(out, sp[si++], y);
@@ -1202,9 +1294,57 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
       }
       double xo = x;
       x += h;
+      /* stop event (dopri5.rs:280-317, tsit5.rs:274-309): a root by 60
+         bisections on the dense output of the step */
+      int ev_hit = 0;
+      double te = x;
+      const double* ye = yn;
+      if (a->event_kind) {
+        double g1;
+        if ((rc = event_value(w, a, ev_ins, x, yn, &g1))) {
+          if (rc == EVENT_COMPLEX) { st->status = LB_TRAJ_EVENT_COMPLEX; st->fail_x = x; }
+          goto done;
+        }
+        if (crossed_zero(ev_old, g1)) {
+          ev_hit = 1;
+          if (ev_old == 0.0) {
+            te = xo;
+            ye = y;
+          } else if (g1 == 0.0) {
+            te = x;
+            ye = yn;
+          } else {
+            double lo = 0.0, hi = 1.0, glo = ev_old;
+            for (int it = 0; it < 60; ++it) {
+              const double mid = 0.5 * (lo + hi);
+              if (tb.is_tsit5) dense_tsit5(mid, h, dim, y, k, ds);
+              else dense_dopri5(mid, h, dim, y, yn, k, ds);
+              double gm;
+              if ((rc = event_value(w, a, ev_ins, xo + mid * h, ds, &gm))) {
+                if (rc == EVENT_COMPLEX) { st->status = LB_TRAJ_EVENT_COMPLEX; st->fail_x = xo + mid * h; }
+                goto done;
+              }
+              if (gm == 0.0) { lo = mid; hi = mid; break; }
+              if ((glo < 0.0 && gm > 0.0) || (glo > 0.0 && gm < 0.0)) {
+                hi = mid;
+              } else {
+                lo = mid;
+                glo = gm;
+              }
+            }
+            const double th = 0.5 * (lo + hi);
+            if (tb.is_tsit5) dense_tsit5(th, h, dim, y, k, ye_buf);
+            else dense_dopri5(th, h, dim, y, yn, k, ye_buf);
+            te = xo + th * h;
+            ye = ye_buf;
+          }
+        }
+        ev_old = g1;
+      }
       if (has_sp) {
         int dr = 0;
-        while (si < nsp && sp[si] <= x + time_tol(x)) {
+        const double tb_end = ev_hit ? te : x;
+        while (si < nsp && sp[si] <= tb_end + time_tol(tb_end)) {
           double ts = sp[si];
           if (ts >= xo - time_tol(xo)) {
             if (fabs(ts - xo) <= time_tol(xo)) coll_push(out, ts, y);
@@ -1219,8 +1359,15 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
           si++;
         }
         (void)dr;
-      } else {
+      } else if (!ev_hit) {
         coll_push(out, x, yn);
+      }
+      if (ev_hit) {
+        /* push_unique (dopri5.rs:149-158, 345-351) */
+        if (!(isfinite(out->last_push_t) && fabs(out->last_push_t - te) <= time_tol(te))) coll_push(out, te, ye);
+        st->event_fired = 1;
+        st->event_t = te;
+        break;
       }
       double* tmp = y; y = yn; yn = tmp;
       memcpy(k[1], k[7], sizeof(double) * dim);
@@ -1231,11 +1378,13 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
     }
     h = hn;
   }
-  if (has_sp && si != nsp) rc = fail("ended before all saveat: %lld of %lld", (long long)si, (long long)nsp);
+  if (has_sp && si != nsp && !st->event_fired)
+    rc = fail("ended before all saveat: %lld of %lld", (long long)si, (long long)nsp);
 done:
   if (rc && st->status == LB_TRAJ_OK) st->status = 5;
   free(buf);
   free(sp);
+  free(ev_ins);
   return rc;
 }
 
@@ -1279,6 +1428,7 @@ static void* batch_worker(void* arg) {
     c.times = b->times + i * b->capacity;
     c.values = b->values + i * b->capacity * b->width;
     c.last_t = NAN;
+    c.last_push_t = NAN;
     ode_stats st;
     double t1 = b->t1_per_traj ? b->t1_per_traj[i] : b->a->t1;
     int rc = solve_one(b->p, &w, b->a, y0, b->a->t0, t1, &c, &st);
@@ -1288,6 +1438,7 @@ static void* batch_worker(void* arg) {
     b->stats[4 * i + 2] = st.rhs_calls;
     b->stats[4 * i + 3] = rc ? (st.status ? st.status : 5) : 0;
     b->fail_x[i] = st.fail_x;
+    if (b->a->event_t) b->a->event_t[i] = (!rc && st.event_fired) ? st.event_t : NAN;
     if (rc) {
       pthread_mutex_lock(&b->mu);
       if (b->first_error_traj < 0 || i < b->first_error_traj) {

@@ -42,6 +42,22 @@ typedef struct lbo_solve_args {
   int32_t n_weights;
   const int32_t* weight_indices;
   const double* weight_values;
+  /* terminal stop event (LB_EVENT_*; rust/src/lindblad/ode_impl.rs:134-153):
+   * population indices + threshold, or an expression in the lb_plan_desc
+   * instruction format; event_t (n_traj, or NULL) receives each root's time,
+   * NaN where the event did not fire */
+  int32_t event_kind;
+  int32_t n_event_indices;
+  const int32_t* event_indices;
+  double event_threshold;
+  int32_t n_event_instr;
+  const int32_t* event_op;
+  const int32_t* event_index;
+  const int32_t* event_argc;
+  const int32_t* event_function;
+  const double* event_re;
+  const double* event_im;
+  double* event_t;
 } lbo_solve_args;
 
 int lbo_plan_create(const lb_plan_desc* desc, lbo_plan** out);

@@ -49,6 +49,18 @@ class LboSolveArgs(C.Structure):
         ("n_weights", C.c_int32),
         ("weight_indices", _i32p),
         ("weight_values", _f64p),
+        ("event_kind", C.c_int32),
+        ("n_event_indices", C.c_int32),
+        ("event_indices", _i32p),
+        ("event_threshold", C.c_double),
+        ("n_event_instr", C.c_int32),
+        ("event_op", _i32p),
+        ("event_index", _i32p),
+        ("event_argc", _i32p),
+        ("event_function", _i32p),
+        ("event_re", _f64p),
+        ("event_im", _f64p),
+        ("event_t", _f64p),
     ]
 
 
@@ -147,8 +159,12 @@ class Oracle:
                     parameter_slots: Sequence[Any] | None = None,
                     parameter_batch: np.ndarray | None = None,
                     t1_per_traj: np.ndarray | None = None, threads: int = 0,
+                    stop_event: dict[str, Any] | None = None,
                     raise_on_error: bool = True) -> dict[str, Any]:
-        """Reference-semantics batch solve; option translation of python_api.rs:589-617."""
+        """Reference-semantics batch solve; option translation of python_api.rs:589-617.
+
+        ``stop_event`` is a spec of ``events.normalize_stop_event``; the root
+        times come back as ``event_t`` (NaN where the event did not fire)."""
         y0 = np.asarray(y0, dtype=np.float64)
         shared = y0.ndim == 1
         n_traj = (parameter_batch.shape[0] if parameter_batch is not None else
@@ -187,6 +203,24 @@ class Oracle:
             keep += [wi, wv]
             a.n_weights = wi.size
             a.weight_indices, a.weight_values = ptr(wi, C.c_int32), ptr(wv, C.c_double)
+        event_t = np.full(n_traj, np.nan)
+        if stop_event is not None:
+            a.event_t = ptr(event_t, C.c_double)
+            if stop_event["kind"] == "population":
+                idx = np.ascontiguousarray(stop_event["indices"], dtype=np.int32)
+                keep.append(idx)
+                a.event_kind, a.n_event_indices = 1, idx.size
+                a.event_indices = ptr(idx, C.c_int32)
+                a.event_threshold = float(stop_event["threshold"])
+            else:
+                ins = stop_event["expression"].instructions
+                cols = [np.ascontiguousarray([q[k] for q in ins], dtype=np.int32) for k in range(4)]
+                re = np.ascontiguousarray([q[4] for q in ins], dtype=np.float64)
+                im = np.ascontiguousarray([q[5] for q in ins], dtype=np.float64)
+                keep += cols + [re, im]
+                a.event_kind, a.n_event_instr = 2, len(ins)
+                a.event_op, a.event_index, a.event_argc, a.event_function = (ptr(c, C.c_int32) for c in cols)
+                a.event_re, a.event_im = ptr(re, C.c_double), ptr(im, C.c_double)
         width = lib().lbo_output_width(self._handle, C.byref(a))
         if sv is not None:
             capacity = sv.size + 1
@@ -227,6 +261,7 @@ class Oracle:
             "rhs_calls": stats[:, 2],
             "status": stats[:, 3],
             "fail_x": fail_x,
+            "event_t": event_t,
             "error": err,
             "width": width,
         }

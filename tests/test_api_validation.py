@@ -78,7 +78,7 @@ def test_execution_modes_and_solvers(prepared, rho0):
             solve_lindblad(prepared, rho0, (0.0, 1.0), solver=solver)
     with pytest.raises(NotImplementedError):
         solve_lindblad(prepared, rho0, (0.0, 1.0), backend="python")
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(TypeError, match="stop_event must be a RuntimeExpression"):
         solve_lindblad(prepared, rho0, (0.0, 1.0), stop_event=object())
 
 
@@ -147,3 +147,30 @@ def test_grid_scan_validation(prepared, rho0):
         grid_scan(prepared, rho0, (0.0, 1.0), scan={"delta": [0.0]}, output_when="saveat")
     with pytest.raises(ValueError, match="rho0 must be a packed vector or density matrix"):
         grid_scan(prepared, np.zeros((1, 1, 1)), (0.0, 1.0), scan={"delta": [0.0]})
+
+
+def test_stop_event_normalization(prepared):
+    """``events.py:27-94``: the population and expression specs and their messages."""
+    import sympy as smp
+
+    from centrex_tlf_amd.lindblad import Parameter, RuntimeExpression, normalize_stop_event, population
+
+    with pytest.raises(ValueError, match="population event requires at least one index"):
+        population([])
+    with pytest.raises(ValueError, match="population event index 2 out of bounds for 2 states"):
+        normalize_stop_event(population([0, 2]), prepared)
+    spec = normalize_stop_event(population([1, 0, 1], threshold=0.25, name="bright"), prepared)
+    assert spec == {"kind": "population", "indices": [1, 0, 1], "threshold": 0.25, "name": "bright"}
+    assert normalize_stop_event(None, prepared) is None
+    with pytest.raises(TypeError, match="stop_event must be a RuntimeExpression"):
+        normalize_stop_event(3.0, prepared)
+    t = smp.Symbol("t", real=True)
+    with pytest.raises(ValueError, match=r"stop_event references unknown parameter\(s\): zeta"):
+        normalize_stop_event(t - smp.Symbol("zeta"), prepared)
+    spec = normalize_stop_event(RuntimeExpression(t - 0.4 * smp.Symbol("Omega")), prepared)
+    assert spec["kind"] == "runtime_expression" and spec["expression"].uses_time
+    slot = prepared.slot_names.index("Omega")
+    assert slot in spec["expression"].slot_reads()
+    # a Parameter is its own expression
+    spec = normalize_stop_event(Parameter("delta"), prepared)
+    assert spec["expression"].slot_reads() == {prepared.slot_names.index("delta")}
