@@ -19,8 +19,11 @@ from .parameters import Parameter
 
 # execution modes: the GPU (batched persistent kernel) answers for the
 # reference's expanded-sparse mode; "gpu_plain" launches one trajectory at a time
-GPU_MODES = {"expanded_sparse": "persistent", "gpu": "persistent", "gpu_plain": "plain"}
-REFERENCE_HOST_MODES = {"reference", "structured", "structured_upper"}
+# The reference's modes are host RHS algorithms for one equation (dense,
+# structured BLAS, structured upper triangle, expanded sparse; rhs.rs:1280-1460):
+# on the GPU they all run the expanded sparse program of the plan.
+GPU_MODES = {"expanded_sparse": "persistent", "reference": "persistent", "structured": "persistent",
+             "structured_upper": "persistent", "gpu": "persistent", "gpu_plain": "plain"}
 GPU_SOLVERS = {"dopri5", "tsit5", "fixed_dopri5", "fixed_rk2", "fixed_rk4"}
 ADAPTIVE_SOLVERS = {"dopri5", "tsit5"}
 FIXED_SOLVERS = {"fixed_dopri5", "fixed_rk2", "fixed_rk4"}
@@ -49,15 +52,11 @@ def rust_float(x: float) -> str:
 
 
 def check_mode_and_solver(execution_mode: str, solver: str) -> str:
-    """Launch mode for ``execution_mode``; NotImplementedError for what only the reference host runs."""
-    if execution_mode in REFERENCE_HOST_MODES:
-        raise NotImplementedError(
-            f"execution_mode={execution_mode!r} is a host RHS of the reference; this solver runs "
-            "'expanded_sparse' (alias 'gpu') or 'gpu_plain' on the GPU"
-        )
+    """Launch mode for ``execution_mode`` (``solve.py:548-552``)."""
     if execution_mode not in GPU_MODES:
         raise NotImplementedError(
-            "supported execution_mode values are 'expanded_sparse' (alias 'gpu') and 'gpu_plain'"
+            "supported execution_mode values are 'reference', 'structured', 'structured_upper', "
+            "and 'expanded_sparse' (also 'gpu' and 'gpu_plain')"
         )
     if solver not in GPU_SOLVERS:
         if solver in REFERENCE_SOLVERS:

@@ -276,6 +276,45 @@ def lower_hamiltonian_decomposed(hamiltonian: smp.Matrix, slot_index_by_name: di
     return {"kind": "decomposed", "n": n, "static_matrix": static, "coefficients": coefficients}
 
 
+def lower_hamiltonian_entrywise(hamiltonian: smp.Matrix, slot_index_by_name: dict[str, int],
+                                tuple_value_names: set[str] | None = None) -> dict[str, Any]:
+    """Entrywise Hamiltonian plan (``ir.py:616-674``): every nonzero upper entry
+    H_ij is evaluated from its own simplified expression (numbers go to the
+    static matrix).  The entries become coefficients of the expanded sparse
+    program with one unit basis term each, so the GPU runs them like a
+    decomposed plan; equal entry expressions share one coefficient (the same
+    value either way).  The entries are not split into numeric and symbolic
+    factors as the decomposed plan does, so the values are rounded as the
+    entry expressions themselves are.
+    """
+    n = int(hamiltonian.rows)
+    static = np.zeros((n, n), dtype=np.complex128)
+    entries: dict[str, tuple[smp.Expr, np.ndarray]] = {}
+    for i in range(n):
+        for j in range(i, n):
+            expr = smp.simplify(hamiltonian[i, j])
+            if expr == 0:
+                continue
+            if expr.is_number:
+                static[i, j] += complex(expr.evalf())
+                if i != j:
+                    static[j, i] += np.conjugate(complex(expr.evalf()))
+                continue
+            key = smp.srepr(expr)
+            if key not in entries:
+                entries[key] = (expr, np.zeros((n, n), dtype=np.complex128))
+            entries[key][1][i, j] = 1.0
+    coefficients = []
+    for expr, basis in entries.values():
+        ii, jj = np.nonzero(np.triu(basis))
+        coefficients.append({
+            "expression": compile_expression(expr, slot_index_by_name, tuple_value_names),
+            "symbolic": expr,
+            "basis_terms": [(int(a), int(b), 1.0 + 0.0j) for a, b in zip(ii, jj)],
+        })
+    return {"kind": "entrywise", "n": n, "static_matrix": static, "coefficients": coefficients}
+
+
 def lower_expanded_sparse_rhs(hamiltonian_plan: dict[str, Any], source_decay_rates: np.ndarray,
                               incoming_transfers_by_target: list[list[dict[str, Any]]]) -> dict[str, Any]:
     """CSR program of the upper-triangle Lindblad RHS (``ir.py:447-613``).

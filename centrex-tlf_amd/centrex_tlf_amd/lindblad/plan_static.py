@@ -14,7 +14,8 @@ from typing import Any
 
 import numpy as np
 
-from .ir import flatten_plan, lower_expanded_sparse_rhs, lower_hamiltonian_decomposed, lower_parameter_graph
+from .ir import (flatten_plan, lower_expanded_sparse_rhs, lower_hamiltonian_decomposed, lower_hamiltonian_entrywise,
+                 lower_parameter_graph)
 from .parameters import LindbladParameters, adapt_lindblad_parameters
 from .state_layout import PackedHermitianLayout
 
@@ -43,12 +44,21 @@ class PreparedLindbladProblem:
     gpu_plan: Any | None = field(default=None, repr=False)
 
     def check_execution_mode(self, execution_mode: str) -> None:
-        """Fail fast when the mode needs the expanded sparse plan (``plan_static.py:37-56``)."""
-        if self.expanded_rhs_plan is None:
-            raise ValueError(
-                f"execution_mode={execution_mode!r} requires a decomposed Hamiltonian plan; "
-                "pass hamiltonian_representation='decomposed' to prepare_lindblad_problem"
-            )
+        """The reference's fail-fast check (``plan_static.py:37-56``): its
+        ``expanded_sparse`` modes need a decomposed plan.  (The GPU runs every
+        plan through its expanded sparse program; the check keeps the
+        reference's error for the same request.)"""
+        if "expanded_sparse" not in execution_mode:
+            return
+        kind = self.hamiltonian_plan.get("kind", "entrywise")
+        if kind == "decomposed":
+            return
+        raise ValueError(
+            f"execution_mode={execution_mode!r} requires a decomposed Hamiltonian "
+            f"plan, but this problem was prepared with kind={kind!r}. Either pass "
+            "hamiltonian_representation='decomposed' to prepare_lindblad_problem, "
+            "or use execution_mode='structured'."
+        )
 
     def device_plan(self):
         """The device plan, created on first use."""
@@ -100,22 +110,20 @@ def prepare_lindblad_problem(obe_system: Any, parameters: Any, backend: str = "g
 
     ``backend="gpu"`` (alias ``"rust"``) creates the device plan immediately;
     ``backend="host"`` only lowers, which is enough to inspect the tables.
+    ``hamiltonian_representation``: "decomposed" (the default, also used for
+    "auto") or "entrywise" (one coefficient per distinct upper entry).
     """
     if backend not in _DEVICE_BACKENDS | _HOST_BACKENDS:
         raise ValueError(f"unsupported backend {backend!r}; use 'gpu' (or 'rust') or 'host'")
-    if hamiltonian_representation not in {"decomposed", "auto"}:
-        raise NotImplementedError(
-            "the GPU solver runs the expanded sparse RHS, which needs "
-            "hamiltonian_representation='decomposed'"
-        )
+    if hamiltonian_representation not in {"auto", "entrywise", "decomposed"}:
+        raise ValueError("representation must be one of 'auto', 'entrywise', or 'decomposed'")
     params = adapt_lindblad_parameters(parameters)
     params.check_hamiltonian_symbols(obe_system.H_symbolic)
     n = int(obe_system.H_symbolic.shape[0])
     layout = PackedHermitianLayout(n)
     graph = lower_parameter_graph(params)
-    hamiltonian_plan = lower_hamiltonian_decomposed(
-        obe_system.H_symbolic, params.slot_index_by_name, graph["tuple_value_names"]
-    )
+    lower = lower_hamiltonian_entrywise if hamiltonian_representation == "entrywise" else lower_hamiltonian_decomposed
+    hamiltonian_plan = lower(obe_system.H_symbolic, params.slot_index_by_name, graph["tuple_value_names"])
     dense_c = np.asarray(obe_system.C_array, dtype=np.complex128)
     jumps, source_decay, incoming = lower_structured_jumps(dense_c)
     rhs_plan = lower_expanded_sparse_rhs(hamiltonian_plan, source_decay, incoming)

@@ -70,9 +70,10 @@ def test_save_plan_matches_reference_rules():
 
 
 def test_execution_modes_and_solvers(prepared, rho0):
-    for mode in ("reference", "structured", "structured_upper", "nonsense"):
-        with pytest.raises(NotImplementedError):
-            solve_lindblad(prepared, rho0, (0.0, 1.0), execution_mode=mode)
+    with pytest.raises(NotImplementedError, match="supported execution_mode values are 'reference'"):
+        solve_lindblad(prepared, rho0, (0.0, 1.0), execution_mode="nonsense")
+    for mode in ("reference", "structured", "structured_upper", "expanded_sparse"):
+        assert D.check_mode_and_solver(mode, "dopri5") == "persistent"
     for solver in ("scipy_bdf", "python_rk45", "rk99"):
         with pytest.raises(NotImplementedError):
             solve_lindblad(prepared, rho0, (0.0, 1.0), solver=solver)
@@ -174,3 +175,19 @@ def test_stop_event_normalization(prepared):
     # a Parameter is its own expression
     spec = normalize_stop_event(Parameter("delta"), prepared)
     assert spec["expression"].slot_reads() == {prepared.slot_names.index("delta")}
+
+
+def test_entrywise_plans_and_the_expanded_sparse_check():
+    """``plan_static.py:37-56``: expanded_sparse modes need a decomposed plan;
+    the structured modes take an entrywise one."""
+    system = two_level_system(decay=0.3, omega_name="Omega", delta_name="delta")
+    entry = prepare_lindblad_problem(system, {"Omega": 0.6, "delta": 0.2}, backend="host",
+                                     hamiltonian_representation="entrywise")
+    assert entry.hamiltonian_plan["kind"] == "entrywise"
+    with pytest.raises(ValueError, match="requires a decomposed Hamiltonian plan, but this problem was "
+                                         "prepared with kind='entrywise'"):
+        entry.check_execution_mode("expanded_sparse")
+    entry.check_execution_mode("structured")
+    entry.check_execution_mode("structured_upper")
+    with pytest.raises(ValueError, match="representation must be one of"):
+        prepare_lindblad_problem(system, {"Omega": 0.6}, backend="host", hamiltonian_representation="dense")

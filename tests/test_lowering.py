@@ -91,3 +91,28 @@ def test_term_counts_and_dedup():
         keys = list(zip(rhs["input_indices"][a:b], rhs["coeff_indices"][a:b], rhs["coeff_conj"][a:b],
                         rhs["input_conj"][a:b]))
         assert len(keys) == len(set(keys)) and keys == sorted(keys)
+
+
+@pytest.mark.parametrize("name,system,params,t", list(_systems())[:2], ids=["two_level", "chain3"])
+def test_entrywise_plan_rhs_matches_dense(name, system, params, t):
+    """The entrywise lowering (one coefficient per distinct upper entry) runs
+    the same equation as the dense formula and as the decomposed plan."""
+    prepared = prepare_lindblad_problem(system, params, backend="host", hamiltonian_representation="entrywise")
+    decomposed = prepare_lindblad_problem(system, params, backend="host")
+    rho = random_density(system.n_states, seed=1)
+    got = Oracle(prepared).rhs(prepared.layout.pack(rho), t)
+    want = prepared.layout.pack(dense_rhs(prepared, rho, t))
+    np.testing.assert_allclose(got, want, atol=1e-12, rtol=1e-9)
+    np.testing.assert_allclose(got, Oracle(decomposed).rhs(prepared.layout.pack(rho), t), atol=1e-14, rtol=1e-12)
+
+
+def test_entrywise_plan_rhs_matches_dense_tlf_n39():
+    system, params, rho0, t_span, _ = tlf_n39()
+    prepared = prepare_lindblad_problem(system, params, backend="host", hamiltonian_representation="entrywise")
+    assert prepared.hamiltonian_plan["kind"] == "entrywise"
+    rho = random_density(system.n_states, seed=5) * 1e-2
+    t = 0.41 * t_span[1]
+    got = Oracle(prepared).rhs(prepared.layout.pack(rho), t)
+    want = prepared.layout.pack(dense_rhs(prepared, rho, t))
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(got, want, atol=1e-12 * scale, rtol=0)
