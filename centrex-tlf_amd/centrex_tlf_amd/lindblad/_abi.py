@@ -20,6 +20,8 @@ __all__ = [
     "LB_SOLVER",
     "LB_TRAJ",
     "LbPlanDesc",
+    "LbEffPlanDesc",
+    "LbEffSolveArgs",
     "LbSolveArgs",
     "make_plan_desc",
     "ptr",
@@ -31,7 +33,7 @@ LB_OUT = {"populations": 0, "selected": 1, "integral": 2, "full": 3, "rate": 4}
 LB_MODE_PERSISTENT = 0
 LB_MODE_PLAIN = 1
 LB_TRAJ = {0: "ok", 1: "maxiters", 2: "underflow", 3: "stiff", 4: "bad_span", 5: "bad_support",
-           6: "saveat", 7: "capacity", 8: "bad_dt", 9: "event_complex"}
+           6: "saveat", 7: "capacity", 8: "bad_dt", 9: "event_complex", 10: "field_below", 11: "field_above"}
 LB_EVENT = {"none": 0, "population": 1, "runtime_expression": 2}
 
 _i32p = C.POINTER(C.c_int32)
@@ -135,6 +137,72 @@ class LbSolveArgs(C.Structure):
         ("event_im", _f64p),
         ("out_event_t", C.c_void_p),
     ]
+
+
+class LbEffPlanDesc(C.Structure):
+    _fields_ = [
+        ("n_states", C.c_int32),
+        ("real_dim", C.c_int32),
+        ("n_grid", C.c_int32),
+        ("field_grid", _f64p),
+        ("row_ptrs", C.c_void_p * 3),
+        ("col_indices", C.c_void_p * 3),
+        ("coeffs", C.c_void_p * 3),
+        ("nnz", C.c_int64 * 3),
+        ("field_slot", C.c_int32),
+        ("rabi_slot", C.c_int32),
+        ("detuning_slot", C.c_int32),
+        ("n_slots", C.c_int32),
+        ("n_base", C.c_int32),
+        ("base_re", _f64p),
+        ("base_im", _f64p),
+        ("n_compounds", C.c_int32),
+        ("compound_slots", _i32p),
+        ("expr_ptrs", _i32p),
+        ("instr_op", _i32p),
+        ("instr_index", _i32p),
+        ("instr_argc", _i32p),
+        ("instr_function", _i32p),
+        ("instr_re", _f64p),
+        ("instr_im", _f64p),
+    ]
+
+
+class LbEffSolveArgs(C.Structure):
+    _fields_ = [
+        ("n_traj", C.c_int64),
+        ("solver", C.c_int32),
+        ("t0", C.c_double),
+        ("t1", C.c_double),
+        ("abstol", C.c_double),
+        ("reltol", C.c_double),
+        ("dt", C.c_double),
+        ("maxiters", C.c_int64),
+        ("save_start", C.c_int32),
+        ("y0", C.c_void_p),
+        ("n_override", C.c_int32),
+        ("override_slots", _i32p),
+        ("override_values", C.c_void_p),
+        ("output_kind", C.c_int32),
+        ("n_selected", C.c_int32),
+        ("selected_rows", _i32p),
+        ("selected_cols", _i32p),
+        ("n_weights", C.c_int32),
+        ("weight_indices", _i32p),
+        ("weight_values", _f64p),
+        ("n_saveat", C.c_int32),
+        ("saveat", C.c_void_p),
+        ("capacity", C.c_int64),
+        ("out_values", C.c_void_p),
+        ("out_times", C.c_void_p),
+        ("out_counts", C.c_void_p),
+        ("out_stats", C.c_void_p),
+        ("out_fail_t", C.c_void_p),
+        ("workgroups", C.c_int32),
+    ]
+
+
+LB_EFF_OUT = {"populations": 0, "selected": 1, "integral": 2, "full": 3}
 
 
 def ptr(array: np.ndarray | None, ctype):

@@ -72,6 +72,15 @@ def library() -> C.CDLL:
     L.lb_solve_group.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     L.lb_variant_info.restype = C.c_int
     L.lb_variant_info.argtypes = [C.c_int32] + [C.POINTER(C.c_int32)] * 3
+    L.lb_eff_plan_create.restype = C.c_int
+    L.lb_eff_plan_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    L.lb_eff_plan_destroy.restype = None
+    L.lb_eff_plan_destroy.argtypes = [C.c_void_p]
+    L.lb_eff_solve.restype = C.c_int
+    L.lb_eff_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.lb_eff_rhs_eval.restype = C.c_int
+    L.lb_eff_rhs_eval.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]
     _lib = L
     return L
 
@@ -473,3 +482,100 @@ def solve_group(calls: Sequence[tuple["DevicePlan", dict[str, Any]]], stream=Non
     if events is not None:
         events[1].record(stream)
     return [d.result for d in deferred]
+
+
+class EffectiveDevicePlan:
+    """An effective-Hamiltonian plan (``lb_eff_plan``) on the current HIP device."""
+
+    def __init__(self, desc, keep):
+        L = library()
+        torch_device()
+        handle = C.c_void_p()
+        _check(L.lb_eff_plan_create(C.byref(desc), C.byref(handle)))
+        self._handle = handle
+        self.real_dim = int(desc.real_dim)
+        self.n_states = int(desc.n_states)
+        del keep
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.lb_eff_plan_destroy(h)
+            self._handle = None
+
+    def rhs(self, y, t):
+        """Batched RHS (``lb_eff_rhs_eval``): y (B, real_dim), t (B,); returns (dy, status)."""
+        import torch
+
+        dev = torch_device()
+        y = torch.as_tensor(y, dtype=torch.float64, device=dev).reshape(-1, self.real_dim).contiguous()
+        t = torch.as_tensor(t, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+        out = torch.empty_like(y)
+        status = torch.zeros(y.shape[0], dtype=torch.int32, device=dev)
+        _check(library().lb_eff_rhs_eval(self._handle, y.shape[0], _dptr(y), _dptr(t), _dptr(out), _dptr(status),
+                                         _stream_ptr()))
+        return out, status
+
+    def solve(self, y0, t0: float, t1: float, *, solver: str = "dopri5", abstol: float = 1e-9,
+              reltol: float = 1e-7, dt: float = 1e-10, maxiters: int = 100_000, save_start: bool = True,
+              save_plan=None, output_kind: str = "full", selected=None, weights=None,
+              override_slots: Sequence[int] = (), override_values=None, n_traj: int = 1,
+              capacity: int = 1, workgroups: int = 0):
+        """``lb_eff_solve``: returns (values (B, capacity, width), stats, fail_t, counts, times) on the device."""
+        import torch
+
+        from ._abi import LB_EFF_OUT, LbEffSolveArgs
+
+        dev = torch_device()
+        y0 = torch.as_tensor(y0, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+        if y0.numel() != self.real_dim:
+            raise ValueError(f"expected state length {self.real_dim}, got {y0.numel()}")
+        B = int(n_traj)
+        kind = LB_EFF_OUT[output_kind]
+        width = {0: self.n_states, 1: 2 * len(selected or ()), 2: 1, 3: self.real_dim}[kind]
+        values = torch.full((B, int(capacity), width), float("nan"), dtype=torch.float64, device=dev)
+        times = torch.full((B, int(capacity)), float("nan"), dtype=torch.float64, device=dev)
+        stats = torch.zeros((B, 4), dtype=torch.int64, device=dev)
+        fail_t = torch.zeros(B, dtype=torch.float64, device=dev)
+        counts = torch.zeros(B, dtype=torch.int32, device=dev)
+        keep = []
+        a = LbEffSolveArgs()
+        a.n_traj = B
+        a.solver = LB_SOLVER[solver]
+        a.t0, a.t1, a.abstol, a.reltol, a.dt = float(t0), float(t1), float(abstol), float(reltol), float(dt)
+        a.maxiters = int(maxiters)
+        a.save_start = int(bool(save_start))
+        a.y0 = _dptr(y0)
+        slots = np.ascontiguousarray(override_slots, dtype=np.int32)
+        ov = None
+        if slots.size:
+            ov = torch.as_tensor(np.asarray(override_values, dtype=np.float64).reshape(B, slots.size),
+                                 device=dev).contiguous()
+            keep.append(slots)
+            a.n_override = slots.size
+            a.override_slots = ptr(slots, C.c_int32)
+            a.override_values = _dptr(ov)
+        a.output_kind = kind
+        if selected is not None:
+            rows = np.asarray([i for i, _ in selected], dtype=np.int32)
+            cols = np.asarray([j for _, j in selected], dtype=np.int32)
+            keep += [rows, cols]
+            a.n_selected = rows.size
+            a.selected_rows, a.selected_cols = ptr(rows, C.c_int32), ptr(cols, C.c_int32)
+        if weights is not None:
+            wi = np.asarray([int(i) for i, _ in weights], dtype=np.int32)
+            wv = np.asarray([float(w) for _, w in weights], dtype=np.float64)
+            keep += [wi, wv]
+            a.n_weights = wi.size
+            a.weight_indices, a.weight_values = ptr(wi, C.c_int32), ptr(wv, C.c_double)
+        plan_t = None
+        if save_plan is not None and np.size(save_plan):
+            plan_t = torch.as_tensor(np.asarray(save_plan, dtype=np.float64).reshape(-1), device=dev).contiguous()
+            a.n_saveat = int(plan_t.numel())
+            a.saveat = _dptr(plan_t)
+        a.capacity = int(capacity)
+        a.out_values, a.out_times, a.out_counts = _dptr(values), _dptr(times), _dptr(counts)
+        a.out_stats, a.out_fail_t = _dptr(stats), _dptr(fail_t)
+        a.workgroups = int(workgroups)
+        _check(library().lb_eff_solve(self._handle, C.byref(a), _stream_ptr()))
+        return values, stats, fail_t, counts, times

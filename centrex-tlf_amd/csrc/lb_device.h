@@ -181,6 +181,59 @@ struct DevCall {
   double* out_event_t;      // device n_traj: root time, NaN when none fired; or null
 };
 
+// ---- effective-Hamiltonian plans (split-real state, interpolated operators)
+struct DevEffPlan {
+  int32_t n, D, G;          // states, real dimension, grid points
+  int32_t S;                // D padded to a multiple of 64
+  int32_t nblk;             // S / 64
+  const double* grid;       // G
+  const int32_t* row_ptr[3];
+  const int32_t* col[3];
+  const double* coef[3];    // (G - 1) x nnz x 4
+  int32_t nnz[3];
+  int32_t field_slot, rabi_slot, det_slot;
+  int32_t n_slots, n_base, n_comp, vm_depth;
+  const double* base;       // n_base x 2
+  const DevOp* instr;
+  const DevExpr* comp;      // compound records in evaluation order (dst = slot)
+};
+
+struct DevEffCall {
+  int64_t n_traj;
+  double t0, t1, abstol, reltol, dt;
+  int64_t maxiters;
+  int32_t save_start;
+  const double* y0;
+  int32_t n_override;
+  int32_t override_slot[LB_MAX_OVERRIDES];
+  const double* override_values;  // n_traj x n_override (real)
+  int32_t output_kind;            // LB_EFF_OUT_*
+  int32_t width;
+  int32_t n_selected;
+  int32_t sel_re[LB_MAX_SELECTED];  // index of the real part in y
+  int32_t sel_im[LB_MAX_SELECTED];  // index of the imaginary part, or -1 (diagonal)
+  int32_t n_weights;
+  int32_t weight_index[LB_MAX_WEIGHTS];
+  double weight_value[LB_MAX_WEIGHTS];
+  int32_t n_saveat;
+  const double* saveat;
+  int64_t capacity;
+  double* out_values;
+  double* out_times;
+  int32_t* out_counts;
+  int64_t* out_stats;
+  double* out_fail_t;
+  int32_t* queue;
+};
+
+struct DevEffRhsCall {
+  int64_t batch;
+  const double* y;
+  const double* t;
+  double* out;
+  int32_t* status;
+};
+
 struct DevRhsCall {
   int64_t batch;
   const double* y;

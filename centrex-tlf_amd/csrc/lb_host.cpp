@@ -1313,3 +1313,284 @@ extern "C" int lb_variant_info(int32_t id, int32_t* slots_per_thread, int32_t* m
   if (waves_per_simd) *waves_per_simd = waves;
   return LB_OK;
 }
+
+// ===========================================================================
+// Effective-Hamiltonian plans (rust/src/effective_lindblad/plan.rs:122-190,
+// solver.rs:78-281).
+// ===========================================================================
+
+extern "C" int lb_dev_eff_variant(int id, int* R, int* maxw);
+extern "C" void* lb_dev_eff_kernel(int id, int solver);
+extern "C" void* lb_dev_eff_rhs_kernel(int id);
+
+struct lb_eff_plan {
+  DevEffPlan dev{};
+  std::vector<void*> allocations;
+  int device = 0, cus = 0;
+  int32_t* queue = nullptr;
+  ~lb_eff_plan() {
+    for (void* a : allocations) (void)hipFree(a);
+    if (queue) (void)hipFree(queue);
+  }
+};
+
+namespace {
+
+// Must match carve_eff() in lb_kernels.hip.
+int64_t lds_bytes_eff(const DevEffPlan& P) {
+  return 8 * 2 * (int64_t)P.S + 16 * 8 * (int64_t)P.n_slots + 16 * 8 * (int64_t)P.vm_depth +
+         8 * 4 * (int64_t)P.nblk + 8 * 32 + 8 * 2 + 4 * 4;
+}
+
+template <typename T>
+int eff_upload(lb_eff_plan* p, const std::vector<T>& host, const T** out) {
+  void* d = nullptr;
+  HIP_TRY(hipMalloc(&d, std::max<size_t>(host.size(), 1) * sizeof(T)));
+  p->allocations.push_back(d);
+  if (!host.empty()) HIP_TRY(hipMemcpy(d, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = static_cast<const T*>(d);
+  return LB_OK;
+}
+
+// The multi-wave variant with the fewest rows per lane that covers D rows.
+int eff_config(const DevEffPlan& P, int* id, int* W) {
+  int R = 0, maxw = 0;
+  for (int v = 0; lb_dev_eff_variant(v, &R, &maxw); ++v) {
+    const int w = std::max(64, ((P.D + R - 1) / R + 63) / 64 * 64);
+    if (w <= maxw && (int64_t)R * w >= P.D) {
+      *id = v;
+      *W = w;
+      return LB_OK;
+    }
+  }
+  return fail(LB_ERR_UNSUPPORTED, "effective plan has more rows than the kernel variants cover (4096)");
+}
+
+}  // namespace
+
+extern "C" int lb_eff_plan_create(const lb_eff_plan_desc* d, lb_eff_plan** out_plan) {
+  if (!out_plan) return fail(LB_ERR_INVALID, "out_plan is null");
+  *out_plan = nullptr;
+  if (!d || d->n_states <= 0) return fail(LB_ERR_INVALID, "n_states must be positive");
+  const int n = d->n_states, D = d->real_dim;
+  if ((int64_t)D != 2 * (int64_t)n * n) return fail(LB_ERR_INVALID, "real_dim must be 2 * n_states^2");
+  if (d->n_grid < 2 || !d->field_grid) return fail(LB_ERR_INVALID, "the operator grid needs at least two points");
+  if (d->n_base < 0 || d->n_base > d->n_slots) return fail(LB_ERR_INVALID, "n_base exceeds n_slots");
+  for (int s : {d->field_slot, d->rabi_slot, d->detuning_slot})
+    if (s < 0 || s >= d->n_slots) return fail(LB_ERR_INVALID, "field, Rabi or detuning slot out of range");
+  std::unique_ptr<lb_eff_plan> p(new lb_eff_plan());
+  DevEffPlan& P = p->dev;
+  P.n = n;
+  P.D = D;
+  P.G = d->n_grid;
+  P.S = (D + 63) / 64 * 64;
+  P.nblk = P.S / 64;
+  P.field_slot = d->field_slot;
+  P.rabi_slot = d->rabi_slot;
+  P.det_slot = d->detuning_slot;
+  P.n_slots = d->n_slots;
+  P.n_base = d->n_base;
+  int rc;
+  if ((rc = eff_upload(p.get(), std::vector<double>(d->field_grid, d->field_grid + d->n_grid), &P.grid))) return rc;
+  for (int o = 0; o < 3; ++o) {
+    const int64_t nnz = d->nnz[o];
+    if (nnz < 0 || nnz > INT32_MAX / 4) return fail(LB_ERR_UNSUPPORTED, "operator has too many stored entries");
+    if (!d->row_ptrs[o] || (nnz > 0 && (!d->col_indices[o] || !d->coeffs[o])))
+      return fail(LB_ERR_INVALID, "operator tables are required");
+    std::vector<int32_t> rp(D + 1), col(nnz);
+    if (d->row_ptrs[o][0] != 0 || d->row_ptrs[o][D] != nnz)
+      return fail(LB_ERR_INVALID, "operator row_ptrs must start at 0 and end at nnz");
+    for (int i = 0; i <= D; ++i) {
+      if (i < D && d->row_ptrs[o][i] > d->row_ptrs[o][i + 1])
+        return fail(LB_ERR_INVALID, "operator row_ptrs must be monotonic");
+      rp[i] = (int32_t)d->row_ptrs[o][i];
+    }
+    for (int64_t q = 0; q < nnz; ++q) {
+      if (d->col_indices[o][q] < 0 || d->col_indices[o][q] >= D)
+        return fail(LB_ERR_INVALID, "operator column index out of range");
+      col[q] = (int32_t)d->col_indices[o][q];
+    }
+    std::vector<double> cf(d->coeffs[o], d->coeffs[o] + (size_t)(d->n_grid - 1) * nnz * 4);
+    P.nnz[o] = (int32_t)nnz;
+    if ((rc = eff_upload(p.get(), rp, &P.row_ptr[o]))) return rc;
+    if ((rc = eff_upload(p.get(), col, &P.col[o]))) return rc;
+    if ((rc = eff_upload(p.get(), cf, &P.coef[o]))) return rc;
+  }
+  // parameter graph: base values and compounds (checked as plan expressions)
+  std::vector<double> base(2 * (size_t)std::max(d->n_base, 0));
+  for (int i = 0; i < d->n_base; ++i) {
+    base[2 * i] = d->base_re[i];
+    base[2 * i + 1] = d->base_im[i];
+  }
+  if ((rc = eff_upload(p.get(), base, &P.base))) return rc;
+  lb_plan_desc g{};
+  g.instr_op = d->instr_op;
+  g.instr_index = d->instr_index;
+  g.instr_argc = d->instr_argc;
+  g.instr_function = d->instr_function;
+  g.instr_re = d->instr_re;
+  g.instr_im = d->instr_im;
+  std::vector<DevOp> ops;
+  std::vector<DevExpr> comp;
+  int depth = 1;
+  for (int c = 0; c < d->n_compounds; ++c) {
+    const int a = d->expr_ptrs[c], b = d->expr_ptrs[c + 1];
+    if (d->compound_slots[c] < d->n_base || d->compound_slots[c] >= d->n_slots)
+      return fail(LB_ERR_INVALID, "compound slot out of range");
+    std::string why;
+    int dep = 0;
+    const int r = check_expression(&g, a, b, d->n_slots, why, &dep);
+    if (r == -1) return fail(LB_ERR_INVALID, "compound " + std::to_string(c) + ": " + why);
+    if (r == -2) return fail(LB_ERR_UNSUPPORTED, "compound " + std::to_string(c) + ": " + why);
+    depth = std::max(depth, dep);
+    DevExpr e{};
+    e.start = (int32_t)ops.size();
+    for (int q = a; q < b; ++q) {
+      DevOp op{};
+      op.code = (d->instr_op[q] & 0xFF) | ((d->instr_argc[q] & 0xFF) << 8) | ((d->instr_function[q] & 0xFFFF) << 16);
+      op.index = d->instr_index[q];
+      op.re = d->instr_re[q];
+      op.im = d->instr_im[q];
+      op.aux = 0.0;
+      ops.push_back(op);
+    }
+    e.end = (int32_t)ops.size();
+    e.dst = d->compound_slots[c];
+    comp.push_back(e);
+  }
+  P.n_comp = (int32_t)comp.size();
+  P.vm_depth = depth;
+  if ((rc = eff_upload(p.get(), ops, &P.instr))) return rc;
+  if ((rc = eff_upload(p.get(), comp, &P.comp))) return rc;
+  HIP_TRY(hipGetDevice(&p->device));
+  HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device));
+  HIP_TRY(hipMalloc(&p->queue, sizeof(int32_t)));
+  HIP_TRY(hipMemset(p->queue, 0, sizeof(int32_t)));
+  int id, W;
+  if ((rc = eff_config(P, &id, &W))) return rc;
+  if (lds_bytes_eff(P) > kMaxLdsBytes) return fail(LB_ERR_UNSUPPORTED, "effective plan needs more LDS than a workgroup has");
+  *out_plan = p.release();
+  return LB_OK;
+}
+
+extern "C" void lb_eff_plan_destroy(lb_eff_plan* p) { delete p; }
+
+extern "C" int lb_eff_solve(const lb_eff_plan* p, const lb_eff_solve_args* a, void* stream) {
+  if (!p || !a) return fail(LB_ERR_INVALID, "null argument");
+  if (a->n_traj <= 0) return LB_OK;
+  if (a->solver != LB_SOLVER_DOPRI5 && a->solver != LB_SOLVER_TSIT5)
+    return fail(LB_ERR_UNSUPPORTED, "the effective-Hamiltonian GPU path runs dopri5 and tsit5");
+  if (!a->y0 || !a->out_values || !a->out_stats || !a->out_fail_t || !a->out_counts)
+    return fail(LB_ERR_INVALID, "y0 and output buffers are required");
+  if (a->maxiters <= 0) return fail(LB_ERR_INVALID, "maxiters must be positive");
+  if (!std::isfinite(a->t0) || !std::isfinite(a->t1)) return fail(LB_ERR_INVALID, "t_span must be finite");
+  if (a->capacity <= 0) return fail(LB_ERR_INVALID, "capacity must be positive");
+  const DevEffPlan& P = p->dev;
+  DevEffCall c{};
+  c.n_traj = a->n_traj;
+  c.t0 = a->t0;
+  c.t1 = a->t1;
+  c.abstol = a->abstol;
+  c.reltol = a->reltol;
+  c.dt = a->dt;
+  c.maxiters = a->maxiters;
+  c.save_start = a->save_start;
+  c.y0 = a->y0;
+  if (a->n_override < 0 || a->n_override > LB_MAX_OVERRIDES)
+    return fail(LB_ERR_UNSUPPORTED, "at most 16 overridden parameters");
+  if (a->n_override > 0 && !a->override_values) return fail(LB_ERR_INVALID, "override_values are required");
+  for (int j = 0; j < a->n_override; ++j) {
+    if (a->override_slots[j] < 0 || a->override_slots[j] >= P.n_slots)
+      return fail(LB_ERR_INVALID, "parameter override slot " + std::to_string(a->override_slots[j]) + " out of range");
+    c.override_slot[j] = a->override_slots[j];
+  }
+  c.n_override = a->n_override;
+  c.override_values = a->override_values;
+  c.output_kind = a->output_kind;
+  switch (a->output_kind) {
+    case LB_EFF_OUT_POPULATIONS: c.width = P.n; break;
+    case LB_EFF_OUT_FULL: c.width = P.D; break;
+    case LB_EFF_OUT_INTEGRAL: c.width = 1; break;
+    case LB_EFF_OUT_SELECTED: {
+      if (a->n_selected <= 0 || a->n_selected > LB_MAX_SELECTED)
+        return fail(LB_ERR_INVALID, "output='selected' requires 1..128 output_indices");
+      const int n = P.n, n2 = n * n;
+      for (int q = 0; q < a->n_selected; ++q) {
+        const int i = a->selected_rows[q], j = a->selected_cols[q];
+        if (i < 0 || j < 0 || i >= n || j >= n) return fail(LB_ERR_INVALID, "selected index out of range");
+        // split_real_selected_extractions (solver.rs:60-76)
+        c.sel_re[q] = i * n + j;
+        c.sel_im[q] = i == j ? -1 : n2 + i * n + j;
+      }
+      c.n_selected = a->n_selected;
+      c.width = 2 * a->n_selected;
+      break;
+    }
+    default: return fail(LB_ERR_INVALID, "unknown output kind");
+  }
+  if (a->output_kind == LB_EFF_OUT_INTEGRAL) {
+    if (a->n_weights <= 0 || a->n_weights > LB_MAX_WEIGHTS)
+      return fail(LB_ERR_INVALID, "integral outputs require 1..64 integral weights");
+    for (int q = 0; q < a->n_weights; ++q) {
+      if (a->weight_indices[q] < 0 || a->weight_indices[q] >= P.D)
+        return fail(LB_ERR_INVALID, "integral weight index out of range");
+      c.weight_index[q] = a->weight_indices[q];
+      c.weight_value[q] = a->weight_values[q];
+    }
+    c.n_weights = a->n_weights;
+  }
+  if (a->n_saveat < 0 || (a->n_saveat > 0 && !a->saveat)) return fail(LB_ERR_INVALID, "bad save plan");
+  c.n_saveat = a->n_saveat;
+  c.saveat = a->saveat;
+  c.capacity = a->capacity;
+  c.out_values = a->out_values;
+  c.out_times = a->out_times;
+  c.out_counts = a->out_counts;
+  c.out_stats = a->out_stats;
+  c.out_fail_t = a->out_fail_t;
+  c.queue = p->queue;
+  int id, W, rc;
+  if ((rc = eff_config(P, &id, &W))) return rc;
+  void* fn = lb_dev_eff_kernel(id, a->solver);
+  if (!fn) return fail(LB_ERR_UNSUPPORTED, "effective kernel variant is not built");
+  const size_t lds = (size_t)lds_bytes_eff(P);
+  if ((rc = set_lds_limit(fn, lds))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  int grid = a->workgroups;
+  if (grid <= 0) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, W, lds));
+    grid = std::max(1, per_cu) * std::max(1, p->cus);
+  }
+  grid = (int)std::min<int64_t>(grid, a->n_traj);
+  void* reset = lb_dev_queue_reset_kernel();
+  int32_t* q = p->queue;
+  void* rargs[] = {&q};
+  HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
+  DevEffPlan Pc = P;
+  void* args[] = {&Pc, &c};
+  HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
+  HIP_TRY(hipGetLastError());
+  return LB_OK;
+}
+
+extern "C" int lb_eff_rhs_eval(const lb_eff_plan* p, int64_t batch, const double* y, const double* t,
+                               double* out, int32_t* out_status, void* stream) {
+  if (!p || !y || !t || !out || !out_status) return fail(LB_ERR_INVALID, "null argument");
+  if (batch <= 0) return LB_OK;
+  const DevEffPlan& P = p->dev;
+  int id, W, rc;
+  if ((rc = eff_config(P, &id, &W))) return rc;
+  void* fn = lb_dev_eff_rhs_kernel(id);
+  if (!fn) return fail(LB_ERR_UNSUPPORTED, "effective kernel variant is not built");
+  const size_t lds = (size_t)lds_bytes_eff(P);
+  if ((rc = set_lds_limit(fn, lds))) return rc;
+  DevEffCall c{};
+  DevEffRhsCall Q{batch, y, t, out, out_status};
+  DevEffPlan Pc = P;
+  void* args[] = {&Pc, &c, &Q};
+  const int grid = (int)std::min<int64_t>(batch, 4096);
+  HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, (hipStream_t)stream));
+  HIP_TRY(hipGetLastError());
+  return LB_OK;
+}

@@ -2850,6 +2850,583 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_rhs_kernel(const DevPlan P, co
   }
 }
 
+// ---------------------------------------------------------------------------
+// Effective-Hamiltonian Lindblad path (rust/src/effective_lindblad/).  The
+// state is real, y = [Re rho.flat, Im rho.flat] (D = 2 n^2), and
+//   dy = A_comb(x) y + (Omega / 2) A_opt(x) y + delta A_det(x) y
+// (rhs.rs:86-138): three CSR operators whose stored entries are cubics in
+// dx = x - grid[k] on grid interval k, with x, Omega, delta the parameter
+// slots at t.  One workgroup per trajectory from the work queue, R rows per
+// lane, the stage input published to LDS (double-buffered) for the column
+// gathers.  The parameter graph runs in six lanes per step, one per stage
+// time; lane 0 then finds the grid intervals in the order the reference's
+// RHS calls would, carrying its cached interval hint (rhs.rs:47-84).  The
+// integrator is the expanded-sparse one: Dopri5 / Tsit5 tableaus,
+// controller, stiffness test, save plan and dense output.
+// ---------------------------------------------------------------------------
+#if LB_PART == 1
+namespace {
+
+struct LdsEff {
+  double* ybuf;  // 2 x S stage inputs
+  dcpx* slots;   // 8 x n_slots: t0 / hinit copy and the six stage times
+  dcpx* vm;      // 8 x vm_depth
+  double* red;   // 4 x nblk block sums
+  double* sp;    // 8 x 4 per RHS site: Omega/2, delta, dx, interval; [4 s + 2] field value before lookup
+  double* ev;    // [0] field value of a failed lookup
+  int* misc;     // [0] queue value, [1] status, [2] interval hint
+};
+
+// Must match lds_bytes_eff() in lb_host.cpp.
+__device__ __forceinline__ LdsEff carve_eff(const DevEffPlan& P) {
+  extern __shared__ double smem[];
+  LdsEff L;
+  L.ybuf = smem;
+  L.slots = reinterpret_cast<dcpx*>(L.ybuf + 2 * (size_t)P.S);
+  L.vm = L.slots + 8 * (size_t)P.n_slots;
+  L.red = reinterpret_cast<double*>(L.vm + 8 * (size_t)P.vm_depth);
+  L.sp = L.red + 4 * (size_t)P.nblk;
+  L.ev = L.sp + 32;
+  L.misc = reinterpret_cast<int*>(L.ev + 2);
+  return L;
+}
+
+__device__ __forceinline__ void eff_block_store(const DevEffPlan& P, int tid, int W, int r, double* red, int q,
+                                                double value) {
+  const int s = tid + W * r;
+  const double sum = wave_sum(value);
+  if ((tid & 63) == 0 && s < P.S) red[q * P.nblk + (s >> 6)] = sum;
+}
+
+__device__ __forceinline__ double eff_block_total(const DevEffPlan& P, const double* red, int q) {
+  const int lane = threadIdx.x & 63;
+  double v = 0.0;
+  for (int b = lane; b < P.nblk; b += 64) v += red[q * P.nblk + b];
+  return wave_sum(v);
+}
+
+// Parameter graph at t with the trajectory's (real) overrides, into slot copy
+// `sl` (evaluate_with_overrides_into); returns the field value and stores
+// Omega/2 and delta in sp[0], sp[1].  One lane.
+__device__ __forceinline__ double eff_params(const DevEffPlan& P, const DevEffCall& C, int64_t traj, dcpx* sl,
+                                             double t, dcpx* stk, double* sp) {
+  for (int i = 0; i < P.n_slots; ++i) sl[i] = i < P.n_base ? mk(P.base[2 * i], P.base[2 * i + 1]) : mk(0.0, 0.0);
+  for (int j = 0; j < C.n_override; ++j)
+    sl[C.override_slot[j]] = mk(C.override_values ? C.override_values[traj * C.n_override + j] : 0.0, 0.0);
+  const __attribute__((address_space(4))) DevExpr* ex = cst(P.comp);
+  for (int c = 0; c < P.n_comp; ++c) {
+    const DevExpr r = fetch_expr(ex, c);
+    sl[r.dst] = vm_eval(cst(P.instr), r.start, r.end, sl, t, stk);
+  }
+  sp[0] = 0.5 * sl[P.rabi_slot].re;
+  sp[1] = sl[P.det_slot].re;
+  return sl[P.field_slot].re;
+}
+
+// find_interval_cached (rhs.rs:47-84): the interval of field value x and
+// dx = x - grid[interval], with the trajectory's hint; -1 / -2 below / above
+// the grid by more than its tolerance.  One lane.
+__device__ __forceinline__ int eff_interval(const DevEffPlan& P, double x, int& hint, double& dx) {
+  const int n = P.G;
+  const double g0 = P.grid[0], gn = P.grid[n - 1];
+  const double tol = 1e-10 * fmax(fabs(gn - g0), 1.0);
+  if (x < g0 - tol) return -1;
+  if (x > gn + tol) return -2;
+  const double xc = x < g0 ? g0 : (x > gn ? gn : x);
+  int idx;
+  if (xc <= g0) {
+    hint = 0;
+    idx = 0;
+  } else if (xc >= gn) {
+    hint = n - 2;
+    idx = n - 2;
+  } else if (hint < n - 1 && P.grid[hint] <= xc && xc <= P.grid[hint + 1]) {
+    idx = hint;
+  } else {
+    int lo = 0, hi = n;  // first grid point above xc
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (P.grid[mid] <= xc) lo = mid + 1;
+      else hi = mid;
+    }
+    idx = lo > 0 ? lo - 1 : 0;
+    if (idx > n - 2) idx = n - 2;
+    hint = idx;
+  }
+  dx = x - P.grid[idx];
+  return idx;
+}
+
+// Interval lookup for RHS site s (field value in sp[4 s + 2]); lane 0, in
+// call order.  A failed lookup records the first failure and leaves interval
+// 0 so the RHS still reads inside the tables.
+__device__ __forceinline__ void eff_lookup(const DevEffPlan& P, const LdsEff& L, int s) {
+  const double x = L.sp[4 * s + 2];
+  int hint = L.misc[2];
+  double dx = 0.0;
+  const int idx = eff_interval(P, x, hint, dx);
+  L.misc[2] = hint;
+  if (idx < 0 && L.misc[1] == 0) {
+    L.misc[1] = idx == -1 ? LB_TRAJ_FIELD_BELOW : LB_TRAJ_FIELD_ABOVE;
+    L.ev[0] = x;
+  }
+  L.sp[4 * s + 2] = idx < 0 ? 0.0 : dx;
+  L.sp[4 * s + 3] = (double)(idx < 0 ? 0 : idx);
+}
+
+// dy = sum_o scale_o * sum_p cubic_p(dx) * y[col_p] row by row, each operator
+// summed in CSR order and added in the reference's order (rhs.rs:123-135,
+// plan.rs:18-40).  Publishes y to LDS buffer `yb` first (one barrier).
+template <int R>
+__device__ __forceinline__ void eff_rhs(const DevEffPlan& P, double* yb, const double (&y)[R], double (&k)[R],
+                                        const double* sp, int tid, int W) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = tid + W * r;
+    if (row < P.D) yb[row] = y[r];
+  }
+  __syncthreads();
+  const double half_rabi = sp[0], det = sp[1], dx = sp[2];
+  const int idx = (int)sp[3];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = tid + W * r;
+    double acc = 0.0;
+    if (row < P.D) {
+#pragma unroll
+      for (int o = 0; o < 3; ++o) {
+        const double* cb = P.coef[o] + (size_t)idx * P.nnz[o] * 4;
+        const int* col = P.col[o];
+        double sum = 0.0;
+        const int p1 = P.row_ptr[o][row + 1];
+        for (int p = P.row_ptr[o][row]; p < p1; ++p) {
+          const double2 a = *reinterpret_cast<const double2*>(cb + 4 * (size_t)p);
+          const double2 b = *reinterpret_cast<const double2*>(cb + 4 * (size_t)p + 2);
+          const double val = a.x + dx * (a.y + dx * (b.x + dx * b.y));
+          sum += val * yb[col[p]];
+        }
+        acc += (o == 0 ? 1.0 : o == 1 ? half_rabi : det) * sum;
+      }
+    }
+    k[r] = acc;
+  }
+}
+
+template <int R, int SOLVER, int S>
+__device__ __forceinline__ void eff_stage(double h, const double (&y)[R], const double (&k)[8][R], double (&o)[R]) {
+  using T = Tab<SOLVER>;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double sr = T::a(S, 1) * k[1][r];
+#pragma unroll
+    for (int j = 2; j < S; ++j)
+      if (T::a(S, j) != 0.0) sr = fma(T::a(S, j), k[j][r], sr);
+    o[r] = fma(h, sr, y[r]);
+  }
+}
+
+template <int R, int SOLVER>
+__device__ __forceinline__ void eff_dense(double th, double h, const double (&y)[R], const double (&yn)[R],
+                                          const double (&k)[8][R], double (&o)[R]) {
+  if constexpr (SOLVER == 0) {
+    const double D1 = -12715105075.0 / 11282082432.0, D3 = 87487479700.0 / 32700410799.0,
+                 D4 = -10690763975.0 / 1880347072.0, D5 = 701980252875.0 / 199316789632.0,
+                 D6 = -1453857185.0 / 822651844.0, D7 = 69997945.0 / 29380423.0;
+    const double th1 = 1.0 - th;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double d = yn[r] - y[r];
+      const double b = k[1][r] * h - d;
+      const double r3 = -k[7][r] * h + d - b;
+      const double r4 = h * (D1 * k[1][r] + D3 * k[3][r] + D4 * k[4][r] + D5 * k[5][r] + D6 * k[6][r] + D7 * k[7][r]);
+      o[r] = y[r] + (d + (b + (r3 + r4 * th1) * th) * th1) * th;
+    }
+  } else {
+    const double t2 = th * th, t3 = t2 * th, t4 = t3 * th;
+    const double b1 = 1.0 * th + -2.763706197274826 * t2 + 2.9132554618219126 * t3 + -1.0530884977290216 * t4;
+    const double b2 = 0.1317 * t2 + -0.2234 * t3 + 0.1017 * t4;
+    const double b3 = 3.9302962368947516 * t2 + -5.941033872131505 * t3 + 2.490627285651253 * t4;
+    const double b4 = -12.411077166933676 * t2 + 30.33818863028232 * t3 + -16.548102889244902 * t4;
+    const double b5 = 37.50931341651104 * t2 + -88.1789048947664 * t3 + 47.37952196281928 * t4;
+    const double b6 = -27.896526289197286 * t2 + 65.09189467479366 * t3 + -34.87065786149661 * t4;
+    const double b7 = 1.5 * t2 + -4.0 * t3 + 2.5 * t4;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      o[r] = y[r] + h * (b1 * k[1][r] + b2 * k[2][r] + b3 * k[3][r] + b4 * k[4][r] + b5 * k[5][r] +
+                         b6 * k[6][r] + b7 * k[7][r]);
+  }
+}
+
+}  // namespace
+
+template <int R, int SOLVER, int MAXW, int WAVES>
+__global__ __launch_bounds__(MAXW, WAVES) void lb_eff_kernel(const DevEffPlan P, const DevEffCall C) {
+  using T = Tab<SOLVER>;
+  const int tid = threadIdx.x, W = blockDim.x;
+  const LdsEff L = carve_eff(P);
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) L.misc[0] = atomicAdd(C.queue, 1);
+    __syncthreads();
+    const int64_t traj = L.misc[0];
+    if (traj >= C.n_traj) break;
+    const double t0 = C.t0, t1 = C.t1;
+    if (tid == 0) {
+      L.misc[1] = 0;
+      L.misc[2] = 0;
+    }
+    double y[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = tid + W * r;
+      y[r] = row < P.D ? C.y0[row] : 0.0;
+    }
+    int64_t accepted = 0, rejected = 0, rhs_calls = 0, n_pushed = 0, si = 0;
+    int status = 0, buf = 0;
+    double fail_x = 0.0, integral = 0.0, last_t = __builtin_nan(""), last_v = 0.0;
+    const bool step_push = C.n_saveat == 0;
+    const bool want_int = C.output_kind == LB_EFF_OUT_INTEGRAL;
+    // a pushed point: the weighted sum for the integral (in weight order,
+    // output.rs:315-330), or an output row
+    auto push = [&](double tp, const double (&v)[R]) {
+      if (want_int) {
+        double* yb = L.ybuf + buf * P.S;
+        buf ^= 1;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = tid + W * r;
+          if (row < P.D) yb[row] = v[r];
+        }
+        __syncthreads();
+        if (tid == 0) {
+          double w = 0.0;
+          for (int q = 0; q < C.n_weights; ++q) w += C.weight_value[q] * yb[C.weight_index[q]];
+          if (isfinite(last_t)) integral += 0.5 * (last_v + w) * (tp - last_t);
+          last_v = w;
+        }
+        last_t = tp;
+        ++n_pushed;
+        return;
+      }
+      if (n_pushed < C.capacity) {
+        const int64_t base = (traj * C.capacity + n_pushed) * C.width;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = tid + W * r;
+          if (row >= P.D) continue;
+          if (C.output_kind == LB_EFF_OUT_FULL) {
+            C.out_values[base + row] = v[r];
+          } else if (C.output_kind == LB_EFF_OUT_POPULATIONS) {
+            if (row < P.n * P.n && row % (P.n + 1) == 0) C.out_values[base + row / (P.n + 1)] = v[r];
+          } else {
+            for (int q = 0; q < C.n_selected; ++q) {
+              if (C.sel_re[q] == row) {
+                C.out_values[base + 2 * q] = v[r];
+                if (C.sel_im[q] < 0) C.out_values[base + 2 * q + 1] = 0.0;
+              }
+              if (C.sel_im[q] == row) C.out_values[base + 2 * q + 1] = v[r];
+            }
+          }
+        }
+        if (tid == 0 && C.out_times) C.out_times[traj * C.capacity + n_pushed] = tp;
+      }
+      last_t = tp;
+      ++n_pushed;
+    };
+    // field, Omega/2, delta at t for RHS site s (lane 0), then the lookup
+    auto site = [&](int s, double t) {
+      if (tid == 0) {
+        L.sp[4 * s + 2] = eff_params(P, C, traj, L.slots + s * P.n_slots, t, L.vm, L.sp + 4 * s);
+        eff_lookup(P, L, s);
+      }
+      __syncthreads();
+      if (L.misc[1] != 0) {
+        status = L.misc[1];
+        fail_x = L.ev[0];
+      }
+    };
+#define LB_EFF_RHS(yy, kk, s)                              \
+  do {                                                     \
+    eff_rhs<R>(P, L.ybuf + buf * P.S, yy, kk, L.sp + 4 * (s), tid, W); \
+    buf ^= 1;                                              \
+  } while (0)
+
+    double k[8][R];
+    if (t1 < t0) {
+      status = LB_TRAJ_BAD_SPAN;
+      fail_x = t0;
+    } else if (t1 == t0) {
+      if (C.save_start) push(t0, y);
+    } else {
+      double x = t0, h = 0.0;
+      if (isfinite(C.dt) && C.dt > 0.0) {
+        h = fmin(C.dt, t1 - t0);
+      } else {
+        // hinit (rust/src/ode/common.rs:72-118): f0 at t0, f1 at t0 + h0
+        double f0[R], yt[R], f1[R];
+        site(0, t0);
+        if (status == 0) {
+          LB_EFF_RHS(y, f0, 0);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int row = tid + W * r;
+            const double sc = C.abstol + fabs(y[r]) * C.reltol;
+            const double a0 = y[r] / sc, b0 = f0[r] / sc;
+            eff_block_store(P, tid, W, r, L.red, 0, row < P.D ? a0 * a0 : 0.0);
+            eff_block_store(P, tid, W, r, L.red, 1, row < P.D ? b0 * b0 : 0.0);
+          }
+          __syncthreads();
+          const double d0 = eff_block_total(P, L.red, 0), d1 = eff_block_total(P, L.red, 1), hmax = fabs(t1 - x);
+          double h0 = (d0 < 1.0e-10 || d1 < 1.0e-10) ? 1.0e-6 : 0.01 * sqrt(d0 / d1);
+          h0 = fmin(h0, hmax);
+#pragma unroll
+          for (int r = 0; r < R; ++r) yt[r] = y[r] + f0[r] * h0;
+          site(1, x + h0);
+          if (status == 0) {
+            LB_EFF_RHS(yt, f1, 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const int row = tid + W * r;
+              const double sc = C.abstol + fabs(y[r]) * C.reltol;
+              const double a0 = (f1[r] - f0[r]) / sc;
+              eff_block_store(P, tid, W, r, L.red, 2, row < P.D ? a0 * a0 : 0.0);
+            }
+            __syncthreads();
+            const double d2 = sqrt(eff_block_total(P, L.red, 2)) / h0;
+            const double h1 = fmax(sqrt(d1), fabs(d2)) <= 1.0e-15 ? fmax(1.0e-6, fabs(h0) * 1.0e-3)
+                                                                  : pow(0.01 / fmax(sqrt(d1), d2), 1.0 / 5.0);
+            h = fmin(100.0 * fabs(h0), fmin(h1, hmax));
+          }
+        }
+      }
+      // k1 = f(t0, y0)
+      if (status == 0) site(0, t0);
+      if (status == 0) {
+        LB_EFF_RHS(y, k[1], 0);
+        rhs_calls = 1;
+        if (!step_push) {
+          while (si < C.n_saveat && fabs(C.saveat[si] - t0) <= time_tol(t0)) {
+            push(C.saveat[si], y);
+            ++si;
+          }
+        } else if (C.save_start) {
+          push(t0, y);
+        }
+      }
+      const double beta = 0.04, alpha = 0.2 - beta * 0.75, facc1 = 1.0 / 0.2, facc2 = 1.0 / 10.0;
+      const double safety = 0.9, hmax = fabs(t1 - t0);
+      double fac_old = 1.0e-4;
+      bool reject = false, last = false;
+      int64_t ns = 0, nst = 0, ia = 0;
+      while (!last && status == 0) {
+        if (ns > C.maxiters) { status = LB_TRAJ_MAXITERS; fail_x = x; break; }
+        if (0.1 * fabs(h) <= DBL_EPSILON * fabs(x)) { status = LB_TRAJ_UNDERFLOW; fail_x = x; break; }
+        if (x + 1.01 * h - t1 > 0.0) { h = t1 - x; last = true; }
+        ++ns;
+        // parameters at the six stage times (lanes 0-5), then the interval
+        // lookups in call order (lane 0)
+        if (tid < 6) {
+          const int s = tid + 2;
+          L.sp[4 * s + 2] = eff_params(P, C, traj, L.slots + s * P.n_slots, x + h * T::c(s), L.vm + tid * P.vm_depth,
+                                       L.sp + 4 * s);
+        }
+        __syncthreads();
+        if (tid == 0)
+          for (int s = 2; s <= 7; ++s) eff_lookup(P, L, s);
+        __syncthreads();
+        if (L.misc[1] != 0) {
+          status = L.misc[1];
+          fail_x = L.ev[0];
+          break;
+        }
+        const bool need_stiff = ((accepted + 1) % 1000 == 0) || ia > 0;
+        double yt[R], yn[R];
+#pragma unroll 1
+        for (int s = 2; s <= 7; ++s) {
+          switch (s) {
+            case 2: eff_stage<R, SOLVER, 2>(h, y, k, yt); break;
+            case 3: eff_stage<R, SOLVER, 3>(h, y, k, yt); break;
+            case 4: eff_stage<R, SOLVER, 4>(h, y, k, yt); break;
+            case 5: eff_stage<R, SOLVER, 5>(h, y, k, yt); break;
+            case 6: eff_stage<R, SOLVER, 6>(h, y, k, yt); break;
+            default:
+              eff_stage<R, SOLVER, 7>(h, y, k, yn);
+              if (need_stiff) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                  const double a0 = yn[r] - yt[r];
+                  eff_block_store(P, tid, W, r, L.red, 3, a0 * a0);
+                }
+              }
+#pragma unroll
+              for (int r = 0; r < R; ++r) yt[r] = yn[r];
+              break;
+          }
+          double ko[R];
+          LB_EFF_RHS(yt, ko, s);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            switch (s) {
+              case 2: k[2][r] = ko[r]; break;
+              case 3: k[3][r] = ko[r]; break;
+              case 4: k[4][r] = ko[r]; break;
+              case 5: k[5][r] = ko[r]; break;
+              case 6: k[6][r] = ko[r]; break;
+              default: k[7][r] = ko[r]; break;
+            }
+          }
+        }
+        rhs_calls += 6;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = tid + W * r;
+          double er = T::e(1) * k[1][r];
+#pragma unroll
+          for (int j = 2; j <= 7; ++j)
+            if (T::e(j) != 0.0) er = fma(T::e(j), k[j][r], er);
+          er *= h;
+          const double sc = C.abstol + fmax(fabs(y[r]), fabs(yn[r])) * C.reltol;
+          const double q = er / sc;
+          eff_block_store(P, tid, W, r, L.red, 0, row < P.D ? q * q : 0.0);
+          if (need_stiff) {
+            const double a0 = k[7][r] - k[6][r];
+            eff_block_store(P, tid, W, r, L.red, 2, a0 * a0);
+          }
+        }
+        __syncthreads();
+        const double err = sqrt(eff_block_total(P, L.red, 0) / (double)P.D);
+        const int wl = tid & 63;
+        const double pw = pow(wl < 32 ? err : fac_old, wl < 32 ? alpha : -beta);
+        const double fac11 = __shfl(pw, 0, 64), fo = __shfl(pw, 32, 64);
+        double fac = fac11 * fo;
+        fac = fmax(facc2, fmin(facc1, fac / safety));
+        double hn = h / fac;
+        if (err <= 1.0) {
+          fac_old = fmax(err, 1.0e-4);
+          if (fabs(hn) > hmax) hn = hmax;
+          if (reject) hn = fmin(fabs(hn), fabs(h));
+          reject = false;
+          ++accepted;
+          if (accepted % 1000 == 0 || ia > 0) {
+            const double n2 = eff_block_total(P, L.red, 2), dd2 = eff_block_total(P, L.red, 3);
+            const double hl = dd2 > 0.0 ? h * sqrt(n2 / dd2) : 0.0;
+            if (hl > 3.25) {
+              ++ia;
+              nst = 0;
+              if (ia == 15) { status = LB_TRAJ_STIFF; fail_x = x; break; }
+            } else {
+              ++nst;
+              if (nst == 6) ia = 0;
+            }
+          }
+          const double xo = x;
+          x += h;
+          if (step_push) {
+            push(x, yn);
+          } else {
+            while (si < C.n_saveat && C.saveat[si] <= x + time_tol(x)) {
+              const double ts = C.saveat[si];
+              if (ts >= xo - time_tol(xo)) {
+                if (fabs(ts - xo) <= time_tol(xo)) {
+                  push(ts, y);
+                } else if (fabs(ts - x) <= time_tol(x)) {
+                  push(ts, yn);
+                } else {
+                  double ds[R];
+                  eff_dense<R, SOLVER>((ts - xo) / h, h, y, yn, k, ds);
+                  push(ts, ds);
+                }
+              }
+              ++si;
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            y[r] = yn[r];
+            k[1][r] = k[7][r];
+          }
+          if (last) break;
+        } else {
+          hn = h / fmin(facc1, fac11 / safety);
+          reject = true;
+          last = false;
+          if (accepted >= 1) ++rejected;
+        }
+        h = hn;
+      }
+    }
+#undef LB_EFF_RHS
+    if (status == 0 && !step_push && si != C.n_saveat) {
+      status = LB_TRAJ_SAVEAT;
+      fail_x = (double)si;
+    }
+    if (status == 0 && !want_int && n_pushed > C.capacity) status = LB_TRAJ_CAPACITY;
+    if (tid == 0) {
+      if (want_int) {
+        C.out_values[traj * C.capacity] = status == 0 ? integral : __builtin_nan("");
+        if (C.out_times) C.out_times[traj * C.capacity] = last_t;
+        C.out_counts[traj] = n_pushed > 0 ? 1 : 0;
+      } else {
+        C.out_counts[traj] = (int32_t)n_pushed;
+      }
+      C.out_stats[4 * traj + 0] = accepted;
+      C.out_stats[4 * traj + 1] = rejected;
+      C.out_stats[4 * traj + 2] = rhs_calls;
+      C.out_stats[4 * traj + 3] = status;
+      C.out_fail_t[traj] = fail_x;
+    }
+  }
+}
+
+// One effective RHS per workgroup (debug_effective_lindblad_rhs_py): the
+// interval hint starts at 0 for every evaluation.
+template <int R, int MAXW>
+__global__ __launch_bounds__(MAXW, 1) void lb_eff_rhs_kernel(const DevEffPlan P, const DevEffCall C,
+                                                             const DevEffRhsCall Q) {
+  const int tid = threadIdx.x, W = blockDim.x;
+  const LdsEff L = carve_eff(P);
+  for (int64_t b = blockIdx.x; b < Q.batch; b += gridDim.x) {
+    __syncthreads();
+    if (tid == 0) {
+      L.misc[1] = 0;
+      L.misc[2] = 0;
+      L.sp[2] = eff_params(P, C, 0, L.slots, Q.t[b], L.vm, L.sp);
+      eff_lookup(P, L, 0);
+    }
+    __syncthreads();
+    double y[R], k[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = tid + W * r;
+      y[r] = row < P.D ? Q.y[b * (int64_t)P.D + row] : 0.0;
+    }
+    eff_rhs<R>(P, L.ybuf, y, k, L.sp, tid, W);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = tid + W * r;
+      if (row < P.D) Q.out[b * (int64_t)P.D + row] = k[r];
+    }
+    if (tid == 0) Q.status[b] = L.misc[1];
+  }
+}
+
+extern "C" void* lb_dev_eff_kernel(int id, int solver) {
+#define E(i, r, w, v) \
+  if (id == i) return solver == 1 ? (void*)&lb_eff_kernel<r, 1, w, v> : (void*)&lb_eff_kernel<r, 0, w, v>;
+  LB_EFF_CONFIGS(E)
+#undef E
+  return nullptr;
+}
+
+extern "C" void* lb_dev_eff_rhs_kernel(int id) {
+#define E(i, r, w, v) \
+  if (id == i) return (void*)&lb_eff_rhs_kernel<r, w>;
+  LB_EFF_CONFIGS(E)
+#undef E
+  return nullptr;
+}
+#endif  // LB_PART == 1
+
 #if LB_PART == 0
 __global__ void lb_queue_reset_kernel(int32_t* q) { *q = 0; }
 #endif

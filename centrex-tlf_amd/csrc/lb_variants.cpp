@@ -119,6 +119,14 @@ extern "C" int lb_dev_fixed_variant(int id, int* R, int* maxw) {
   return 0;
 }
 
+extern "C" int lb_dev_eff_variant(int id, int* R, int* maxw) {
+#define E(i, r, w, v)                                                                           \
+  if (id == i) { *R = r; *maxw = w; return 1; }
+  LB_EFF_CONFIGS(E)
+#undef E
+  return 0;
+}
+
 extern "C" int lb_dev_event_variant(int id, int* R, int* maxw) {
 #define E(i, r, w, v)                                                                           \
   if (id == i) { *R = r; *maxw = w; return 1; }

@@ -26,6 +26,9 @@
 // leaves the other kernels' register allocation alone): one slot per lane up
 // to 1024 slots, six per lane up to 3072.
 #define LB_EVENT_CONFIGS(E) E(0, 1, 1024, 4) E(1, 6, 512, 1)
+// Effective-Hamiltonian integrator: rows of the real state per lane, largest
+// workgroup, waves-per-SIMD hint.
+#define LB_EFF_CONFIGS(E) E(0, 1, 1024, 4) E(1, 4, 512, 2) E(2, 8, 512, 1)
 
 // Adjoint kernel variants: Z(id, slots per thread R, largest workgroup, waves-per-SIMD hint).
 #define LB_ADJ_CONFIGS(Z) Z(0, 1, 1024, 2) Z(1, 2, 512, 2) Z(2, 4, 256, 2) Z(3, 8, 256, 1)

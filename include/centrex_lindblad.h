@@ -334,6 +334,109 @@ int lb_solve_group(const lb_plan* const* plans, const lb_solve_args* args, int32
 int lb_variant_info(int32_t id, int32_t* slots_per_thread, int32_t* max_threads,
                     int32_t* waves_per_simd);
 
+/* ------------------------------------------------------------------------
+ * Effective-Hamiltonian Lindblad plans (rust/src/effective_lindblad/).
+ *
+ * The state is the split-real density matrix y = [Re rho.flat, Im rho.flat]
+ * (real_dim = 2 n^2 doubles, centrex_tlf/effective_hamiltonian/rust_plan.py:18-39)
+ * and the right-hand side (rust/src/effective_lindblad/rhs.rs:86-138) is
+ *
+ *   dy = A_comb(x) y + (Omega / 2) A_opt(x) y + delta A_det(x) y
+ *
+ * with x, Omega and delta the real parts of three parameter slots at t.  Each
+ * operator is CSR over real_dim rows; every stored entry is a cubic in
+ * dx = x - field_grid[k] on grid interval k, coefficients c0..c3 at
+ * coeffs[(k * nnz + entry) * 4 ...] (rust/src/effective_lindblad/plan.rs:18-40).
+ * x outside the grid by more than 1e-10 of its span fails the trajectory
+ * (LB_TRAJ_FIELD_BELOW / LB_TRAJ_FIELD_ABOVE, fail_t = x).
+ * ------------------------------------------------------------------------ */
+#define LB_TRAJ_FIELD_BELOW 10 /* "field coordinate {x:.6e} is below the operator grid minimum {g:.6e}" */
+#define LB_TRAJ_FIELD_ABOVE 11 /* "field coordinate {x:.6e} is above the operator grid maximum {g:.6e}" */
+
+#define LB_EFF_OUT_POPULATIONS 0 /* y[i*n + i], width n */
+#define LB_EFF_OUT_SELECTED 1    /* (re, im) of rho_ij, width 2 * n_selected */
+#define LB_EFF_OUT_INTEGRAL 2    /* trapezoid of sum_k w_k y[idx_k] over the pushed points, width 1 */
+#define LB_EFF_OUT_FULL 3        /* y, width real_dim */
+
+typedef struct lb_eff_plan_desc {
+  int32_t n_states;
+  int32_t real_dim;               /* 2 * n_states^2 */
+  int32_t n_grid;                 /* >= 2 */
+  const double* field_grid;       /* n_grid, ascending */
+  const int64_t* row_ptrs[3];     /* combined, optical, detuning: real_dim + 1 each */
+  const int64_t* col_indices[3];  /* nnz[k] each */
+  const double* coeffs[3];        /* (n_grid - 1) * nnz[k] * 4 each */
+  int64_t nnz[3];
+  int32_t field_slot, rabi_slot, detuning_slot;
+  /* parameter graph, in the lb_plan_desc format (no coefficients) */
+  int32_t n_slots;
+  int32_t n_base;
+  const double* base_re;
+  const double* base_im;
+  int32_t n_compounds;
+  const int32_t* compound_slots;
+  const int32_t* expr_ptrs;       /* n_compounds + 1 */
+  const int32_t* instr_op;
+  const int32_t* instr_index;
+  const int32_t* instr_argc;
+  const int32_t* instr_function;
+  const double* instr_re;
+  const double* instr_im;
+} lb_eff_plan_desc;
+
+typedef struct lb_eff_plan lb_eff_plan;
+
+/* Replaces prepare_effective_lindblad_plan_py (rust/src/effective_lindblad/python_api.rs:16-26). */
+int lb_eff_plan_create(const lb_eff_plan_desc* desc, lb_eff_plan** out_plan);
+void lb_eff_plan_destroy(lb_eff_plan* plan);
+
+/*
+ * Arguments of lb_eff_solve: trajectory b integrates the shared y0 from t0 to
+ * t1 with base-slot overrides override_values[b * n_override + m] (real, as
+ * rust/src/effective_lindblad/rhs.rs:26-44).  Points are pushed at t0 (when
+ * save_start) and every accepted step, or at the save plan `saveat` (device,
+ * ascending, t0 first when saving the start) through the dense output; each
+ * push writes a row of out_values (n_traj x capacity x width) and out_times,
+ * except for the integral, which writes its final value (one row).
+ */
+typedef struct lb_eff_solve_args {
+  int64_t n_traj;
+  int32_t solver;                 /* LB_SOLVER_DOPRI5 or LB_SOLVER_TSIT5 */
+  double t0, t1, abstol, reltol, dt;
+  int64_t maxiters;
+  int32_t save_start;
+  const double* y0;               /* device, real_dim */
+  int32_t n_override;
+  const int32_t* override_slots;  /* host */
+  const double* override_values;  /* device, n_traj x n_override */
+  int32_t output_kind;            /* LB_EFF_OUT_* */
+  int32_t n_selected;
+  const int32_t* selected_rows;   /* host */
+  const int32_t* selected_cols;
+  int32_t n_weights;
+  const int32_t* weight_indices;  /* host, indices into y */
+  const double* weight_values;    /* host */
+  int32_t n_saveat;
+  const double* saveat;           /* device */
+  int64_t capacity;               /* rows per trajectory */
+  double* out_values;             /* device */
+  double* out_times;              /* device, n_traj x capacity, or NULL */
+  int32_t* out_counts;            /* device, n_traj: points pushed */
+  int64_t* out_stats;             /* device, n_traj x 4 as lb_solve_args */
+  double* out_fail_t;             /* device, n_traj */
+  int32_t workgroups;             /* 0 = fill the device */
+} lb_eff_solve_args;
+
+/* Replaces solve_effective_lindblad_py / solve_effective_lindblad_batch_py
+ * (rust/src/effective_lindblad/python_api.rs:28-91, 152-250). */
+int lb_eff_solve(const lb_eff_plan* plan, const lb_eff_solve_args* args, void* stream);
+
+/* Batched RHS: out[b] = dy at (y[b], t[b]) (device arrays, batch rows); the
+ * counterpart of debug_effective_lindblad_rhs_py (python_api.rs:93-150).
+ * out_status[b] (device) gets 0 or an LB_TRAJ_FIELD_* code. */
+int lb_eff_rhs_eval(const lb_eff_plan* plan, int64_t batch, const double* y, const double* t,
+                    double* out, int32_t* out_status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -291,6 +291,15 @@ struct lbo_plan {
   /* split inputs (n > 40) */
   int64_t *gr_ptr, *gc_ptr;
   dterm_t *gr, *gc;
+  /* effective-Hamiltonian plan (rust/src/effective_lindblad/): dim = 2 n^2 */
+  int eff;
+  int n_grid;
+  double* grid;
+  int64_t* e_rows[3];
+  int64_t* e_cols[3];
+  double* e_coef[3];
+  int64_t e_nnz[3];
+  int field_slot, rabi_slot, det_slot;
 };
 
 #define PARTITIONED_MAX_STATES 40
@@ -312,7 +321,9 @@ void lbo_plan_destroy(lbo_plan* p) {
   void* ptrs[] = {p->base, p->compound_slots, p->exprs, p->ins, p->out_ptrs, p->in_idx,
                   p->coeff_idx, p->cconj, p->iconj, p->factor, p->dyn_coeff, p->up_re, p->up_im,
                   p->sr_ptr, p->sc_ptr, p->dr_ptr, p->dc_ptr, p->sr, p->sc, p->dr, p->dc, p->src,
-                  p->gr_ptr, p->gc_ptr, p->gr, p->gc};
+                  p->gr_ptr, p->gc_ptr, p->gr, p->gc, p->grid,
+                  p->e_rows[0], p->e_rows[1], p->e_rows[2], p->e_cols[0], p->e_cols[1], p->e_cols[2],
+                  p->e_coef[0], p->e_coef[1], p->e_coef[2]};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   free(p);
 }
@@ -555,6 +566,7 @@ typedef struct {
   const int32_t* ov_slots;
   const cpx* ov_values;
   int cache;
+  int64_t eff_hint; /* the effective plan's cached grid interval (rhs.rs:10) */
 } workspace;
 
 static void ws_init(workspace* w, const lbo_plan* p) {
@@ -578,6 +590,7 @@ static void ws_set_overrides(workspace* w, int n, const int32_t* slots, const cp
   w->ov_slots = slots;
   w->ov_values = values;
   w->cache = CACHE_NONE;
+  w->eff_hint = 0; /* a new trajectory starts a new interval hint */
 }
 
 /* evaluate_parameter_graph (rhs.rs:1176-1212): base values, overrides, compounds. */
@@ -623,8 +636,11 @@ static inline void write_out(const lbo_plan* p, int o, double re, double im, dou
   if (p->up_im[o] >= 0) out[p->up_im[o]] = im;
 }
 
+static int eff_rhs(workspace* w, const double* y, double t, double* out);
+
 static int rhs_eval(workspace* w, const double* y, double t, double* out) {
   const lbo_plan* p = w->p;
+  if (p->eff) return eff_rhs(w, y, t, out);
   if (p->partitioned) {
     int init_static = w->cache != CACHE_PARTITIONED;
     if (init_static || p->time_dependent) {
@@ -712,6 +728,127 @@ static int rhs_eval(workspace* w, const double* y, double t, double* out) {
   return 0;
 }
 
+/* ---------------- effective-Hamiltonian plans ------------------------------ */
+
+/* Rust's {:.6e}: six fraction digits, exponent without sign padding. */
+static void rust_exp6(double x, char* buf, size_t size) {
+  char tmp[64];
+  snprintf(tmp, sizeof(tmp), "%.6e", x);
+  char* e = strchr(tmp, 'e');
+  if (!e) { snprintf(buf, size, "%s", tmp); return; }
+  *e = 0;
+  int ex = atoi(e + 1);
+  snprintf(buf, size, "%se%d", tmp, ex);
+}
+
+/* find_interval_cached (rust/src/effective_lindblad/rhs.rs:47-84). */
+static int eff_find_interval(const lbo_plan* p, double x, int64_t* hint, int64_t* idx_out) {
+  const int64_t n = p->n_grid;
+  const double* g = p->grid;
+  if (n < 2) { *idx_out = 0; return 0; }
+  const double tol = 1e-10 * fmax(fabs(g[n - 1] - g[0]), 1.0);
+  char a[64], b[64];
+  if (x < g[0] - tol) {
+    rust_exp6(x, a, sizeof(a)); rust_exp6(g[0], b, sizeof(b));
+    fail("field coordinate %s is below the operator grid minimum %s", a, b);
+    return LB_TRAJ_FIELD_BELOW;
+  }
+  if (x > g[n - 1] + tol) {
+    rust_exp6(x, a, sizeof(a)); rust_exp6(g[n - 1], b, sizeof(b));
+    fail("field coordinate %s is above the operator grid maximum %s", a, b);
+    return LB_TRAJ_FIELD_ABOVE;
+  }
+  const double xc = x < g[0] ? g[0] : (x > g[n - 1] ? g[n - 1] : x);
+  if (xc <= g[0]) { *hint = 0; *idx_out = 0; return 0; }
+  if (xc >= g[n - 1]) { *hint = n - 2; *idx_out = n - 2; return 0; }
+  const int64_t h = *hint;
+  if (h < n - 1 && g[h] <= xc && xc <= g[h + 1]) { *idx_out = h; return 0; }
+  int64_t count = 0; /* partition_point(|v| v <= xc) */
+  while (count < n && g[count] <= xc) ++count;
+  int64_t idx = count > 0 ? count - 1 : 0;
+  if (idx > n - 2) idx = n - 2;
+  *hint = idx;
+  *idx_out = idx;
+  return 0;
+}
+
+/* rhs_effective_lindblad (rhs.rs:86-138): dy = 0; then each operator's
+   interpolated CSR row sums scaled by 1, Omega/2, delta (plan.rs:18-40). */
+static int eff_rhs(workspace* w, const double* y, double t, double* out) {
+  const lbo_plan* p = w->p;
+  int rc = eval_graph(w, t);
+  if (rc) return rc;
+  const double x = w->slots[p->field_slot].re;
+  const double rabi = w->slots[p->rabi_slot].re;
+  const double det = w->slots[p->det_slot].re;
+  int64_t idx = 0;
+  if ((rc = eff_find_interval(p, x, &w->eff_hint, &idx))) return rc;
+  const double dx = x - p->grid[idx];
+  const double scale[3] = {1.0, 0.5 * rabi, det};
+  for (int i = 0; i < p->dim; ++i) out[i] = 0.0;
+  for (int o = 0; o < 3; ++o) {
+    const int64_t interval = idx < p->n_grid - 2 ? idx : p->n_grid - 2;
+    const double* cb = p->e_coef[o] + interval * p->e_nnz[o] * 4;
+    for (int i = 0; i < p->dim; ++i) {
+      double sum = 0.0;
+      for (int64_t q = p->e_rows[o][i]; q < p->e_rows[o][i + 1]; ++q) {
+        const double* c = cb + q * 4;
+        const double val = c[0] + dx * (c[1] + dx * (c[2] + dx * c[3]));
+        sum += val * y[p->e_cols[o][q]];
+      }
+      out[i] += scale[o] * sum;
+    }
+  }
+  return 0;
+}
+
+int lbo_eff_plan_create(const lb_eff_plan_desc* d, lbo_plan** out) {
+  *out = NULL;
+  if (!d || d->n_states <= 0) return fail("n_states must be positive");
+  if (d->real_dim != 2 * d->n_states * d->n_states) return fail("real_dim must be 2 * n_states^2");
+  if (d->n_grid < 2) return fail("the operator grid needs at least two points");
+  lbo_plan* p = calloc(1, sizeof(*p));
+  p->eff = 1;
+  p->n = d->n_states;
+  p->dim = d->real_dim;
+  p->n_slots = d->n_slots;
+  p->n_base = d->n_base;
+  p->n_compounds = d->n_compounds;
+  p->n_grid = d->n_grid;
+  p->field_slot = d->field_slot;
+  p->rabi_slot = d->rabi_slot;
+  p->det_slot = d->detuning_slot;
+  p->grid = malloc(sizeof(double) * d->n_grid);
+  memcpy(p->grid, d->field_grid, sizeof(double) * d->n_grid);
+  for (int o = 0; o < 3; ++o) {
+    const int64_t nnz = d->nnz[o];
+    p->e_nnz[o] = nnz;
+    p->e_rows[o] = malloc(sizeof(int64_t) * (p->dim + 1));
+    memcpy(p->e_rows[o], d->row_ptrs[o], sizeof(int64_t) * (p->dim + 1));
+    p->e_cols[o] = malloc(sizeof(int64_t) * (nnz + 1));
+    if (nnz) memcpy(p->e_cols[o], d->col_indices[o], sizeof(int64_t) * nnz);
+    const size_t nc = (size_t)(d->n_grid - 1) * nnz * 4;
+    p->e_coef[o] = malloc(sizeof(double) * (nc + 1));
+    if (nc) memcpy(p->e_coef[o], d->coeffs[o], sizeof(double) * nc);
+  }
+  p->base = calloc(p->n_base + 1, sizeof(cpx));
+  for (int i = 0; i < p->n_base; ++i) p->base[i] = C(d->base_re[i], d->base_im[i]);
+  p->compound_slots = calloc(p->n_compounds + 1, sizeof(int32_t));
+  p->exprs = calloc(p->n_compounds + 1, sizeof(expr_t));
+  const int32_t n_ins = p->n_compounds ? d->expr_ptrs[p->n_compounds] : 0;
+  p->ins = calloc(n_ins + 1, sizeof(instr_t));
+  for (int32_t q = 0; q < n_ins; ++q)
+    p->ins[q] = (instr_t){d->instr_op[q], d->instr_index[q], d->instr_argc[q], d->instr_function[q], d->instr_re[q],
+                          d->instr_im[q]};
+  for (int c = 0; c < p->n_compounds; ++c) {
+    p->compound_slots[c] = d->compound_slots[c];
+    p->exprs[c].start = d->expr_ptrs[c];
+    p->exprs[c].end = d->expr_ptrs[c + 1];
+  }
+  *out = p;
+  return LB_OK;
+}
+
 int lbo_rhs(lbo_plan* plan, const double* y, double t, int32_t n_override,
             const int32_t* override_slots, const double* override_values, double* out) {
   workspace w;
@@ -772,6 +909,22 @@ static void coll_push(collector* c, double t, const double* y) {
   if (c->count >= c->capacity) { c->count++; return; }
   double* row = c->values + c->count * c->width;
   c->times[c->count] = t;
+  if (c->p->eff && a->output_kind == LBO_OUT_POPULATIONS) {
+    /* split_real_diagonal_indices (solver.rs:56-58) */
+    for (int i = 0; i < n; ++i) row[i] = y[i * n + i];
+    c->count++;
+    return;
+  }
+  if (c->p->eff && a->output_kind == LBO_OUT_SELECTED) {
+    /* split_real_selected_extractions (solver.rs:60-76) */
+    for (int s = 0; s < a->n_selected; ++s) {
+      int i = a->selected_rows[s], j = a->selected_cols[s];
+      row[2 * s] = y[i * n + j];
+      row[2 * s + 1] = i == j ? 0.0 : y[n * n + i * n + j];
+    }
+    c->count++;
+    return;
+  }
   switch (a->output_kind) {
     case LBO_OUT_POPULATIONS:
       for (int i = 0; i < n; ++i) row[i] = y[i];
@@ -1381,6 +1534,7 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
   if (has_sp && si != nsp && !st->event_fired)
     rc = fail("ended before all saveat: %lld of %lld", (long long)si, (long long)nsp);
 done:
+  if (rc == LB_TRAJ_FIELD_BELOW || rc == LB_TRAJ_FIELD_ABOVE) st->status = rc;
   if (rc && st->status == LB_TRAJ_OK) st->status = 5;
   free(buf);
   free(sp);

@@ -61,6 +61,9 @@ typedef struct lbo_solve_args {
 } lbo_solve_args;
 
 int lbo_plan_create(const lb_plan_desc* desc, lbo_plan** out);
+/* An effective-Hamiltonian plan: the same solvers and batch driver over the
+ * split-real state and the interpolated-operator RHS. */
+int lbo_eff_plan_create(const lb_eff_plan_desc* desc, lbo_plan** out);
 void lbo_plan_destroy(lbo_plan* plan);
 const char* lbo_last_error(void);
 int64_t lbo_plan_terms(const lbo_plan* plan);

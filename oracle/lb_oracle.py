@@ -87,6 +87,8 @@ def lib() -> C.CDLL:
         L.lbo_plan_create.restype = C.c_int
         L.lbo_plan_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         L.lbo_plan_destroy.argtypes = [C.c_void_p]
+        L.lbo_eff_plan_create.restype = C.c_int
+        L.lbo_eff_plan_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         L.lbo_last_error.restype = C.c_char_p
         L.lbo_plan_terms.restype = C.c_int64
         L.lbo_plan_terms.argtypes = [C.c_void_p]
@@ -270,6 +272,26 @@ class Oracle:
         if output_when == "final" and count > 0:
             result["final"] = result["values"][:, -1, :]
         return result
+
+
+class EffectiveOracle(Oracle):
+    """The oracle on an effective-Hamiltonian plan (``lbo_eff_plan_create``):
+    the same solvers, collectors and batch driver with the interpolated
+    split-real RHS.  ``plan`` is a ``centrex_tlf_amd.lindblad.effective.EffectivePlan``
+    (its host tables; no device needed)."""
+
+    def __init__(self, plan: Any):
+        class _Slots:
+            slot_names = plan.slot_names
+
+        self.prepared = _Slots()
+        self.n = plan.n_states
+        self.dim = plan.real_dim
+        handle = C.c_void_p()
+        rc = lib().lbo_eff_plan_create(C.byref(plan.desc), C.byref(handle))
+        if rc != 0:
+            raise OracleError(lib().lbo_last_error().decode())
+        self._handle = handle
 
 
 def grid_overrides(axes: Sequence[np.ndarray]) -> np.ndarray:

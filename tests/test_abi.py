@@ -26,7 +26,8 @@ def _functions(text: str) -> list[str]:
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(lb_\w+)\s*\(", body, flags=re.M)))
 
 
-def _struct_fields(text: str, name: str) -> list[tuple[str, str]]:
+def _struct_fields(text: str, name: str) -> list[tuple[str, str, int]]:
+    """(type, name, count) per field; handles `T a, b;` and `T a[3];`."""
     body = _strip_comments(text)
     m = re.search(r"typedef struct " + name + r"\s*\{(.*?)\}\s*" + name + ";", body, flags=re.S)
     assert m, name
@@ -35,8 +36,12 @@ def _struct_fields(text: str, name: str) -> list[tuple[str, str]]:
         decl = " ".join(decl.split())
         if not decl:
             continue
-        fm = re.match(r"(.*?)([\w]+)$", decl)
-        fields.append((fm.group(1).strip(), fm.group(2)))
+        fm = re.match(r"(.*?)(\w+)\s*(?:\[(\d+)\])?((?:\s*,\s*\w+)*)$", decl)
+        ctype = fm.group(1).strip()
+        count = int(fm.group(3)) if fm.group(3) else 1
+        fields.append((ctype, fm.group(2), count))
+        for extra in filter(None, (x.strip() for x in fm.group(4).split(","))):
+            fields.append((ctype, extra, 1))
     return fields
 
 
@@ -68,9 +73,11 @@ def test_library_exports_every_declared_function(lib):
     assert not missing, missing
 
 
-@pytest.mark.parametrize("name,mirror", [("lb_solve_args", _abi.LbSolveArgs), ("lb_plan_desc", _abi.LbPlanDesc)])
+@pytest.mark.parametrize("name,mirror", [("lb_solve_args", _abi.LbSolveArgs), ("lb_plan_desc", _abi.LbPlanDesc),
+                                         ("lb_eff_plan_desc", _abi.LbEffPlanDesc),
+                                         ("lb_eff_solve_args", _abi.LbEffSolveArgs)])
 def test_struct_mirrors_match_header(name, mirror):
     fields = _struct_fields(HEADER.read_text(), name)
-    assert [f for f, _ in mirror._fields_] == [n for _, n in fields]
-    for (ctype, fname), (mname, mtype) in zip(fields, mirror._fields_):
-        assert C.sizeof(mtype) == _size_of(ctype), (fname, ctype, mtype)
+    assert [f for f, _ in mirror._fields_] == [n for _, n, _ in fields]
+    for (ctype, fname, count), (mname, mtype) in zip(fields, mirror._fields_):
+        assert C.sizeof(mtype) == _size_of(ctype) * count, (fname, ctype, mtype)
