@@ -349,6 +349,201 @@ def adjoint_leg(steps: int, warmup: int) -> dict:
     }
 
 
+def _effective_synthetic_plan(n: int, seed: int = 7, density: float = 0.02, n_grid: int = 6):
+    """A synthetic effective-Hamiltonian plan: random sparse split-real
+    operators (spectral radius ~1, the combined one damped) on a field grid,
+    field x(t) = x0 + v t, Rabi rate a Gaussian pulse."""
+    import numpy as np
+
+    from centrex_tlf_amd.lindblad import LindbladParameters, Time, gaussian
+    from centrex_tlf_amd.lindblad.effective import effective_operator_payload
+
+    rng = np.random.default_rng(seed)
+    D = 2 * n * n
+    grid = np.sort(rng.uniform(-1.0, 1.0, n_grid))
+    grid[0], grid[-1] = -1.0, 1.0
+    norm = 1.0 / math.sqrt(density * D)
+
+    def mats(scale):
+        mask = rng.random((D, D)) < density
+        a, b, c = (rng.normal(size=(D, D)) * mask * norm for _ in range(3))
+        return [scale * (a + b * g + 0.3 * c * g * g) - 1.5 * np.eye(D) * (scale > 0.5) for g in grid]
+
+    params = LindbladParameters()
+    params.real("delta", 0.3)
+    x0 = params.real("x0", -0.9)
+    v = params.real("v", 0.35)
+    params.bind("field", x0 + v * Time(), finalize=False)
+    amp = params.real("Omega_amp", 1.0)
+    params.bind("Omega", gaussian(Time(), center=2.5, sigma=1.0, amplitude=amp), finalize=False)
+    params._finalize()
+    names = list(params.slot_names)
+    return {
+        "n_states": n, "real_dim": D, "n_grid": n_grid, "field_grid": grid,
+        "sparse_combined": effective_operator_payload(mats(1.0), grid, "pchip"),
+        "sparse_opt": effective_operator_payload(mats(0.3), grid, "pchip"),
+        "sparse_det": effective_operator_payload(mats(0.2), grid, "pchip"),
+        "field_coordinate_slot": names.index("field"), "rabi_rate_slot": names.index("Omega"),
+        "detuning_slot": names.index("delta"), "operator_interpolation": "pchip", "parameter_graph": params,
+    }
+
+
+def extras_leg(cpu_seconds: float) -> dict:
+    """The paths beyond the headline workload, each timed on the GPU with a
+    bounded CPU-oracle baseline and a parity sample: fixed-step RK4, a
+    population stop event and 101-point save-plan traces on the n=39 64x64
+    grid, and the effective-Hamiltonian path on a synthetic n=16 plan."""
+    import numpy as np
+    import torch
+
+    from centrex_tlf_amd.lindblad import normalize_stop_event, population, prepare_lindblad_problem
+    from centrex_tlf_amd.lindblad._driver import build_save_plan
+    from centrex_tlf_amd.lindblad.effective import prepare_effective_lindblad_plan, split_real_rho
+    from centrex_tlf_amd.lindblad.native import GridAxes
+    from oracle.lb_oracle import EffectiveOracle, Oracle, host_threads
+    from tests.systems import GAMMA, tlf_n39
+
+    dev = torch.device("cuda")
+    cores = host_threads()
+    system, params, rho0, t_span, weights = tlf_n39()
+    prepared = prepare_lindblad_problem(system, params, backend="gpu")
+    plan = prepared.device_plan()
+    oracle = Oracle(prepared)
+    names = prepared.slot_names
+    slots = [names.index("delta0"), names.index("Omega0")]
+    deltas = np.linspace(-5.0 * GAMMA, 5.0 * GAMMA, 64)
+    omegas = np.geomspace(0.1 * GAMMA, 10.0 * GAMMA, 64)
+    axes = GridAxes([deltas, omegas])
+    y0_np = prepared.layout.pack(rho0)
+    y0 = torch.as_tensor(y0_np, device=dev)
+    support = plan.support_of(y0_np)
+    n_traj = axes.count
+    rng = np.random.default_rng(20261202 + 5)
+    tol = dict(abstol=1e-10, reltol=1e-8, dt=1e-8)
+
+    def timed(fn, reps=3, around_launch=True):
+        """Median of ``reps`` timed calls after one warm call; the events
+        bracket the launch itself where the entry point records them."""
+        fn()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            if around_launch:
+                out = fn(events=(a, b))
+            else:
+                a.record()
+                out = fn()
+                b.record()
+            torch.cuda.synchronize()
+            times.append(a.elapsed_time(b))
+        return out, float(np.median(times))
+
+    def sample_size(solve_idx, total):
+        """A seeded sample sized to about ``cpu_seconds`` of oracle work on
+        all host cores, from the rate of a short probe of 2 x cores."""
+        probe = np.sort(rng.choice(total, size=min(total, 2 * cores), replace=False))
+        t0 = time.perf_counter()
+        solve_idx(probe)
+        rate = probe.size / max(time.perf_counter() - t0, 1e-6)
+        k = int(min(total, max(2 * cores, rate * cpu_seconds)))
+        return np.sort(rng.choice(total, size=k, replace=False))
+
+    def cpu_sample(solve, total=n_traj):
+        """Oracle rate on the bounded seeded sample (all host cores)."""
+        def solve_idx(idx):
+            return solve(np.stack([deltas[idx // 64], omegas[idx % 64]], axis=1).astype(np.complex128))
+        idx = sample_size(solve_idx, total)
+        t0 = time.perf_counter()
+        ref = solve_idx(idx)
+        return idx, ref, idx.size / (time.perf_counter() - t0)
+
+    out: dict = {}
+    grid_kw = dict(override_slots=slots, grid_axes=axes, support=support)
+    # ---- fixed-step RK4, dt = 2 ns (~1000 steps over 20/Gamma)
+    dt_fixed = 2e-9
+    (pops, stats, _), ms = timed(lambda **e: plan.solve(y0, *t_span, solver="fixed_rk4", dt=dt_fixed,
+                                                        output_kind="populations", **grid_kw, **e))
+    idx, ref, cpu_rate = cpu_sample(lambda pb: oracle.solve_batch(
+        y0_np, t_span, solver="fixed_rk4", dt=dt_fixed, output="populations", output_when="final",
+        parameter_slots=["delta0", "Omega0"], parameter_batch=pb, threads=cores))
+    err = float(np.max(np.abs(pops.cpu().numpy()[idx] - ref["final"])))
+    out["fixed_rk4"] = {"value": n_traj / (ms / 1e3), "unit": "trajectories/s", "kernel_ms": ms,
+                        "config": "tlf_n39 64x64 grid, fixed_rk4 dt 2 ns, final populations",
+                        "cpu_baseline": {"value": cpu_rate, "unit": "trajectories/s", "cores": cores, "kind": "port",
+                                         "sample": f"{len(idx)} seeded grid trajectories, oracle"},
+                        "parity": {"trajectories": len(idx), "max_abs_population_error": err, "bar": "1e-12"}}
+    log(f"bench: extras fixed_rk4 {out['fixed_rk4']['value']:.0f} traj/s ({ms:.2f} ms), parity {err:.1e}")
+    # ---- a population stop event (excited manifold reaches 2 %)
+    ev = normalize_stop_event(population([36, 37, 38], threshold=0.02), prepared)
+    (pops, stats, _, ev_t), ms = timed(lambda **e: plan.solve(y0, *t_span, output_kind="populations",
+                                                              stop_event=ev, **grid_kw, **tol, **e))
+    idx, ref, cpu_rate = cpu_sample(lambda pb: oracle.solve_batch(
+        y0_np, t_span, output="populations", output_when="final", parameter_slots=["delta0", "Omega0"],
+        parameter_batch=pb, stop_event=ev, threads=cores, **tol))
+    got_t = ev_t.cpu().numpy()[idx]
+    fired = np.isfinite(ref["event_t"])
+    terr = float(np.max(np.abs(got_t[fired] / ref["event_t"][fired] - 1.0))) if fired.any() else 0.0
+    out["stop_event"] = {"value": n_traj / (ms / 1e3), "unit": "trajectories/s", "kernel_ms": ms,
+                         "config": "tlf_n39 64x64 grid, dopri5 rtol 1e-8, population event sum(B) = 0.02",
+                         "fired": int(np.isfinite(ev_t.cpu().numpy()).sum()),
+                         "cpu_baseline": {"value": cpu_rate, "unit": "trajectories/s", "cores": cores,
+                                          "kind": "port", "sample": f"{len(idx)} seeded grid trajectories, oracle"},
+                         "parity": {"trajectories": len(idx), "flags_equal": bool(np.array_equal(
+                             np.isfinite(got_t), fired)), "max_rel_event_time_error": terr, "bar": "1e-7"}}
+    log(f"bench: extras stop_event {out['stop_event']['value']:.0f} traj/s ({ms:.2f} ms), event times {terr:.1e}")
+    # ---- save-plan traces: populations at 101 points through the dense output
+    sv = np.linspace(t_span[0], t_span[1], 101)[1:]  # with the start: 101 points
+    plan_t = build_save_plan(sv, *t_span, True)
+    (vals, stats, _, _, counts, _), ms = timed(lambda **e: plan.solve(
+        y0, *t_span, output_kind="populations", save_plan=plan_t, trace=True, capacity=plan_t.size, **grid_kw,
+        **tol, **e))
+    idx, ref, cpu_rate = cpu_sample(lambda pb: oracle.solve_batch(
+        y0_np, t_span, output="populations", output_when="saveat", saveat=sv, parameter_slots=["delta0", "Omega0"],
+        parameter_batch=pb, threads=cores, **tol))
+    err = float(np.max(np.abs(vals.cpu().numpy()[idx] - ref["values"])))
+    out["saveat_trace"] = {"value": n_traj / (ms / 1e3), "unit": "trajectories/s", "kernel_ms": ms,
+                           "config": "tlf_n39 64x64 grid, dopri5 rtol 1e-8, populations at 101 save points",
+                           "cpu_baseline": {"value": cpu_rate, "unit": "trajectories/s", "cores": cores,
+                                            "kind": "port", "sample": f"{len(idx)} seeded grid trajectories, oracle"},
+                           "parity": {"trajectories": len(idx), "max_abs_population_error": err, "bar": "1e-7"}}
+    log(f"bench: extras saveat_trace {out['saveat_trace']['value']:.0f} traj/s ({ms:.2f} ms), parity {err:.1e}")
+    # ---- effective-Hamiltonian path: synthetic n=16 plan (512 reals), 4096 trajectories
+    eplan = prepare_effective_lindblad_plan(_effective_synthetic_plan(16))
+    eor = EffectiveOracle(eplan)
+    r0 = np.zeros((16, 16), dtype=complex)
+    r0[0, 0] = r0[1, 1] = 0.5
+    ey0 = split_real_rho(r0)
+    B = 4096
+    amps = np.linspace(0.2, 2.0, B)
+    eslots = [eplan.slot_names.index("Omega_amp")]
+    save = build_save_plan(np.array([5.0]), 0.0, 5.0, False)
+    etol = dict(abstol=1e-10, reltol=1e-8, dt=1e-3)
+    (evals, estats, _, _, _), ms = timed(lambda: eplan.device.solve(
+        ey0, 0.0, 5.0, output_kind="populations", override_slots=eslots, override_values=amps[:, None], n_traj=B,
+        save_plan=save, save_start=False, capacity=1, **etol), around_launch=False)
+    def eff_solve(idx):
+        return eor.solve_batch(ey0, (0.0, 5.0), output="populations", output_when="final",
+                               parameter_slots=["Omega_amp"], parameter_batch=amps[idx, None].astype(complex),
+                               threads=cores, **etol)
+    eidx = sample_size(eff_solve, B)
+    t0 = time.perf_counter()
+    eref = eff_solve(eidx)
+    ecpu = eidx.size / (time.perf_counter() - t0)
+    # the synthetic operators damp the state (trace ~1e-3 at t1): error relative to the largest value
+    err = float(np.max(np.abs(evals.cpu().numpy()[eidx, 0] - eref["final"])) / np.max(np.abs(eref["final"])))
+    out["effective"] = {"value": B / (ms / 1e3), "unit": "trajectories/s", "kernel_ms": ms,
+                        "timing": "events around the call (host argument set-up included)",
+                        "config": "synthetic effective plan n=16 (512 reals, ~5.5k stored entries per operator, "
+                                  "6 field points, pchip), 4096 Rabi amplitudes, dopri5 rtol 1e-8, populations at t1",
+                        "accepted_steps_mean": float(estats.cpu().numpy()[:, 0].mean()),
+                        "cpu_baseline": {"value": ecpu, "unit": "trajectories/s", "cores": cores, "kind": "port",
+                                         "sample": f"{len(eidx)} seeded trajectories, oracle"},
+                        "parity": {"trajectories": len(eidx), "max_rel_population_error": err, "bar": "1e-6"}}
+    log(f"bench: extras effective {out['effective']['value']:.0f} traj/s ({ms:.2f} ms), parity {err:.1e}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,6 +568,9 @@ def main() -> int:
     ap.add_argument("--fit-iterations", type=int, default=50)
     ap.add_argument("--fit-time-scale", type=float, default=1.0, help="scale of the transit times in the fit")
     ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
+    ap.add_argument("--no-extras", dest="extras", action="store_false",
+                    help="skip the fixed-step, stop-event, trace and effective-path legs")
+    ap.add_argument("--extras-only", action="store_true", help="run only those legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -409,6 +607,9 @@ def main() -> int:
         return 0
     if args.uneven_only:
         print(json.dumps(uneven_leg(min(args.steps, 3), 1, args.cpu_seconds)))
+        return 0
+    if args.extras_only:
+        print(json.dumps(extras_leg(min(args.cpu_seconds, 4.0))))
         return 0
     system, params, rho0, t_span, weights = tlf_n39()
     prepared = prepare_lindblad_problem(system, params, backend="gpu")
@@ -602,6 +803,8 @@ def main() -> int:
         result["adjoint"] = adjoint_leg(args.steps, args.warmup)
     if args.uneven:
         result["uneven"] = uneven_leg(min(args.steps, 3), 1, args.cpu_seconds)
+    if args.extras:
+        result["extras"] = extras_leg(min(args.cpu_seconds, 4.0))
     print(json.dumps(result))
     return 0
 
