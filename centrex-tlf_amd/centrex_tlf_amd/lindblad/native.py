@@ -55,6 +55,8 @@ def library() -> C.CDLL:
     L.lb_plan_destroy.restype = None
     L.lb_plan_destroy.argtypes = [C.c_void_p]
     L.lb_last_error.restype = C.c_char_p
+    L.lb_plan_specialize.restype = C.c_int
+    L.lb_plan_specialize.argtypes = [C.c_void_p, C.c_int32]
     L.lb_plan_info.restype = C.c_int
     L.lb_plan_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.lb_plan_support_info.restype = C.c_int
@@ -174,6 +176,16 @@ class DevicePlan:
         self.info = dict(zip(_INFO_KEYS, [int(v) for v in info]))
         self.n = self.info["n_states"]
         self.dim = self.info["dim"]
+        self.specialized = False
+
+    def specialize(self, enable: bool = True) -> "DevicePlan":
+        """``lb_plan_specialize``: run the paired kernels of this plan with its
+        time-dependent coefficient programs compiled in (bitwise the values of
+        the expression machine).  Each (variant, solver) compiles on its first
+        solve, about 2 s; a plan without time dependence is unchanged."""
+        _check(library().lb_plan_specialize(self._handle, 1 if enable else 0))
+        self.specialized = bool(enable)
+        return self
 
     def support_of(self, y0) -> np.ndarray:
         """uint8 mask (dim) of the packed entries that are nonzero in any row of ``y0``."""

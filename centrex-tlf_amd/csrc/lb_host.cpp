@@ -20,6 +20,7 @@
 
 #include "lb_device.h"
 #include "lb_program.h"
+#include "lb_rtc.h"
 
 extern "C" int lb_dev_variant_count(void);
 extern "C" int lb_dev_variant(int id, int* R, int* maxw, int* waves);
@@ -102,7 +103,13 @@ struct lb_plan {
   void* ev_pinned = nullptr;
   size_t ev_cap = 0;
   hipEvent_t ev_copied = nullptr;
+  // plan-specialised paired kernels (lb_plan_specialize): the generated
+  // coefficient function, and (variant, solver) -> compiled module
+  bool spec = false;
+  std::string spec_source;
+  std::map<std::pair<int, int>, std::pair<hipModule_t, hipFunction_t>> spec_kernels;
   ~lb_plan() {
+    for (auto& kv : spec_kernels) (void)hipModuleUnload(kv.second.first);
     auto release = [](DevImage* im) {
       if (!im) return;
       for (void* a : im->allocations) (void)hipFree(a);
@@ -646,6 +653,22 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
 
 extern "C" void lb_plan_destroy(lb_plan* p) { delete p; }
 
+extern "C" int lb_plan_specialize(lb_plan* p, int32_t enable) {
+  if (!p) return fail(LB_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lock(p->mu);
+  if (!enable || !p->src.time_dependent) {  // nothing is evaluated per step without time dependence
+    p->spec = false;
+    return LB_OK;
+  }
+  if (p->spec_source.empty()) {
+    std::string why;
+    p->spec_source = lb_rtc_coef_source(p->src.prog, p->src.expr_dyn, p->src.K, why);
+    if (p->spec_source.empty()) return fail(LB_ERR_UNSUPPORTED, why);
+  }
+  p->spec = true;
+  return LB_OK;
+}
+
 extern "C" int lb_plan_info(const lb_plan* p, int64_t* info) {
   if (!p || !info) return fail(LB_ERR_INVALID, "null argument");
   std::memcpy(info, p->info, sizeof(p->info));
@@ -1051,6 +1074,33 @@ int launch_table(const lb_plan* p, const lb_solve_args* a, const DevImage* im, D
 
 }  // namespace
 
+namespace {
+
+// The plan-specialised form of paired variant `id` for `solver`, compiled on
+// first use; nullptr when the plan is not specialised.
+int spec_kernel(lb_plan* p, int id, int solver, hipFunction_t* fn) {
+  *fn = nullptr;
+  std::lock_guard<std::mutex> lock(p->mu);
+  if (!p->spec) return LB_OK;
+  auto it = p->spec_kernels.find({id, solver});
+  if (it == p->spec_kernels.end()) {
+    int R, maxw, waves;
+    if (!lb_dev_variant(id, &R, &maxw, &waves)) return fail(LB_ERR_INVALID, "no such kernel variant");
+    char name[96];
+    std::snprintf(name, sizeof name, "lb_solve_pair_kernel<%d, %d, %d>", solver == LB_SOLVER_TSIT5 ? 1 : 0,
+                  lb_dev_variant_rows(id), waves);
+    hipModule_t mod = nullptr;
+    hipFunction_t f = nullptr;
+    std::string why;
+    if (lb_rtc_compile(p->spec_source, name, &mod, &f, why)) return fail(LB_ERR_DEVICE, why);
+    it = p->spec_kernels.emplace(std::make_pair(id, solver), std::make_pair(mod, f)).first;
+  }
+  *fn = it->second.second;
+  return LB_OK;
+}
+
+}  // namespace
+
 extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) {
   if (!p || !a) return fail(LB_ERR_INVALID, "null argument");
   if (a->n_traj <= 0) return LB_OK;
@@ -1105,11 +1155,18 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
     grid = std::max(1, per_cu) * std::max(1, p->cus);
   }
   grid = (int)std::min<int64_t>(grid, pair ? (a->n_traj + 1) / 2 : a->n_traj);
+  hipFunction_t spec_fn = nullptr;
+  if (pair && P.time_dependent && lds <= 65536) {
+    if ((rc = spec_kernel(const_cast<lb_plan*>(p), id, a->solver, &spec_fn))) return rc;
+  }
   c.traj_offset = 0;
   void* rargs[] = {&c.queue};
   HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
   void* args[] = {&P, &c};
-  HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
+  if (spec_fn)
+    HIP_TRY(hipModuleLaunchKernel(spec_fn, grid, 1, 1, W, 1, 1, (unsigned)lds, s, args, nullptr));
+  else
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
   HIP_TRY(hipGetLastError());
   return LB_OK;
 }

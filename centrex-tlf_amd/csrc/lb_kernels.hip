@@ -16,7 +16,14 @@
 #include "lb_device.h"
 #include "lb_variants.h"
 
+#ifndef __HIPCC_RTC__
 #include <float.h>
+#else
+#define DBL_EPSILON 2.220446049250313080847e-16
+#define DBL_MAX 1.79769313486231570815e+308
+#define DBL_MIN 2.22507385850720138309e-308
+#define M_PI 3.14159265358979323846
+#endif
 
 namespace {
 
@@ -1796,6 +1803,15 @@ __device__ __forceinline__ LdsPair carve_pair(const DevPlan& P) {
 
 }  // namespace
 
+#ifdef LB_GEN_COEF
+// The plan's time-dependent compound and coefficient programs as straight-line
+// code: the operations of eval_coefficients(.., only_dynamic = true, ..) in the
+// same order on the same operands, so the values are bitwise those of the
+// expression machine.  Defined by the source that lb_rtc.cpp generates per
+// plan and compiles with this file.
+__device__ __forceinline__ void lb_gen_coef(dcpx* sl, dcpx* cf, double t);
+#endif
+
 template <int SOLVER, int G, int WAVES>
 __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan P, const DevCall C) {
   using T = Tab<SOLVER>;
@@ -1955,8 +1971,12 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
       L.park[64 + lane] = double2{k_r[1][0], k_i[1][0]};
       if (P.time_dependent && sl < 6) {
         const int st = sl + 2;
+#ifdef LB_GEN_COEF
+        lb_gen_coef(slots + st * P.n_slots, coef + st * KK, x + h * T::c(st));
+#else
         eval_coefficients(P, slots + st * P.n_slots, coef + st * KK, x + h * T::c(st), true,
                           L.vm + (half * 8 + sl) * P.vm_depth);
+#endif
       }
       wave_sync();  // stage coefficients are read by every lane of the half
       {

@@ -15,7 +15,9 @@
 #ifndef CENTREX_LINDBLAD_H
 #define CENTREX_LINDBLAD_H
 
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -130,6 +132,21 @@ typedef struct lb_plan lb_plan;
  */
 int lb_plan_create(const lb_plan_desc* desc, lb_plan** out_plan);
 void lb_plan_destroy(lb_plan* plan);
+
+/*
+ * Plan-specialised paired kernels (no counterpart in the reference, whose
+ * Rust RHS interprets the coefficient programs on every call,
+ * rust/src/lindblad/eval.rs:791-911).  With enable != 0, the plan's
+ * time-dependent compound and coefficient programs are emitted as
+ * straight-line device code; the first lb_solve call that takes a paired
+ * variant for a solver compiles that variant with the code at run time
+ * (hiprtc, about 2 s) and later calls reuse it.  Values are bitwise those of
+ * the expression machine.  A plan without time dependence is left as it is.
+ * Returns LB_ERR_UNSUPPORTED (with the reason) when an instruction has no
+ * generated form, LB_ERR_DEVICE from lb_solve when compilation fails.
+ * enable = 0 returns the plan to the precompiled kernels.
+ */
+int lb_plan_specialize(lb_plan* plan, int32_t enable);
 
 /* Thread-local message for the last failing call on this thread. */
 const char* lb_last_error(void);
