@@ -568,6 +568,8 @@ def main() -> int:
     ap.add_argument("--fit-iterations", type=int, default=50)
     ap.add_argument("--fit-time-scale", type=float, default=1.0, help="scale of the transit times in the fit")
     ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
+    ap.add_argument("--no-specialize", dest="specialize", action="store_false",
+                    help="run the precompiled kernels (expression machine) instead of the plan-specialised ones")
     ap.add_argument("--no-extras", dest="extras", action="store_false",
                     help="skip the fixed-step, stop-event, trace and effective-path legs")
     ap.add_argument("--extras-only", action="store_true", help="run only those legs")
@@ -613,6 +615,7 @@ def main() -> int:
         return 0
     system, params, rho0, t_span, weights = tlf_n39()
     prepared = prepare_lindblad_problem(system, params, backend="gpu")
+    prepared.specialize = args.specialize
     plan = prepared.device_plan()
     info = plan.info
     slot_names = prepared.slot_names
@@ -643,6 +646,9 @@ def main() -> int:
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     graph = None
     if args.graph:
+        if args.warmup == 0:  # a specialised plan compiles on its first call, not inside a capture
+            out = step()
+            torch.cuda.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             out = step()
@@ -699,6 +705,7 @@ def main() -> int:
             "integrated_elements": f"{rinfo['upper_len']} of {info['upper_len']} upper-triangle elements",
             "kernel_variant": args.variant or plan.launch_info(support=None if args.full_plan else support)["variant"],
             "hip_graph": bool(args.graph),
+            "plan_specialized": bool(plan.specialized),
             "threads_per_traj": args.threads_per_traj or rinfo["threads_per_traj"],
         },
         "roofline": {

@@ -42,6 +42,10 @@ class PreparedLindbladProblem:
     flat_plan: dict[str, Any]
     backend: str
     gpu_plan: Any | None = field(default=None, repr=False)
+    # Adaptive solves run kernels with the time-dependent coefficient programs
+    # compiled in (DevicePlan.specialize); False keeps the precompiled kernels
+    # with the expression machine.  Results are bitwise the same.
+    specialize: bool = True
 
     def check_execution_mode(self, execution_mode: str) -> None:
         """The reference's fail-fast check (``plan_static.py:37-56``): its
@@ -66,6 +70,8 @@ class PreparedLindbladProblem:
             from .native import DevicePlan
 
             self.gpu_plan = DevicePlan(self.flat_plan)
+            if self.specialize:
+                self.gpu_plan.specialize()
         return self.gpu_plan
 
     @property

@@ -195,19 +195,34 @@ def autotune(groups: Sequence[UnevenGroup], *, t0: float = 0.0, solver: str = "d
     for g, (plan, kw) in enumerate(calls):
         support = kw["support"]
         best = None
-        for info in _candidates(plan, support):
-            k = dict(kw, variant=info["variant"], threads_per_traj=info["threads_per_traj"], workgroups=0)
-            ms = _time_ms([(plan, k)], repeats)
-            trials.append({"group": g, "variant": info["variant"], "threads": info["threads_per_traj"],
-                           "solo_ms": ms})
-            if best is None or ms < best[0]:
-                best = (ms, info)
+        # candidates are ranked on the precompiled kernels: a specialised plan
+        # would compile every candidate (seconds each); the chosen ones
+        # compile once below
+        specialized = getattr(plan, "specialized", False)
+        if specialized:
+            plan.specialize(False)
+        try:
+            for info in _candidates(plan, support):
+                k = dict(kw, variant=info["variant"], threads_per_traj=info["threads_per_traj"], workgroups=0)
+                ms = _time_ms([(plan, k)], repeats)
+                trials.append({"group": g, "variant": info["variant"], "threads": info["threads_per_traj"],
+                               "solo_ms": ms})
+                if best is None or ms < best[0]:
+                    best = (ms, info)
+        finally:
+            if specialized:
+                plan.specialize(True)
         if best is None:
             raise NativeError(f"group {g}: no kernel variant can run this plan")
         solo_ms.append(best[0])
         best_var.append(best[1])
     variants = [b["variant"] for b in best_var]
     threads = [b["threads_per_traj"] for b in best_var]
+    if any(getattr(p, "specialized", False) for p, _ in calls):  # compile the chosen variants, untimed
+        import torch
+
+        solve_group(_with_tuning(calls, UnevenTuning(variants, threads, [0] * len(calls), mode="sequential")))
+        torch.cuda.synchronize()
     configs = [("sequential", [0] * len(calls))]
     if len(calls) > 1:
         cus = best_var[0]["cus"]

@@ -1077,7 +1077,10 @@ int launch_table(const lb_plan* p, const lb_solve_args* a, const DevImage* im, D
 namespace {
 
 // The plan-specialised form of paired variant `id` for `solver`, compiled on
-// first use; nullptr when the plan is not specialised.
+// first use; nullptr when the plan is not specialised.  (Only the paired
+// kernels take the generated code: a specialised multi-wave kernel launched
+// with 1024-thread workgroups had its queue aborted at dispatch, cause not
+// found, so the single- and multi-wave kernels stay on the precompiled form.)
 int spec_kernel(lb_plan* p, int id, int solver, hipFunction_t* fn) {
   *fn = nullptr;
   std::lock_guard<std::mutex> lock(p->mu);
@@ -1086,6 +1089,7 @@ int spec_kernel(lb_plan* p, int id, int solver, hipFunction_t* fn) {
   if (it == p->spec_kernels.end()) {
     int R, maxw, waves;
     if (!lb_dev_variant(id, &R, &maxw, &waves)) return fail(LB_ERR_INVALID, "no such kernel variant");
+    if (!lb_dev_variant_pair(id)) return fail(LB_ERR_INVALID, "only paired variants are specialised");
     char name[96];
     std::snprintf(name, sizeof name, "lb_solve_pair_kernel<%d, %d, %d>", solver == LB_SOLVER_TSIT5 ? 1 : 0,
                   lb_dev_variant_rows(id), waves);
@@ -1156,14 +1160,20 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   }
   grid = (int)std::min<int64_t>(grid, pair ? (a->n_traj + 1) / 2 : a->n_traj);
   hipFunction_t spec_fn = nullptr;
-  if (pair && P.time_dependent && lds <= 65536) {
-    if ((rc = spec_kernel(const_cast<lb_plan*>(p), id, a->solver, &spec_fn))) return rc;
+  if (pair && P.time_dependent && lds <= 65536 &&
+      (rc = spec_kernel(const_cast<lb_plan*>(p), id, a->solver, &spec_fn)))
+    return rc;
+  if (spec_fn && a->workgroups <= 0) {  // its own register allocation sets its residency
+    int per_cu = 0;
+    HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spec_fn, W, lds));
+    grid = (int)std::min<int64_t>((int64_t)std::max(1, per_cu) * std::max(1, p->cus),
+                                  pair ? (a->n_traj + 1) / 2 : a->n_traj);
   }
   c.traj_offset = 0;
   void* rargs[] = {&c.queue};
   HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
   void* args[] = {&P, &c};
-  if (spec_fn)
+  if (spec_fn)  // dynamic LDS within 64 KB: no per-kernel opt-in needed
     HIP_TRY(hipModuleLaunchKernel(spec_fn, grid, 1, 1, W, 1, 1, (unsigned)lds, s, args, nullptr));
   else
     HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));

@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <sstream>
 
 extern const int lb_src_count;
@@ -136,8 +138,19 @@ std::string lb_rtc_coef_source(const LbProgram& prog, const std::vector<int32_t>
   return o.str();
 }
 
-int lb_rtc_compile(const std::string& coef_source, const std::string& name_expr, hipModule_t* module,
-                   hipFunction_t* function, std::string& why) {
+namespace {
+
+// Code objects compiled in this process, by generated source and kernel
+// name: plans with the same programs (several plans of one system) compile once.
+struct CodeObject {
+  std::vector<char> code;
+  std::string lowered;
+};
+std::mutex g_code_mu;
+std::map<std::string, CodeObject> g_code;
+
+int compile_code(const std::string& coef_source, const std::string& name_expr, CodeObject* out,
+                 std::string& why) {
   // No variant tables (no part matches), the generated function after the
   // kernels that call it.
   const std::string src = "#define LB_PART 99\n#define LB_NPART 100\n#define LB_GEN_COEF 1\n"
@@ -163,19 +176,37 @@ int lb_rtc_compile(const std::string& coef_source, const std::string& name_expr,
   }
   const char* lowered = nullptr;
   size_t size = 0;
-  std::vector<char> code;
-  std::string name;
   if (hiprtcGetLoweredName(prog, name_expr.c_str(), &lowered) != HIPRTC_SUCCESS || !lowered ||
       hiprtcGetCodeSize(prog, &size) != HIPRTC_SUCCESS || size == 0) {
     why = "runtime compilation produced no kernel " + name_expr;
     hiprtcDestroyProgram(&prog);
     return -1;
   }
-  name = lowered;
-  code.resize(size);
-  hiprtcGetCode(prog, code.data());
+  out->lowered = lowered;
+  out->code.resize(size);
+  hiprtcGetCode(prog, out->code.data());
   hiprtcDestroyProgram(&prog);
-  hipError_t e = hipModuleLoadData(module, code.data());
+  return 0;
+}
+
+}  // namespace
+
+int lb_rtc_compile(const std::string& coef_source, const std::string& name_expr, hipModule_t* module,
+                   hipFunction_t* function, std::string& why) {
+  const std::string key = name_expr + "\n" + coef_source;
+  CodeObject obj;
+  {
+    std::lock_guard<std::mutex> lock(g_code_mu);
+    auto it = g_code.find(key);
+    if (it != g_code.end()) obj = it->second;
+  }
+  if (obj.code.empty()) {
+    if (compile_code(coef_source, name_expr, &obj, why)) return -1;
+    std::lock_guard<std::mutex> lock(g_code_mu);
+    g_code.emplace(key, obj);
+  }
+  const std::string& name = obj.lowered;
+  hipError_t e = hipModuleLoadData(module, obj.code.data());
   if (e != hipSuccess) {
     why = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
     return -1;
