@@ -1081,12 +1081,17 @@ namespace {
 // kernels take the generated code: a specialised multi-wave kernel launched
 // with 1024-thread workgroups had its queue aborted at dispatch, cause not
 // found, so the single- and multi-wave kernels stay on the precompiled form.)
-int spec_kernel(lb_plan* p, int id, int solver, hipFunction_t* fn) {
+int spec_kernel(lb_plan* p, int id, int solver, hipStream_t stream, hipFunction_t* fn) {
   *fn = nullptr;
   std::lock_guard<std::mutex> lock(p->mu);
   if (!p->spec) return LB_OK;
   auto it = p->spec_kernels.find({id, solver});
   if (it == p->spec_kernels.end()) {
+    // no compilation or module load inside a stream capture: a captured
+    // first call takes the precompiled kernel (bitwise the same results)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return LB_OK;
     int R, maxw, waves;
     if (!lb_dev_variant(id, &R, &maxw, &waves)) return fail(LB_ERR_INVALID, "no such kernel variant");
     if (!lb_dev_variant_pair(id)) return fail(LB_ERR_INVALID, "only paired variants are specialised");
@@ -1161,7 +1166,7 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   grid = (int)std::min<int64_t>(grid, pair ? (a->n_traj + 1) / 2 : a->n_traj);
   hipFunction_t spec_fn = nullptr;
   if (pair && P.time_dependent && lds <= 65536 &&
-      (rc = spec_kernel(const_cast<lb_plan*>(p), id, a->solver, &spec_fn)))
+      (rc = spec_kernel(const_cast<lb_plan*>(p), id, a->solver, s, &spec_fn)))
     return rc;
   if (spec_fn && a->workgroups <= 0) {  // its own register allocation sets its residency
     int per_cu = 0;

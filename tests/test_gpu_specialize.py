@@ -109,3 +109,42 @@ def test_static_plan_is_left_alone():
     got = plan.solve(y0, 0.0, 5.0, output_kind="populations", abstol=1e-10, reltol=1e-8, dt=1e-3)
     torch.cuda.synchronize()
     _same(got, ref, "static")
+
+
+def test_first_call_inside_a_capture_takes_the_precompiled_kernel():
+    """No compilation inside a stream capture: the captured first call runs the
+    precompiled kernel, a later eager call compiles; every result is bitwise
+    the expression machine's."""
+    torch = require_gpu()
+    system, params = _rich_two_level()
+    prepared = prepare_lindblad_problem(system, params, backend="gpu")
+    base = DevicePlan(prepared.flat_plan)
+    spec = DevicePlan(prepared.flat_plan).specialize()
+    dev = torch.device("cuda")
+    y0_np = prepared.layout.pack(np.diag([1.0, 0.0]).astype(complex))
+    y0 = torch.as_tensor(y0_np, device=dev)
+    names = prepared.slot_names
+    ov = torch.as_tensor(np.stack([np.linspace(0.3, 1.2, 16) + 0.1j, np.linspace(0.6, 2.0, 16) + 0j], axis=1),
+                         device=dev)
+    kw = dict(output_kind="populations", weights=[(1, 0.3)], with_integral=True,
+              override_slots=[names.index("a"), names.index("w")], override_values=ov, workgroups=8,
+              support=base.support_of(y0_np), abstol=1e-10, reltol=1e-8, dt=1e-3)  # no host copy in the capture
+    want = [t.clone() for t in base.solve(y0, 0.0, 6.0, **kw)]
+    spec.specialize(False)  # builds the plan's restricted tables (allocations) before the capture
+    spec.solve(y0, 0.0, 6.0, **kw)
+    spec.specialize(True)
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        captured = spec.solve(y0, 0.0, 6.0, stream=stream, **kw)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        for t in captured:
+            t.fill_(-1.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        _same(captured, want, "captured first call")
+    _same(spec.solve(y0, 0.0, 6.0, **kw), want, "eager call after the capture")
