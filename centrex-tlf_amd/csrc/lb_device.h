@@ -198,10 +198,15 @@ struct DevEffPlan {
   int32_t S;                // D padded to a multiple of 64
   int32_t nblk;             // S / 64
   const double* grid;       // G
-  const int32_t* row_ptr[3];
-  const int32_t* col[3];
-  const double* coef[3];    // (G - 1) x nnz x 4
-  int32_t nnz[3];
+  // Each operator in sliced form: the 64 rows of a wave are one slice, and
+  // entry j of a slice's rows lies at cell slice_ptr + 64 j + (row mod 64),
+  // so a wave's lanes read consecutive cells; rows keep their CSR order.
+  const int32_t* row_len[3];    // S (0 on padding rows)
+  const int64_t* slice_ptr[3];  // nblk
+  const int32_t* col[3];        // cells
+  const double2* coef_lo[3];    // (G - 1) x cells: (c0, c1) of the cubic
+  const double2* coef_hi[3];    // (G - 1) x cells: (c2, c3)
+  int64_t cells[3];
   int32_t field_slot, rabi_slot, det_slot;
   int32_t n_slots, n_base, n_comp, vm_depth;
   const double* base;       // n_base x 2

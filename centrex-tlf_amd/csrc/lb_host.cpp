@@ -1482,11 +1482,42 @@ extern "C" int lb_eff_plan_create(const lb_eff_plan_desc* d, lb_eff_plan** out_p
         return fail(LB_ERR_INVALID, "operator column index out of range");
       col[q] = (int32_t)d->col_indices[o][q];
     }
-    std::vector<double> cf(d->coeffs[o], d->coeffs[o] + (size_t)(d->n_grid - 1) * nnz * 4);
-    P.nnz[o] = (int32_t)nnz;
-    if ((rc = eff_upload(p.get(), rp, &P.row_ptr[o]))) return rc;
-    if ((rc = eff_upload(p.get(), col, &P.col[o]))) return rc;
-    if ((rc = eff_upload(p.get(), cf, &P.coef[o]))) return rc;
+    // sliced layout (lb_device.h): per 64-row slice, its longest row's
+    // entries x 64 cells; padding cells are never read
+    const int S = P.S, nblk = P.nblk, GI = d->n_grid - 1;
+    std::vector<int32_t> len(S, 0);
+    std::vector<int64_t> sp(nblk);
+    int64_t cells = 0;
+    for (int b = 0; b < nblk; ++b) {
+      int longest = 0;
+      for (int r = 64 * b; r < std::min(D, 64 * b + 64); ++r) {
+        len[r] = rp[r + 1] - rp[r];
+        longest = std::max(longest, len[r]);
+      }
+      sp[b] = cells;
+      cells += 64 * (int64_t)longest;
+    }
+    if (cells * (int64_t)std::max(GI, 1) > ((int64_t)1 << 31))
+      return fail(LB_ERR_UNSUPPORTED, "operator tables too large in sliced form");
+    std::vector<int32_t> scol(std::max<int64_t>(cells, 1), 0);
+    std::vector<double2> lo((size_t)std::max<int64_t>(cells * GI, 1), double2{0.0, 0.0}), hi(lo);
+    for (int r = 0; r < D; ++r) {
+      for (int j = 0; j < len[r]; ++j) {
+        const int64_t q = rp[r] + j, c = sp[r / 64] + 64 * (int64_t)j + r % 64;
+        scol[c] = col[q];
+        for (int k = 0; k < GI; ++k) {
+          const double* e = d->coeffs[o] + ((size_t)k * nnz + q) * 4;
+          lo[(size_t)k * cells + c] = double2{e[0], e[1]};
+          hi[(size_t)k * cells + c] = double2{e[2], e[3]};
+        }
+      }
+    }
+    P.cells[o] = cells;
+    if ((rc = eff_upload(p.get(), len, &P.row_len[o]))) return rc;
+    if ((rc = eff_upload(p.get(), sp, &P.slice_ptr[o]))) return rc;
+    if ((rc = eff_upload(p.get(), scol, &P.col[o]))) return rc;
+    if ((rc = eff_upload(p.get(), lo, &P.coef_lo[o]))) return rc;
+    if ((rc = eff_upload(p.get(), hi, &P.coef_hi[o]))) return rc;
   }
   // parameter graph: base values and compounds (checked as plan expressions)
   std::vector<double> base(2 * (size_t)std::max(d->n_base, 0));

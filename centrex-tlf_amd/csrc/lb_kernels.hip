@@ -3015,15 +3015,17 @@ __device__ __forceinline__ void eff_rhs(const DevEffPlan& P, double* yb, const d
     if (row < P.D) {
 #pragma unroll
       for (int o = 0; o < 3; ++o) {
-        const double* cb = P.coef[o] + (size_t)idx * P.nnz[o] * 4;
+        const double2* lo = P.coef_lo[o] + (size_t)idx * P.cells[o];
+        const double2* hi = P.coef_hi[o] + (size_t)idx * P.cells[o];
         const int* col = P.col[o];
+        const int len = P.row_len[o][row];
+        const int64_t c0 = P.slice_ptr[o][row >> 6] + (row & 63);
         double sum = 0.0;
-        const int p1 = P.row_ptr[o][row + 1];
-        for (int p = P.row_ptr[o][row]; p < p1; ++p) {
-          const double2 a = *reinterpret_cast<const double2*>(cb + 4 * (size_t)p);
-          const double2 b = *reinterpret_cast<const double2*>(cb + 4 * (size_t)p + 2);
+        for (int j = 0; j < len; ++j) {  // the row's entries in CSR order
+          const int64_t c = c0 + 64 * (int64_t)j;
+          const double2 a = lo[c], b = hi[c];
           const double val = a.x + dx * (a.y + dx * (b.x + dx * b.y));
-          sum += val * yb[col[p]];
+          sum += val * yb[col[c]];
         }
         acc += (o == 0 ? 1.0 : o == 1 ? half_rabi : det) * sum;
       }
