@@ -104,7 +104,7 @@ struct lb_plan {
   size_t ev_cap = 0;
   hipEvent_t ev_copied = nullptr;
   // plan-specialised paired kernels (lb_plan_specialize): the generated
-  // coefficient function, and (variant, solver) -> compiled module
+  // coefficient function, and (variant | gather rows << 8, solver) -> compiled module
   bool spec = false;
   std::string spec_source;
   std::map<std::pair<int, int>, std::pair<hipModule_t, hipFunction_t>> spec_kernels;
@@ -1081,11 +1081,14 @@ namespace {
 // kernels take the generated code: a specialised multi-wave kernel launched
 // with 1024-thread workgroups had its queue aborted at dispatch, cause not
 // found, so the single- and multi-wave kernels stay on the precompiled form.)
-int spec_kernel(lb_plan* p, int id, int solver, hipStream_t stream, hipFunction_t* fn) {
+int spec_kernel(lb_plan* p, int id, int solver, int rows, hipStream_t stream, hipFunction_t* fn) {
   *fn = nullptr;
   std::lock_guard<std::mutex> lock(p->mu);
   if (!p->spec) return LB_OK;
-  auto it = p->spec_kernels.find({id, solver});
+  // the kernel holds exactly the plan's gather rows in registers (fewer than
+  // the variant's maximum frees registers; the same rows in the same order)
+  const int G = std::max(1, std::min(rows, lb_dev_variant_rows(id)));
+  auto it = p->spec_kernels.find({id | (G << 8), solver});
   if (it == p->spec_kernels.end()) {
     // no compilation or module load inside a stream capture: a captured
     // first call takes the precompiled kernel (bitwise the same results)
@@ -1096,13 +1099,13 @@ int spec_kernel(lb_plan* p, int id, int solver, hipStream_t stream, hipFunction_
     if (!lb_dev_variant(id, &R, &maxw, &waves)) return fail(LB_ERR_INVALID, "no such kernel variant");
     if (!lb_dev_variant_pair(id)) return fail(LB_ERR_INVALID, "only paired variants are specialised");
     char name[96];
-    std::snprintf(name, sizeof name, "lb_solve_pair_kernel<%d, %d, %d>", solver == LB_SOLVER_TSIT5 ? 1 : 0,
-                  lb_dev_variant_rows(id), waves);
+    std::snprintf(name, sizeof name, "lb_solve_pair_kernel<%d, %d, %d>", solver == LB_SOLVER_TSIT5 ? 1 : 0, G,
+                  waves);
     hipModule_t mod = nullptr;
     hipFunction_t f = nullptr;
     std::string why;
     if (lb_rtc_compile(p->spec_source, name, &mod, &f, why)) return fail(LB_ERR_DEVICE, why);
-    it = p->spec_kernels.emplace(std::make_pair(id, solver), std::make_pair(mod, f)).first;
+    it = p->spec_kernels.emplace(std::make_pair(id | (G << 8), solver), std::make_pair(mod, f)).first;
   }
   *fn = it->second.second;
   return LB_OK;
@@ -1166,7 +1169,7 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   grid = (int)std::min<int64_t>(grid, pair ? (a->n_traj + 1) / 2 : a->n_traj);
   hipFunction_t spec_fn = nullptr;
   if (pair && P.time_dependent && lds <= 65536 &&
-      (rc = spec_kernel(const_cast<lb_plan*>(p), id, a->solver, s, &spec_fn)))
+      (rc = spec_kernel(const_cast<lb_plan*>(p), id, a->solver, (int)im->rows_total, s, &spec_fn)))
     return rc;
   if (spec_fn && a->workgroups <= 0) {  // its own register allocation sets its residency
     int per_cu = 0;
