@@ -25,6 +25,7 @@ from centrex_tlf_amd.lindblad import (
     two_level_system,
 )
 from centrex_tlf_amd.lindblad.native import DevicePlan, GridAxes
+from oracle.lb_oracle import Oracle
 
 from ._gpu import require_gpu
 from .systems import GAMMA, tlf_n39
@@ -96,6 +97,22 @@ def test_rich_expressions_are_bitwise_the_expression_machine(solver):
     torch.cuda.synchronize()
     _same(got, want, "two-level")
     assert np.all(np.isfinite(got[0].cpu().numpy()))
+    # and the oracle on the same expressions.  The right-hand side, where the
+    # expressions enter, agrees to rounding at 40 times across the span; the
+    # solves agree to 1e-6: the square-wave drive jumps, and steps that land
+    # an ulp apart (the device's sin/exp round apart from the host's) cross
+    # the jumps with different local errors
+    oracle = Oracle(prepared)
+    ts = np.linspace(0.013, 5.97, 40)
+    ys = np.repeat(prepared.layout.pack(np.array([[0.6, 0.2 + 0.1j], [0.2 - 0.1j, 0.4]]))[None], ts.size, 0)
+    rhs = base.rhs(torch.as_tensor(ys, device="cuda"), torch.as_tensor(ts, device="cuda")).cpu().numpy()
+    for b in range(ts.size):
+        want = oracle.rhs(ys[b], ts[b])
+        np.testing.assert_allclose(rhs[b], want, rtol=0, atol=1e-12 * np.abs(want).max(), err_msg=f"t={ts[b]}")
+    ref = oracle.solve_batch(prepared.layout.pack(rho0), (0.0, 6.0), solver=solver, output="full",
+                             output_when="final", parameter_slots=["a", "w"], parameter_batch=ov,
+                             abstol=1e-10, reltol=1e-8, dt=1e-3)
+    np.testing.assert_allclose(got[0].cpu().numpy(), ref["final"], rtol=0, atol=1e-6)
 
 
 def test_static_plan_is_left_alone():
