@@ -71,7 +71,14 @@ class PreparedLindbladProblem:
 
             self.gpu_plan = DevicePlan(self.flat_plan)
             if self.specialize:
-                self.gpu_plan.specialize()
+                from .native import NativeError
+
+                try:
+                    self.gpu_plan.specialize()
+                except NativeError as exc:  # an instruction without a generated form
+                    import warnings
+
+                    warnings.warn(f"plan not specialised, precompiled kernels used: {exc}", stacklevel=2)
         return self.gpu_plan
 
     @property
