@@ -577,8 +577,8 @@ def main() -> int:
                     help="run a plain 1 GiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards "
-                                   "by trajectory range, see DESIGN.md)"}))
+        print(json.dumps({"error": "the solver runs on one GPU; --gpus must be 1 (on several GPUs each "
+                                   "would solve its own contiguous range of trajectories)"}))
         return 2
 
     import numpy as np
