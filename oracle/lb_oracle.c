@@ -869,6 +869,9 @@ typedef struct {
   double* values;
   /* integral state */
   double integral, last_t, last_value;
+  /* the weighted trapezoid over t0 (save_start) and every accepted step,
+   * collected alongside another output (lbo_solve_args.integral_out) */
+  double side_integral, side_last_t, side_last_value;
   double last_push_t; /* time of the latest push (any output kind), NaN before the first */
 } collector;
 
@@ -945,6 +948,16 @@ static void coll_push(collector* c, double t, const double* y) {
       break;
   }
   c->count++;
+}
+
+/* The integral solve the reference would run beside this one (saveat None):
+ * pushes at t0 when saving the start and at every accepted step. */
+static void side_push(collector* c, double t, const double* y) {
+  if (!c->a->integral_out) return;
+  double v = weighted(c, y);
+  if (isfinite(c->side_last_t)) c->side_integral += 0.5 * (c->side_last_value + v) * (t - c->side_last_t);
+  c->side_last_t = t;
+  c->side_last_value = v;
 }
 
 int64_t lbo_output_width(const lbo_plan* p, const lbo_solve_args* a) {
@@ -1156,8 +1169,12 @@ static int solve_fixed(const lbo_plan* p, workspace* w, const lbo_solve_args* a,
   }
   double* sp = NULL;
   int64_t nsp = 0;
-  int has_sp = a->saveat != NULL;
-  if (has_sp) {
+  int has_sp = a->saveat != NULL || a->saveat_is_t1;
+  if (a->saveat_is_t1) {
+    /* output_when="final": the save plan {t1} of this trajectory (python_api.rs:589-617) */
+    sp = malloc(sizeof(double));
+    sp[nsp++] = t1;
+  } else if (has_sp) {
     sp = malloc(sizeof(double) * (a->n_saveat + 1));
     if (a->save_start) sp[nsp++] = t0;
     for (int64_t q = 0; q < a->n_saveat; ++q) {
@@ -1288,6 +1305,72 @@ done:
   return rc;
 }
 
+/* hinit (common.rs:72-118): Hairer's starting step from f0 = f(x, y) and one
+ * explicit Euler probe f1 = f(x + h0, y + h0 f0); order 5. */
+static int hinit_step(workspace* w, const double* y, double x, double t1, double atol, double rtol,
+                      double* f0, double* f1, double* yt, double* h_out) {
+  const int dim = w->p->dim;
+  int rc;
+  if ((rc = rhs_eval(w, y, x, f0))) return rc;
+  double posneg = sign_of(1.0, t1 - x), hmax = fabs(t1 - x), d0 = 0, d1 = 0, d2 = 0;
+  for (int i = 0; i < dim; ++i) {
+    double sci = atol + fabs(y[i]) * rtol;
+    d0 += (y[i] / sci) * (y[i] / sci);
+    d1 += (f0[i] / sci) * (f0[i] / sci);
+  }
+  double h0 = (d0 < 1.0e-10 || d1 < 1.0e-10) ? 1.0e-6 : 0.01 * sqrt(d0 / d1);
+  h0 = fmin(h0, hmax);
+  h0 = sign_of(h0, posneg);
+  for (int i = 0; i < dim; ++i) yt[i] = y[i] + f0[i] * h0;
+  if ((rc = rhs_eval(w, yt, x + h0, f1))) return rc;
+  for (int i = 0; i < dim; ++i) {
+    double sci = atol + fabs(y[i]) * rtol;
+    d2 += ((f1[i] - f0[i]) / sci) * ((f1[i] - f0[i]) / sci);
+  }
+  d2 = sqrt(d2) / h0;
+  double h1 = fmax(sqrt(d1), fabs(d2)) <= 1.0e-15 ? fmax(1.0e-6, fabs(h0) * 1.0e-3)
+                                                  : pow(0.01 / fmax(sqrt(d1), d2), 1.0 / 5.0);
+  *h_out = sign_of(fmin(100.0 * fabs(h0), fmin(h1, hmax)), posneg);
+  return 0;
+}
+
+int lbo_hinit(lbo_plan* plan, const double* y, double t0, double t1, double abstol, double reltol,
+              int32_t n_override, const int32_t* override_slots, const double* override_values,
+              double* h_out) {
+  const int dim = plan->dim;
+  double* buf = calloc((size_t)dim * 3, sizeof(double));
+  workspace w;
+  ws_init(&w, plan);
+  ws_set_overrides(&w, n_override, override_slots, (const cpx*)override_values);
+  int rc = hinit_step(&w, y, t0, t1, abstol, reltol, buf, buf + dim, buf + 2 * dim, h_out);
+  ws_free(&w);
+  free(buf);
+  return rc;
+}
+
+int lbo_eval_expression(int32_t n_instr, const int32_t* op, const int32_t* index, const int32_t* argc,
+                        const int32_t* function, const double* re, const double* im, int32_t n_slots,
+                        const double* slots, double t, double* out) {
+  if (n_instr <= 0) return fail("empty expression");
+  instr_t* ins = calloc((size_t)n_instr, sizeof(instr_t));
+  for (int32_t i = 0; i < n_instr; ++i) {
+    if (op[i] == OP_SLOT && (index[i] < 0 || index[i] >= n_slots)) {
+      free(ins);
+      return fail("slot index %d out of range", index[i]);
+    }
+    ins[i] = (instr_t){op[i], index[i], argc[i], function[i], re[i], im[i]};
+  }
+  expr_t e = {0, n_instr, 0};
+  cpx v;
+  int rc = eval_expr(ins, &e, (const cpx*)slots, t, &v);
+  free(ins);
+  if (!rc) {
+    out[0] = v.re;
+    out[1] = v.im;
+  }
+  return rc;
+}
+
 static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, const double* y0,
                      double t0, double t1, collector* out, ode_stats* st) {
   int dim = p->dim;
@@ -1302,8 +1385,12 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
   /* save plan (common.rs:124-151) */
   double* sp = NULL;
   int64_t nsp = 0;
-  int has_sp = a->saveat != NULL;
-  if (has_sp) {
+  int has_sp = a->saveat != NULL || a->saveat_is_t1;
+  if (a->saveat_is_t1) {
+    /* output_when="final": the save plan {t1} of this trajectory (python_api.rs:589-617) */
+    sp = malloc(sizeof(double));
+    sp[nsp++] = t1;
+  } else if (has_sp) {
     sp = malloc(sizeof(double) * (a->n_saveat + 1));
     if (a->save_start) sp[nsp++] = t0;
     for (int64_t q = 0; q < a->n_saveat; ++q) {
@@ -1336,27 +1423,7 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
   if (isfinite(a->dt) && a->dt > 0.0) {
     h = fmin(a->dt, t1 - t0);
   } else {
-    /* hinit (common.rs:72-118) */
-    if ((rc = rhs_eval(w, y, x, f0))) goto done;
-    double posneg = sign_of(1.0, t1 - x), hmax = fabs(t1 - x), d0 = 0, d1 = 0, d2 = 0;
-    for (int i = 0; i < dim; ++i) {
-      double sci = a->abstol + fabs(y[i]) * a->reltol;
-      d0 += (y[i] / sci) * (y[i] / sci);
-      d1 += (f0[i] / sci) * (f0[i] / sci);
-    }
-    double h0 = (d0 < 1.0e-10 || d1 < 1.0e-10) ? 1.0e-6 : 0.01 * sqrt(d0 / d1);
-    h0 = fmin(h0, hmax);
-    h0 = sign_of(h0, posneg);
-    for (int i = 0; i < dim; ++i) yt[i] = y[i] + f0[i] * h0;
-    if ((rc = rhs_eval(w, yt, x + h0, f1))) goto done;
-    for (int i = 0; i < dim; ++i) {
-      double sci = a->abstol + fabs(y[i]) * a->reltol;
-      d2 += ((f1[i] - f0[i]) / sci) * ((f1[i] - f0[i]) / sci);
-    }
-    d2 = sqrt(d2) / h0;
-    double h1 = fmax(sqrt(d1), fabs(d2)) <= 1.0e-15 ? fmax(1.0e-6, fabs(h0) * 1.0e-3)
-                                                    : pow(0.01 / fmax(sqrt(d1), d2), 1.0 / 5.0);
-    h = sign_of(fmin(100.0 * fabs(h0), fmin(h1, hmax)), posneg);
+    if ((rc = hinit_step(w, y, x, t1, a->abstol, a->reltol, f0, f1, yt, &h))) goto done;
   }
   controller ctrl;
   ctrl_init(&ctrl, t0, t1);
@@ -1374,6 +1441,7 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
   } else if (a->save_start) {
     coll_push(out, x, y);
   }
+  if (a->integral_out && a->integral_save_start) side_push(out, x, y);
   int64_t ns = 0, nst = 0, ia = 0;
   int last = 0;
   while (!last) {
@@ -1515,6 +1583,7 @@ static int solve_one(const lbo_plan* p, workspace* w, const lbo_solve_args* a, c
       } else if (!ev_hit) {
         coll_push(out, x, yn);
       }
+      side_push(out, x, yn);
       if (ev_hit) {
         /* push_unique (dopri5.rs:149-158, 345-351) */
         if (!(isfinite(out->last_push_t) && fabs(out->last_push_t - te) <= time_tol(te))) coll_push(out, te, ye);
@@ -1583,6 +1652,7 @@ static void* batch_worker(void* arg) {
     c.values = b->values + i * b->capacity * b->width;
     c.last_t = NAN;
     c.last_push_t = NAN;
+    c.side_last_t = NAN;
     ode_stats st;
     double t1 = b->t1_per_traj ? b->t1_per_traj[i] : b->a->t1;
     int rc = solve_one(b->p, &w, b->a, y0, b->a->t0, t1, &c, &st);
@@ -1592,6 +1662,7 @@ static void* batch_worker(void* arg) {
     b->stats[4 * i + 2] = st.rhs_calls;
     b->stats[4 * i + 3] = rc ? (st.status ? st.status : 5) : 0;
     b->fail_x[i] = st.fail_x;
+    if (b->a->integral_out) b->a->integral_out[i] = rc ? NAN : c.side_integral;
     if (b->a->event_t) b->a->event_t[i] = (!rc && st.event_fired) ? st.event_t : NAN;
     if (rc) {
       pthread_mutex_lock(&b->mu);

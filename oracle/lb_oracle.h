@@ -58,6 +58,14 @@ typedef struct lbo_solve_args {
   const double* event_re;
   const double* event_im;
   double* event_t;
+  /* n_traj, or NULL: the weighted trapezoid over t0 (save_start) and every
+   * accepted step, computed alongside the output (the integral solve the
+   * reference would run separately with saveat None; adaptive solvers, no
+   * stop event) */
+  double* integral_out;
+  int32_t integral_save_start; /* push t0 into integral_out's trapezoid */
+  int32_t saveat_is_t1;        /* the save plan is {t1} of each trajectory
+                                  (output_when="final" with per-trajectory end times) */
 } lbo_solve_args;
 
 int lbo_plan_create(const lb_plan_desc* desc, lbo_plan** out);
@@ -71,6 +79,18 @@ int64_t lbo_plan_terms(const lbo_plan* plan);
 /* One RHS evaluation; values are n_override complex pairs. */
 int lbo_rhs(lbo_plan* plan, const double* y, double t, int32_t n_override,
             const int32_t* override_slots, const double* override_values, double* out);
+
+/* The starting step of hinit (rust/src/ode/common.rs:72-118) for y at t0
+ * towards t1, with the base-slot overrides of lbo_rhs. */
+int lbo_hinit(lbo_plan* plan, const double* y, double t0, double t1, double abstol, double reltol,
+              int32_t n_override, const int32_t* override_slots, const double* override_values,
+              double* h_out);
+
+/* One expression program in the lb_plan_desc instruction format over
+ * `slots` (n_slots complex pairs) at time t; out = (re, im). */
+int lbo_eval_expression(int32_t n_instr, const int32_t* op, const int32_t* index, const int32_t* argc,
+                        const int32_t* function, const double* re, const double* im, int32_t n_slots,
+                        const double* slots, double t, double* out);
 
 /* Output width in doubles per pushed point for these arguments. */
 int64_t lbo_output_width(const lbo_plan* plan, const lbo_solve_args* args);

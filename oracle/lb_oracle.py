@@ -61,6 +61,9 @@ class LboSolveArgs(C.Structure):
         ("event_re", _f64p),
         ("event_im", _f64p),
         ("event_t", _f64p),
+        ("integral_out", _f64p),
+        ("integral_save_start", C.c_int32),
+        ("saveat_is_t1", C.c_int32),
     ]
 
 
@@ -94,6 +97,12 @@ def lib() -> C.CDLL:
         L.lbo_plan_terms.argtypes = [C.c_void_p]
         L.lbo_rhs.restype = C.c_int
         L.lbo_rhs.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_int32, _i32p, _f64p, _f64p]
+        L.lbo_hinit.restype = C.c_int
+        L.lbo_hinit.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_int32, _i32p, _f64p, _f64p]
+        L.lbo_eval_expression.restype = C.c_int
+        L.lbo_eval_expression.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p,
+                                          C.c_int32, _f64p, C.c_double, _f64p]
         L.lbo_output_width.restype = C.c_int64
         L.lbo_output_width.argtypes = [C.c_void_p, C.c_void_p]
         L.lbo_solve_batch.restype = C.c_int
@@ -146,6 +155,19 @@ class Oracle:
             raise OracleError(lib().lbo_last_error().decode())
         return out
 
+    def hinit(self, y: np.ndarray, t_span: tuple[float, float], abstol: float, reltol: float,
+              overrides: dict[Any, complex] | None = None) -> float:
+        """The starting step hinit chooses for y at t_span[0] (common.rs:72-118)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        slots, values = self._override_arrays(overrides)
+        h = np.zeros(1, np.float64)
+        rc = lib().lbo_hinit(self._handle, ptr(y, C.c_double), float(t_span[0]), float(t_span[1]),
+                             float(abstol), float(reltol), len(slots), ptr(slots, C.c_int32),
+                             ptr(values, C.c_double), ptr(h, C.c_double))
+        if rc != 0:
+            raise OracleError(lib().lbo_last_error().decode())
+        return float(h[0])
+
     def _override_arrays(self, overrides):
         if not overrides:
             return np.zeros(0, np.int32), np.zeros(0, np.float64)
@@ -162,7 +184,7 @@ class Oracle:
                     parameter_batch: np.ndarray | None = None,
                     t1_per_traj: np.ndarray | None = None, threads: int = 0,
                     stop_event: dict[str, Any] | None = None,
-                    raise_on_error: bool = True) -> dict[str, Any]:
+                    raise_on_error: bool = True, with_integral: bool = False) -> dict[str, Any]:
         """Reference-semantics batch solve; option translation of python_api.rs:589-617.
 
         ``stop_event`` is a spec of ``events.normalize_stop_event``; the root
@@ -188,6 +210,9 @@ class Oracle:
         a.abstol, a.reltol, a.dt = float(abstol), float(reltol), float(dt)
         a.maxiters = int(maxiters)
         a.save_start = int(ss)
+        if t1_per_traj is not None and output_when == "final" and not final_integral:
+            sv = None  # the plan {t1} of each trajectory
+            a.saveat_is_t1 = 1
         a.saveat = ptr(sv, C.c_double)
         a.n_saveat = 0 if sv is None else sv.size
         a.output_kind = OUT_KIND[kind]
@@ -205,6 +230,11 @@ class Oracle:
             keep += [wi, wv]
             a.n_weights = wi.size
             a.weight_indices, a.weight_values = ptr(wi, C.c_int32), ptr(wv, C.c_double)
+        side = np.full(n_traj, np.nan)
+        if with_integral:
+            # the photon count of the reference's second solve (saveat None, save_start as given)
+            a.integral_out = ptr(side, C.c_double)
+            a.integral_save_start = int(bool(save_start))
         event_t = np.full(n_traj, np.nan)
         if stop_event is not None:
             a.event_t = ptr(event_t, C.c_double)
@@ -226,6 +256,8 @@ class Oracle:
         width = lib().lbo_output_width(self._handle, C.byref(a))
         if sv is not None:
             capacity = sv.size + 1
+        elif a.saveat_is_t1:
+            capacity = 2
         elif kind == "integral" and a.store_trace == 0:
             capacity = 1
         else:
@@ -264,6 +296,7 @@ class Oracle:
             "status": stats[:, 3],
             "fail_x": fail_x,
             "event_t": event_t,
+            "integral": side if with_integral else None,
             "error": err,
             "width": width,
         }
@@ -292,6 +325,28 @@ class EffectiveOracle(Oracle):
         if rc != 0:
             raise OracleError(lib().lbo_last_error().decode())
         self._handle = handle
+
+
+def eval_expression(instructions: Sequence[tuple], t: float = 0.0,
+                    slots: Sequence[complex] = ()) -> complex:
+    """Evaluate one program of (op, index, argc, function, value) records, the
+    lb_plan_desc instruction format, with the oracle's expression machine."""
+    n = len(instructions)
+    op = np.asarray([q[0] for q in instructions], np.int32)
+    idx = np.asarray([q[1] for q in instructions], np.int32)
+    argc = np.asarray([q[2] for q in instructions], np.int32)
+    fn = np.asarray([q[3] for q in instructions], np.int32)
+    vals = np.asarray([complex(q[4]) for q in instructions], np.complex128)
+    re = np.ascontiguousarray(vals.real)
+    im = np.ascontiguousarray(vals.imag)
+    sl = np.ascontiguousarray(np.asarray(list(slots) or [0j], np.complex128).view(np.float64))
+    out = np.zeros(2, np.float64)
+    rc = lib().lbo_eval_expression(n, ptr(op, C.c_int32), ptr(idx, C.c_int32), ptr(argc, C.c_int32),
+                                   ptr(fn, C.c_int32), ptr(re, C.c_double), ptr(im, C.c_double),
+                                   max(len(slots), 1), ptr(sl, C.c_double), float(t), ptr(out, C.c_double))
+    if rc != 0:
+        raise OracleError(lib().lbo_last_error().decode())
+    return complex(out[0], out[1])
 
 
 def grid_overrides(axes: Sequence[np.ndarray]) -> np.ndarray:

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from centrex_tlf_amd.lindblad import prepare_lindblad_problem, two_level_system
-from centrex_tlf_amd.lindblad.native import kernel_variants
+from centrex_tlf_amd.lindblad.native import Tape, kernel_variants
 from oracle.lb_oracle import Oracle
 
 from ._dense import CHAIN_PARAMETERS, chain_system, random_density
@@ -112,6 +112,44 @@ def test_two_level_grid_step_totals_and_scan(golden_dir):
                                   grid_axes=[np.linspace(-2.0, 2.0, 101)])
     want = np.asarray([float(r["in_solver_final"]) for r in rows])
     np.testing.assert_allclose(values.cpu().numpy()[:, 0], want, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("solver", ["dopri5", "tsit5"])
+def test_starting_step_hinit_matches_oracle(solver):
+    """dt = 0 asks for hinit (common.rs:72-118), whose oracle form is pinned to
+    Hairer's HINIT (tests/test_oracle_hinit.py): the GPU's first accepted step
+    is the oracle's starting step, and the solves take the same steps."""
+    torch = require_gpu()
+    system, params, rho0, t_span, weights = tlf_n39()
+    prepared = prepare_lindblad_problem(system, params, backend="host")
+    oracle = Oracle(prepared)
+    plan = _device_plan(prepared)
+    pb = np.asarray([[0.0, 1.0], [2.5, 0.3], [-4.0, 7.0], [1.0, 10.0]], dtype=complex) * GAMMA
+    slots = oracle.slot_indices(["delta0", "Omega0"])
+    y0 = prepared.layout.pack(rho0)
+    kw = dict(abstol=1e-10, reltol=1e-8, dt=0.0)
+    ref = oracle.solve_batch(y0, t_span, solver=solver, output="populations", output_when="final",
+                             integral_weights=weights, with_integral=True,
+                             parameter_slots=["delta0", "Omega0"], parameter_batch=pb, **kw)
+    pops, stats, _, ints = plan.solve(y0, *t_span, solver=solver, output_kind="populations", weights=weights,
+                                      with_integral=True, override_slots=slots, override_values=pb, **kw)
+    stats = stats.cpu().numpy()
+    assert np.all(stats[:, 3] == 0)
+    np.testing.assert_array_equal(stats[:, 0], ref["accepted"])
+    np.testing.assert_array_equal(stats[:, 1], ref["rejected"])
+    np.testing.assert_allclose(pops.cpu().numpy(), ref["final"], rtol=0, atol=1e-7)
+    np.testing.assert_allclose(ints.cpu().numpy(), ref["integral"], rtol=0, atol=1e-7)
+    # the first accepted step of each trajectory, from an adjoint tape
+    support = plan.support_of(y0)
+    tape = Tape(len(pb), plan.support_info(support)["slot_count"], 4096)
+    _, st_tape, _ = plan.solve(y0, *t_span, solver=solver, output_kind="integral", weights=weights,
+                               override_slots=slots, override_values=pb, support=support, tape=tape, **kw)
+    np.testing.assert_array_equal(st_tape.cpu().numpy(), stats)
+    h0 = tape.h.cpu().numpy()[:, 0]
+    for b in range(len(pb)):
+        want = oracle.hinit(y0, t_span, 1e-10, 1e-8, {"delta0": pb[b, 0], "Omega0": pb[b, 1]})
+        if ref["rejected"][b] == 0:
+            assert h0[b] == pytest.approx(want, rel=1e-12), b
 
 
 @pytest.mark.parametrize("restrict", [True, False], ids=["restricted", "full"])
