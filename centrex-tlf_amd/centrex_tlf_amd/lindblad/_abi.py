@@ -15,8 +15,10 @@ __all__ = [
     "LB_OK",
     "LB_MODE_PERSISTENT",
     "LB_MODE_PLAIN",
+    "LB_ADJOINT",
     "LB_EVENT",
     "LB_OUT",
+    "LB_TAPE",
     "LB_SOLVER",
     "LB_TRAJ",
     "LbPlanDesc",
@@ -35,6 +37,9 @@ LB_MODE_PLAIN = 1
 LB_TRAJ = {0: "ok", 1: "maxiters", 2: "underflow", 3: "stiff", 4: "bad_span", 5: "bad_support",
            6: "saveat", 7: "capacity", 8: "bad_dt", 9: "event_complex", 10: "field_below", 11: "field_above"}
 LB_EVENT = {"none": 0, "population": 1, "runtime_expression": 2}
+# attempt-tape flags and adjoint modes
+LB_TAPE = {"accepted": 1, "clipped": 2, "clamped": 4, "hmax": 8, "minrule": 16, "errq": 32, "first": 64}
+LB_ADJOINT = {"adaptive": 0, "frozen": 1}
 
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
@@ -69,6 +74,7 @@ class LbPlanDesc(C.Structure):
         ("factors_im", _f64p),
         ("n_diff_params", C.c_int32),
         ("diff_param_slots", _i32p),
+        ("diff_time", C.c_int32),
     ]
 
 
@@ -136,6 +142,9 @@ class LbSolveArgs(C.Structure):
         ("event_re", _f64p),
         ("event_im", _f64p),
         ("out_event_t", C.c_void_p),
+        ("tape_c", C.c_void_p),
+        ("tape_f", C.c_void_p),
+        ("adjoint_mode", C.c_int32),
     ]
 
 
@@ -255,4 +264,5 @@ def make_plan_desc(flat: dict[str, Any]) -> tuple[LbPlanDesc, list[np.ndarray]]:
     d.n_diff_params = int(flat.get("n_diff_params", 0))
     if d.n_diff_params:
         d.diff_param_slots = ptr(arr("diff_param_slots", np.int32), C.c_int32)
+    d.diff_time = int(flat.get("diff_time", 0))
     return d, keep

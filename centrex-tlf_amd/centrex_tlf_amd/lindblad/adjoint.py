@@ -1,14 +1,20 @@
 """Gradients of photon counts through the discrete adjoint on the GPU.
 
 ``DifferentiablePhotonCount`` wraps a differentiable device plan: the
-prepared problem's plan plus the parameter-derivative programs of its
-coefficients (``derivatives.differentiate_flat_plan``) and the transposed
-term tables.  Its forward pass is one batched solve that records a tape (the
-accepted steps and a checkpoint every ``ckpt_every`` steps); its backward
-pass is one launch of the adjoint kernel (``lb_adjoint``), which recomputes
-each checkpoint segment and sweeps the Dopri5/Tsit5 stage adjoints backwards.
-The gradient is that of the discrete solution with the accepted step sizes
-held fixed.  ``photon_count`` exposes it as a ``torch.autograd.Function``.
+prepared problem's plan plus the parameter- and time-derivative programs of
+its coefficients (``derivatives.differentiate_flat_plan``) and the
+transposed term tables.  Its forward pass is one batched solve that records
+a tape (every step attempt, with the controller's inputs and decision, and a
+checkpoint every ``ckpt_every`` attempts); its backward pass is one launch of
+the adjoint kernel (``lb_adjoint``), which recomputes each checkpoint segment
+and sweeps the Dopri5/Tsit5 stage adjoints backwards.
+
+With ``mode="adaptive"`` (the default) the gradient is that of the adaptive
+solve itself: the error norm, the step-size controller, rejected attempts,
+the last-step clip and the trapezoid's step lengths are differentiated, so it
+is the derivative the finite differences of the forward see.  ``mode="frozen"``
+holds the accepted step sizes fixed (the derivative of ``replay``).
+``photon_count`` exposes it as a ``torch.autograd.Function``.
 
 The reference has no gradients; the objective is its photon-count output
 (``output="photon_integral"``: trapezoid of sum_k w_k rho_kk over t0 and every
@@ -42,9 +48,13 @@ class DifferentiablePhotonCount:
     def __init__(self, prepared: Any, parameters: Sequence[Any], integral_weights: Sequence[tuple[int, float]],
                  t_span: Sequence[float], *, solver: str = "dopri5", abstol: float = 1e-7,
                  reltol: float = 1e-4, dt: float = 1e-8, maxiters: int = 100_000, save_start: bool = True,
-                 ckpt_every: int = 16, tape_capacity: int = 1024, fixed: dict[str, Any] | None = None):
+                 ckpt_every: int = 16, tape_capacity: int = 1024, fixed: dict[str, Any] | None = None,
+                 mode: str = "adaptive"):
         if solver not in D.GPU_SOLVERS:
             raise NotImplementedError("the adjoint supports 'dopri5' and 'tsit5'")
+        if mode not in ("adaptive", "frozen"):
+            raise ValueError("mode must be 'adaptive' or 'frozen'")
+        self.mode = mode
         self.prepared = prepared
         self.names = [D.parameter_slot_name(p) for p in parameters]
         self.slots = D.parameter_slot_indices(prepared, parameters)
@@ -64,7 +74,7 @@ class DifferentiablePhotonCount:
                        maxiters=int(maxiters), save_start=bool(save_start))
         self.every = int(ckpt_every)
         self.capacity = int(tape_capacity)
-        self.plan = DevicePlan(differentiate_flat_plan(prepared.flat_plan, self.slots))
+        self.plan = DevicePlan(differentiate_flat_plan(prepared.flat_plan, self.slots, with_time=True))
 
     def _common(self, params, y0):
         import torch
@@ -116,8 +126,9 @@ class DifferentiablePhotonCount:
             return vals[:, 0], tape
 
     def replay(self, params, y0, tape):
-        """Photon counts (B,) integrated with the step sizes recorded in ``tape``
-        (no step-size control): the discrete map the adjoint differentiates."""
+        """Photon counts (B,) integrated with the accepted step sizes recorded
+        in ``tape`` (no step-size control): the map the frozen adjoint
+        differentiates."""
         import torch
 
         p, y, support, _ = self._common(params, y0)
@@ -128,10 +139,11 @@ class DifferentiablePhotonCount:
             raise ValueError("replay of the recorded step sizes failed")
         return vals[:, 0]
 
-    def gradient(self, params, y0, tape, variant: int = 0, events=None):
+    def gradient(self, params, y0, tape, variant: int = 0, events=None, mode: str | None = None):
         """dI_b/dp_b (B, P) for a tape recorded by ``forward`` with the same params.
 
-        ``variant`` = k + 1 forces adjoint kernel variant k (0: the library's choice).
+        ``variant`` = k + 1 forces adjoint kernel variant k (0: the library's
+        choice); ``mode`` overrides the engine's ("adaptive" or "frozen").
         """
         import torch
 
@@ -140,7 +152,7 @@ class DifferentiablePhotonCount:
         return self.plan.solve(
             y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self._overrides(p)[0],
             override_values=self._overrides(p)[1], support=support, tape=tape, adjoint_grad=grad,
-            variant=int(variant), events=events, **self.kw)
+            variant=int(variant), events=events, adjoint_mode=mode or self.mode, **self.kw)
 
 
 def _autograd_function():

@@ -12,6 +12,12 @@ bytecode, so the device expression machine evaluates them unchanged:
 * K * P derivative programs are appended after the K coefficient programs,
   program k * P + m being d c_k / d p_m.
 
+With ``with_time`` the time t is differentiated as one more column: the
+programs are then K * (P + 1), program k * (P + 1) + P being d c_k / d t.  The
+adjoint of the adaptive solve needs it because the accepted step sizes, and
+so the stage times, depend on the parameters through the step-size
+controller.
+
 Rules (p real): linear ops and conj commute with d/dp; products, quotients,
 powers with a parameter-free exponent, exp/sin/cos/tan by the usual
 formulas; |z|' = Re(conj(z) z') / |z|; Re/Im of the derivative; comparisons
@@ -133,21 +139,24 @@ def _call(function: int, *args: tuple) -> tuple:
     return ("call", Op.BUILTIN_FUNC, int(function), tuple(args))
 
 
-def _deriv(node: tuple, dslot: dict[int, tuple]) -> tuple:
-    """d node / d p, with dslot[s] the derivative of slot s (missing = 0)."""
+def _deriv(node: tuple, dslot: dict[int, tuple], wrt_time: bool = False) -> tuple:
+    """d node / d p, with dslot[s] the derivative of slot s (missing = 0);
+    d node / d t when ``wrt_time``."""
     kind = node[0]
-    if kind in ("zero", "const", "time"):
+    if kind == "time":
+        return ONE if wrt_time else ZERO
+    if kind in ("zero", "const"):
         return ZERO
     if kind == "slot":
         return dslot.get(node[1], ZERO)
     if kind == "un":
-        d = _deriv(node[2], dslot)
+        d = _deriv(node[2], dslot, wrt_time)
         return ZERO if _is_zero(d) else ("un", node[1], d)
     if kind == "bin":
         op, a, b = node[1], node[2], node[3]
         if op in _COMPARE:
             return ZERO
-        da, db = _deriv(a, dslot), _deriv(b, dslot)
+        da, db = _deriv(a, dslot, wrt_time), _deriv(b, dslot, wrt_time)
         if op == Op.ADD:
             return _add(da, db)
         if op == Op.SUB:
@@ -167,7 +176,7 @@ def _deriv(node: tuple, dslot: dict[int, tuple]) -> tuple:
         raise NotImplementedError(f"binary op {op}")
     if kind == "call":
         op, function, args = node[1], node[2], node[3]
-        dargs = [_deriv(a, dslot) for a in args]
+        dargs = [_deriv(a, dslot, wrt_time) for a in args]
         if all(_is_zero(d) for d in dargs):
             return ZERO
         if op == Op.HELPER_FUNC:
@@ -200,13 +209,15 @@ def _program(node: tuple) -> list[tuple]:
     return out
 
 
-def differentiate_flat_plan(flat: dict[str, Any], param_slots: Sequence[int]) -> dict[str, Any]:
+def differentiate_flat_plan(flat: dict[str, Any], param_slots: Sequence[int],
+                            with_time: bool = False) -> dict[str, Any]:
     """A copy of ``flat`` with derivative compounds and K*P derivative programs.
 
     ``param_slots`` are base-slot indices (the differentiated parameters).  The
     returned plan has ``n_diff_params = P`` and ``diff_param_slots``; its
     expression list is compounds (original, then derivative compounds),
-    the K coefficients, then the derivative programs (k * P + m).
+    the K coefficients, then the derivative programs (k * P' + m, P' = P, or
+    P + 1 with ``with_time``: column P is d c_k / d t and ``diff_time = 1``).
     """
     n_base = int(flat["n_base"])
     params = [int(s) for s in param_slots]
@@ -222,10 +233,11 @@ def differentiate_flat_plan(flat: dict[str, Any], param_slots: Sequence[int]) ->
     next_slot = int(flat["n_slots"])
     deriv_compounds: list[tuple[int, list[tuple]]] = []
     dslots: list[dict[int, tuple]] = []
-    for m, p in enumerate(params):
-        dslot: dict[int, tuple] = {p: ONE}
+    columns = [(p, False) for p in params] + ([(None, True)] if with_time else [])
+    for p, wrt_time in columns:
+        dslot: dict[int, tuple] = {} if wrt_time else {p: ONE}
         for c in range(n_comp):
-            d = _deriv(trees[c], dslot)
+            d = _deriv(trees[c], dslot, wrt_time)
             if _is_zero(d):
                 continue
             if d[0] == "const" or d[0] == "slot":
@@ -235,7 +247,8 @@ def differentiate_flat_plan(flat: dict[str, Any], param_slots: Sequence[int]) ->
             deriv_compounds.append((next_slot, _program(d)))
             next_slot += 1
         dslots.append(dslot)
-    deriv_coefs = [_program(_deriv(trees[n_comp + k], dslots[m])) for k in range(K) for m in range(len(params))]
+    deriv_coefs = [_program(_deriv(trees[n_comp + k], dslots[m], columns[m][1]))
+                   for k in range(K) for m in range(len(columns))]
 
     programs = [rows[ptrs[e]:ptrs[e + 1]] for e in range(n_comp)]
     programs += [p for _, p in deriv_compounds]
@@ -259,5 +272,6 @@ def differentiate_flat_plan(flat: dict[str, Any], param_slots: Sequence[int]) ->
         "instr_im": np.ascontiguousarray(ins[:, 5]),
         "n_diff_params": len(params),
         "diff_param_slots": np.asarray(params, dtype=np.int32),
+        "diff_time": int(bool(with_time)),
     })
     return out

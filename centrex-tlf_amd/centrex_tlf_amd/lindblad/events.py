@@ -1,12 +1,19 @@
-"""Terminal stop events (``centrex_tlf/lindblad/events.py:20-94``).
+"""Terminal stop events (the ``stop_event`` argument of the reference's
+solvers, ``centrex_tlf/lindblad/events.py:20-94``).
 
-A stop event is a function g(t, y) whose sign change (or zero) between two
-accepted steps ends a trajectory at the root: the sum of some populations
-minus a threshold (``population(...)``), or a real runtime expression of t and
-the parameters (a ``RuntimeExpression``, ``Parameter`` or sympy expression).
-``normalize_stop_event`` checks one against a prepared problem and returns the
-spec the GPU solver takes: the population indices and threshold, or the
-expression compiled to the plan's bytecode over the plan's parameter slots.
+A stop event is a function g(t, rho) whose zero, or sign change between two
+accepted steps, ends a trajectory at the root.  Two kinds exist:
+
+* a population event, g = sum of rho_kk over some states minus a threshold,
+  made with :func:`population`;
+* an expression event, g = Re of a runtime expression of t and the
+  parameters (a ``RuntimeExpression``, a ``Parameter`` or a sympy expression).
+
+:func:`normalize_stop_event` checks an event against a prepared problem and
+returns the spec the GPU solver and the oracle take: a dict with ``kind``
+"population" (``indices``, ``threshold``) or "runtime_expression" (the
+expression compiled to the plan's bytecode over its parameter slots), and a
+``name``.
 """
 
 from __future__ import annotations
@@ -23,59 +30,59 @@ from .parameters import Parameter, RuntimeExpression
 
 __all__ = ["PopulationEvent", "normalize_stop_event", "population"]
 
+_ACCEPTED = "a RuntimeExpression, sympy expression, Parameter, PopulationEvent, or None"
+
 
 @dataclass(frozen=True)
 class PopulationEvent:
+    """g = sum of the populations at ``indices`` minus ``threshold``."""
+
     indices: tuple[int, ...]
     threshold: float = 0.0
     name: str = "population"
 
+    def spec(self, n_states: int) -> dict[str, Any]:
+        """The solver's spec, after checking every index against the system size."""
+        outside = [k for k in self.indices if not 0 <= k < n_states]
+        if outside:
+            raise ValueError(f"population event index {outside[0]} out of bounds for {n_states} states")
+        return {"kind": "population", "indices": list(self.indices), "threshold": float(self.threshold),
+                "name": self.name}
+
 
 def population(index_or_indices: int | Sequence[int], *, threshold: float = 0.0,
                name: str | None = None) -> PopulationEvent:
-    """Event on the sum of the populations at ``index_or_indices`` crossing ``threshold``."""
-    if isinstance(index_or_indices, int | np.integer):
-        indices = (int(index_or_indices),)
-    else:
-        indices = tuple(int(index) for index in index_or_indices)
-    if not indices:
+    """Event on the summed population of one state or several crossing ``threshold``."""
+    picked = tuple(int(k) for k in np.atleast_1d(np.asarray(index_or_indices, dtype=object)).ravel())
+    if len(picked) == 0:
         raise ValueError("population event requires at least one index")
-    return PopulationEvent(indices=indices, threshold=float(threshold),
-                           name=name if name is not None else "population")
+    return PopulationEvent(indices=picked, threshold=float(threshold), name="population" if name is None else name)
 
 
-def _as_runtime_expression(stop_event: Any) -> RuntimeExpression | None:
-    if isinstance(stop_event, RuntimeExpression):
-        return stop_event
-    if isinstance(stop_event, Parameter):
-        return stop_event.as_expression()
-    if isinstance(stop_event, smp.Expr):
-        return RuntimeExpression(stop_event, {})
-    return None
+def _expression_event(event: Any) -> RuntimeExpression:
+    match event:
+        case RuntimeExpression():
+            return event
+        case Parameter():
+            return event.as_expression()
+        case smp.Expr():
+            return RuntimeExpression(event, {})
+    raise TypeError(f"stop_event must be {_ACCEPTED}")
 
 
 def normalize_stop_event(stop_event: Any | None, prepared: Any) -> dict[str, Any] | None:
-    """The event spec for the solver (messages as ``events.py:57-94``)."""
+    """The solver's spec of ``stop_event`` for ``prepared`` (None passes through);
+    the checks and messages of the reference's ``normalize_stop_event``."""
     if stop_event is None:
         return None
-    n_states = int(prepared.layout.n)
     if isinstance(stop_event, PopulationEvent):
-        indices = [int(index) for index in stop_event.indices]
-        for index in indices:
-            if index < 0 or index >= n_states:
-                raise ValueError(f"population event index {index} out of bounds for {n_states} states")
-        return {"kind": "population", "indices": indices, "threshold": float(stop_event.threshold),
-                "name": stop_event.name}
-    runtime = _as_runtime_expression(stop_event)
-    if runtime is None:
-        raise TypeError("stop_event must be a RuntimeExpression, sympy expression, Parameter, "
-                        "PopulationEvent, or None")
-    slot_index_by_name = {name: idx for idx, name in enumerate(prepared.slot_names)}
-    missing = sorted(str(s) for s in runtime.expr.free_symbols
-                     if str(s) != "t" and str(s) not in slot_index_by_name)
-    if missing:
-        raise ValueError(f"stop_event references unknown parameter(s): {', '.join(missing)}")
-    compiled = compile_expression(runtime.expr, slot_index_by_name)
-    if compiled.output_is_tuple:
+        return stop_event.spec(int(prepared.layout.n))
+    runtime = _expression_event(stop_event)
+    slot_of = {name: k for k, name in enumerate(prepared.slot_names)}
+    unknown = sorted({str(s) for s in runtime.expr.free_symbols} - set(slot_of) - {"t"})
+    if unknown:
+        raise ValueError(f"stop_event references unknown parameter(s): {', '.join(unknown)}")
+    program = compile_expression(runtime.expr, slot_of)
+    if program.output_is_tuple:
         raise ValueError("stop_event RuntimeExpression must evaluate to a scalar")
-    return {"kind": "runtime_expression", "expression": compiled, "runtime": runtime, "name": repr(runtime)}
+    return {"kind": "runtime_expression", "expression": program, "runtime": runtime, "name": repr(runtime)}

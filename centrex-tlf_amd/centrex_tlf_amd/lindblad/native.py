@@ -13,7 +13,8 @@ from typing import Any, Sequence
 
 import numpy as np
 
-from ._abi import LB_EVENT, LB_MODE_PERSISTENT, LB_MODE_PLAIN, LB_OUT, LB_SOLVER, LbSolveArgs, make_plan_desc, ptr
+from ._abi import (LB_ADJOINT, LB_EVENT, LB_MODE_PERSISTENT, LB_MODE_PLAIN, LB_OUT, LB_SOLVER, LB_TAPE, LbSolveArgs,
+                   make_plan_desc, ptr)
 
 __all__ = ["DevicePlan", "NativeError", "Tape", "library", "library_path", "torch_device"]
 
@@ -141,7 +142,10 @@ class GridAxes:
 
 
 class Tape:
-    """Device arrays of an adjoint tape for B trajectories (see lb_solve_args)."""
+    """Device arrays of an adjoint tape for B trajectories (see lb_solve_args):
+    one record per step attempt (start ``t``, size ``h``, ``c`` = (error norm,
+    proposed next step, fac_old), LB_TAPE_* flags ``f``), the state ``y`` at
+    the start of every ``every``-th attempt, and ``n`` attempts per trajectory."""
 
     def __init__(self, n_traj: int, slots: int, cap: int, every: int = 16):
         import torch
@@ -151,12 +155,31 @@ class Tape:
         n_ckpt = (self.cap + self.every - 1) // self.every
         self.t = torch.zeros((n_traj, self.cap), dtype=torch.float64, device=dev)
         self.h = torch.zeros((n_traj, self.cap), dtype=torch.float64, device=dev)
+        self.c = torch.zeros((n_traj, self.cap, 3), dtype=torch.float64, device=dev)
+        self.f = torch.zeros((n_traj, self.cap), dtype=torch.int32, device=dev)
         self.y = torch.zeros((n_traj, n_ckpt, int(slots), 2), dtype=torch.float64, device=dev)
         self.n = torch.zeros(n_traj, dtype=torch.int32, device=dev)
 
     @property
     def nbytes(self) -> int:
-        return sum(int(x.numel() * x.element_size()) for x in (self.t, self.h, self.y, self.n))
+        return sum(int(x.numel() * x.element_size()) for x in (self.t, self.h, self.c, self.f, self.y, self.n))
+
+    def accepted(self) -> "Schedule":
+        """The accepted step sizes in order (a replay schedule): the attempts
+        flagged accepted, moved to the front of each row by a stable sort."""
+        import torch
+
+        live = torch.arange(self.cap, device=self.n.device)[None, :] < self.n[:, None].long()
+        acc = live & ((self.f & LB_TAPE["accepted"]) != 0)
+        order = torch.argsort((~acc).to(torch.int8), dim=1, stable=True)
+        return Schedule(torch.gather(self.h, 1, order).contiguous(), acc.sum(dim=1).to(torch.int32), self.cap)
+
+
+class Schedule:
+    """Accepted step sizes (B, cap) and their counts (B,): a replay of a tape."""
+
+    def __init__(self, h, n, cap: int):
+        self.h, self.n, self.cap = h, n, int(cap)
 
 
 class DevicePlan:
@@ -256,7 +279,7 @@ class DevicePlan:
               restrict_support: bool = True, support=None, events=None,
               save_plan=None, trace: bool = False, capacity: int = 0, with_times: bool = False,
               tape=None, adjoint_grad=None, replay=None, queue_order=None, stop_event=None,
-              _launch: bool = True):
+              adjoint_mode: str = "adaptive", _launch: bool = True):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
@@ -275,9 +298,14 @@ class DevicePlan:
         (``times`` only with ``with_times``, ``integral`` only with
         ``with_integral``).
 
-        ``tape`` (a :class:`Tape`) records the accepted steps for the adjoint;
+        ``tape`` (a :class:`Tape`) records every step attempt for the adjoint;
         with ``adjoint_grad`` (a (B, P) float64 tensor) the call runs the
-        adjoint of that recorded tape instead of a solve (``lb_adjoint``).
+        adjoint of that recorded tape instead of a solve (``lb_adjoint``):
+        ``adjoint_mode`` "adaptive" differentiates the adaptive solve (the
+        plan needs the time column, ``differentiate_flat_plan(...,
+        with_time=True)``), "frozen" holds the accepted step sizes fixed.
+        ``replay`` (a :class:`Tape` or :class:`Schedule`) integrates with the
+        accepted step sizes of a recorded tape instead of the controller.
 
         ``stop_event`` (a spec from ``events.normalize_stop_event``) ends each
         trajectory at the first root of its event function; the root times
@@ -405,13 +433,18 @@ class DevicePlan:
         if support is not None:
             keep.append(support)
             a.y0_support = ptr(support, C.c_uint8)
-        if replay is not None:  # integrate with the step sizes of a recorded tape
+        if replay is not None:  # integrate with the accepted step sizes of a recorded tape
+            if isinstance(replay, Tape):
+                replay = replay.accepted()
+            keep.append(replay)
             a.replay_h, a.replay_n = _dptr(replay.h), _dptr(replay.n)
             a.tape_cap = int(replay.cap)
         if tape is not None:
             a.tape_t, a.tape_h = _dptr(tape.t), _dptr(tape.h)
+            a.tape_c, a.tape_f = _dptr(tape.c), _dptr(tape.f)
             a.tape_y, a.tape_n = _dptr(tape.y), _dptr(tape.n)
             a.tape_cap, a.ckpt_every = int(tape.cap), int(tape.every)
+        a.adjoint_mode = LB_ADJOINT[adjoint_mode]
         order_t = None
         if queue_order is not None:  # queue position -> trajectory (e.g. longest first)
             order_t = (queue_order.to(device=dev, dtype=torch.int32).reshape(-1).contiguous()

@@ -18,6 +18,7 @@ refer to.
 
 from __future__ import annotations
 
+import functools
 import numbers
 from collections import OrderedDict
 from collections.abc import Mapping, Sequence
@@ -74,30 +75,41 @@ def _normalize_symbol_name(value: str) -> str:
     return value.replace("ϕ", "φ")
 
 
-def _is_sequence(value: Any) -> bool:
-    if isinstance(value, np.ndarray):
-        return value.ndim == 1
-    return isinstance(value, Sequence) and not isinstance(value, str | bytes | bytearray)
-
-
-def _coerce_base_value(value: Any) -> complex | tuple[complex, ...]:
-    if isinstance(value, bool):
-        return complex(float(value), 0.0)
-    if isinstance(value, np.generic):
-        return _coerce_base_value(value.item())
-    if isinstance(value, numbers.Number):
-        return complex(value)
-    if _is_sequence(value):
-        if isinstance(value, np.ndarray):
-            return tuple(_coerce_base_value(v.item()) for v in value.reshape(-1))
-        return tuple(_coerce_base_value(v) for v in value)
+@functools.singledispatch
+def _slot_value(value: Any) -> complex | tuple[complex, ...]:
+    """A base parameter's stored value: a complex scalar, or a tuple of them for
+    a one-dimensional sequence (the tuple-valued parameters of the helpers)."""
+    if isinstance(value, Sequence) and not isinstance(value, str | bytes | bytearray):
+        return tuple(_slot_value(item) for item in value)
     raise TypeError(f"unsupported base parameter type {type(value).__name__}")
 
 
+@_slot_value.register
+def _(value: bool) -> complex:
+    return complex(float(value), 0.0)
+
+
+@_slot_value.register
+def _(value: numbers.Number) -> complex:
+    return complex(value)
+
+
+@_slot_value.register
+def _(value: np.generic) -> complex | tuple[complex, ...]:
+    return _slot_value(value.item())
+
+
+@_slot_value.register
+def _(value: np.ndarray) -> tuple[complex, ...]:
+    if value.ndim != 1:
+        raise TypeError(f"unsupported base parameter type {type(value).__name__}")
+    return tuple(_slot_value(item.item()) for item in value)
+
+
 def _infer_kind(default: Any) -> str:
-    coerced = _coerce_base_value(default)
-    values = coerced if isinstance(coerced, tuple) else (coerced,)
-    return "complex" if any(abs(v.imag) > 0.0 for v in values) else "real"
+    stored = _slot_value(default)
+    parts = stored if isinstance(stored, tuple) else (stored,)
+    return "complex" if any(part.imag != 0.0 for part in parts) else "real"
 
 
 def _parse_expr(expr: str) -> smp.Expr:
@@ -396,47 +408,72 @@ def pchip_tabulated(x, grid: Parameter, values: Parameter) -> RuntimeExpression:
     return _call_function("pchip_interp", x, grid, values)
 
 
+def _is_binding_mapping(mapping: Mapping[Any, Any]) -> bool:
+    """A mapping of Hamiltonian symbols to expressions (rather than names to numbers)."""
+    expression_types = RuntimeExpression | Parameter | smp.Expr
+    return any(isinstance(k, smp.Symbol) or isinstance(v, expression_types) for k, v in mapping.items())
+
+
+def _symbol_names(expr: smp.Expr) -> set[str]:
+    return {_normalize_symbol_name(str(symbol)) for symbol in expr.free_symbols}
+
+
+def _greedy_dependency_order(names: Sequence[str], needs: Mapping[str, set[str]], known: set[str]) -> list[str]:
+    """Names in sweeps over the pending list, each name placed as soon as every
+    name it needs is known or placed (the slot order of parameters.py:1000-1033)."""
+    placed: list[str] = []
+    resolved = set(known)
+    pending = list(names)
+    while pending:
+        before = len(placed)
+        for name in pending:
+            if needs[name] <= resolved:
+                placed.append(name)
+                resolved.add(name)
+        if len(placed) == before:
+            raise ValueError("could not resolve compound parameter dependency order")
+        pending = [name for name in pending if name not in resolved]
+    return placed
+
+
 class LindbladParameters:
-    """Base values plus compound expressions bound to Hamiltonian symbols."""
+    """Base values plus compound expressions bound to Hamiltonian symbols.
+
+    ``base_parameters`` maps names to stored values (complex, or tuples for
+    table-valued parameters); ``compound_parameters`` maps bound symbols to
+    their expression text, kept in slot order after every change.
+    """
 
     def __init__(self, base_parameters: Mapping[Any, Any] | None = None,
                  compound_parameters: Mapping[Any, Any] | None = None) -> None:
         self.base_parameters: OrderedDict[str, complex | tuple[complex, ...]] = OrderedDict()
         self.compound_parameters: OrderedDict[str, str] = OrderedDict()
-        self._compound_expressions: OrderedDict[str, smp.Expr] = OrderedDict()
-        self._parameters_by_name: dict[str, Parameter] = {}
-        if base_parameters is not None:
-            if compound_parameters is None and self._looks_like_binding_mapping(base_parameters):
-                for symbol, expression in base_parameters.items():
-                    self.bind(symbol, expression, finalize=False)
-            else:
-                for name, value in base_parameters.items():
-                    self._set_base(str(name), value)
-        if compound_parameters is not None:
-            for name, expression in compound_parameters.items():
-                self._set_compound(str(name), expression)
+        self._exprs: OrderedDict[str, smp.Expr] = OrderedDict()
+        self._registry: dict[str, Parameter] = {}
+        if base_parameters is not None and compound_parameters is None and _is_binding_mapping(base_parameters):
+            for symbol, expression in base_parameters.items():
+                self.bind(symbol, expression, finalize=False)
+        else:
+            for name, value in (base_parameters or {}).items():
+                self._store_base(str(name), value)
+            for name, expression in (compound_parameters or {}).items():
+                self._store_compound(str(name), expression)
         self._finalize()
-
-    @staticmethod
-    def _looks_like_binding_mapping(mapping: Mapping[Any, Any]) -> bool:
-        return any(
-            isinstance(key, smp.Symbol) or isinstance(value, RuntimeExpression | Parameter | smp.Expr)
-            for key, value in mapping.items()
-        )
 
     @classmethod
     def from_kwargs(cls, **kwargs: Any) -> "LindbladParameters":
-        if any(isinstance(v, RuntimeExpression | Parameter | smp.Expr) for v in kwargs.values()):
+        """Keyword form: strings are compound expressions, numbers base values;
+        any expression object makes the whole set a binding mapping."""
+        if _is_binding_mapping(kwargs):
             return cls(kwargs)
-        base, compound = OrderedDict(), OrderedDict()
-        for key, value in kwargs.items():
-            (compound if isinstance(value, str) else base)[key] = value
-        return cls(base, compound)
+        texts = OrderedDict((k, v) for k, v in kwargs.items() if isinstance(v, str))
+        values = OrderedDict((k, v) for k, v in kwargs.items() if not isinstance(v, str))
+        return cls(values, texts)
 
     def parameter(self, name: str, default: Any = 0.0, kind: str | None = None) -> Parameter:
-        parameter = Parameter(name, default, _infer_kind(default) if kind is None else kind)
-        self._register_parameter(parameter)
-        return parameter
+        created = Parameter(name, default, _infer_kind(default) if kind is None else kind)
+        self._register_parameter(created)
+        return created
 
     def real(self, name: str, default: Any = 0.0) -> Parameter:
         return self.parameter(name, default, kind="real")
@@ -449,31 +486,36 @@ class LindbladParameters:
 
     def bind(self, symbol: Any, expression: Any, *, overwrite: bool = True,
              finalize: bool = True) -> "LindbladParameters":
-        name = _normalize_symbol_name(str(symbol))
-        runtime = _to_runtime_expression(expression)
-        for parameter in runtime.parameters.values():
-            self._register_parameter(parameter)
-        if name in self.compound_parameters and not overwrite:
-            raise ValueError(f"symbol {name!r} is already bound")
-        if str(runtime.expr) == name and name in self.base_parameters:
-            self.compound_parameters.pop(name, None)
-            self._compound_expressions.pop(name, None)
+        """Bind a Hamiltonian symbol to an expression (registering the
+        expression's parameters); binding a base parameter to itself removes
+        an earlier compound of that name."""
+        target = _normalize_symbol_name(str(symbol))
+        rhs = _to_runtime_expression(expression)
+        for used in rhs.parameters.values():
+            self._register_parameter(used)
+        if not overwrite and target in self.compound_parameters:
+            raise ValueError(f"symbol {target!r} is already bound")
+        is_self_binding = target in self.base_parameters and str(rhs.expr) == target
+        if is_self_binding:
+            self.compound_parameters.pop(target, None)
+            self._exprs.pop(target, None)
         else:
-            self._set_compound(name, runtime.expr)
+            self._store_compound(target, rhs.expr)
         if finalize:
             self._finalize()
         return self
 
     def drive(self, selector: Any, *, rabi: Any, detuning: Any,
               overwrite: bool = True) -> "LindbladParameters":
-        self.bind(selector.Ω, rabi, overwrite=overwrite, finalize=False)
-        self.bind(selector.δ, detuning, overwrite=overwrite, finalize=False)
+        """Bind a transition's Rabi rate and detuning symbols at once."""
+        for symbol, expression in ((selector.Ω, rabi), (selector.δ, detuning)):
+            self.bind(symbol, expression, overwrite=overwrite, finalize=False)
         self._finalize()
         return self
 
     @property
     def all_parameter_names(self) -> list[str]:
-        return list(self.base_parameters) + list(self.compound_parameters)
+        return [*self.base_parameters, *self.compound_parameters]
 
     @property
     def slot_names(self) -> list[str]:
@@ -481,29 +523,27 @@ class LindbladParameters:
 
     @property
     def slot_index_by_name(self) -> dict[str, int]:
-        return {name: idx for idx, name in enumerate(self.slot_names)}
+        return dict(zip(self.slot_names, range(len(self.slot_names))))
 
     @property
     def parsed_compound_parameters(self) -> OrderedDict[str, smp.Expr]:
-        return OrderedDict(self._compound_expressions)
+        return OrderedDict(self._exprs)
 
     def validate_symbols(self, symbols: Sequence[str]) -> None:
-        allowed = set(self.all_parameter_names)
-        missing = sorted({
-            _normalize_symbol_name(str(s)) for s in symbols
-            if _normalize_symbol_name(str(s)) != "t" and _normalize_symbol_name(str(s)) not in allowed
-        })
-        if missing:
-            raise AssertionError(f"Symbol(s) not defined: {', '.join(missing)}")
+        known = set(self.all_parameter_names) | {"t"}
+        unknown = {_normalize_symbol_name(str(s)) for s in symbols} - known
+        if unknown:
+            raise AssertionError(f"Symbol(s) not defined: {', '.join(sorted(unknown))}")
 
     def check_hamiltonian_symbols(self, hamiltonian: smp.Matrix) -> None:
         self.validate_symbols([str(s) for s in hamiltonian.free_symbols])
 
-    def _set_base(self, name: str, value: Any) -> None:
-        self.base_parameters[_normalize_symbol_name(name)] = _coerce_base_value(value)
+    # ---- storage
 
-    def _set_compound(self, name: str, expression: Any) -> None:
-        normalized = _normalize_symbol_name(name)
+    def _store_base(self, name: str, value: Any) -> None:
+        self.base_parameters[_normalize_symbol_name(name)] = _slot_value(value)
+
+    def _store_compound(self, name: str, expression: Any) -> None:
         if isinstance(expression, RuntimeExpression):
             expr = expression.expr
         elif isinstance(expression, smp.Expr):
@@ -514,47 +554,30 @@ class LindbladParameters:
             expr = smp.sympify(expression)
         else:
             raise TypeError(f"unsupported compound expression type {type(expression).__name__}")
-        self.compound_parameters[normalized] = _normalize_symbol_name(str(expr))
-        self._compound_expressions[normalized] = expr
+        key = _normalize_symbol_name(name)
+        self._exprs[key] = expr
+        self.compound_parameters[key] = _normalize_symbol_name(str(expr))
 
     def _register_parameter(self, parameter: Parameter) -> None:
-        existing = self._parameters_by_name.get(parameter.name)
-        if existing is not None and existing is not parameter:
+        """Record a Parameter object as a base slot; one object per name, and
+        one default per name."""
+        if self._registry.setdefault(parameter.name, parameter) is not parameter:
             raise ValueError(f"duplicate Parameter name {parameter.name!r}")
-        self._parameters_by_name[parameter.name] = parameter
-        value = _coerce_base_value(parameter.default)
-        current = self.base_parameters.get(parameter.name)
-        if current is not None and current != value:
+        value = _slot_value(parameter.default)
+        if self.base_parameters.setdefault(parameter.name, value) != value:
             raise ValueError(f"base parameter {parameter.name!r} already has a different default")
-        self.base_parameters[parameter.name] = value
 
     def _finalize(self) -> None:
-        defined = set(self.all_parameter_names) | {"t"}
-        missing = sorted({
-            _normalize_symbol_name(str(s))
-            for expr in self._compound_expressions.values()
-            for s in expr.free_symbols
-            if _normalize_symbol_name(str(s)) not in defined
-        })
-        if missing:
-            raise AssertionError(f"Symbol(s) not defined: {', '.join(missing)}")
-        # Order compounds so every compound follows the compounds it reads.
-        numeric = set(self.base_parameters) | {"t"}
-        unordered = list(self.compound_parameters)
-        ordered: list[str] = []
-        while unordered:
-            progressed = False
-            for name in unordered:
-                deps = {_normalize_symbol_name(str(s))
-                        for s in self._compound_expressions[name].free_symbols}
-                if all(d in numeric or d in ordered for d in deps):
-                    ordered.append(name)
-                    progressed = True
-            if not progressed:
-                raise ValueError("could not resolve compound parameter dependency order")
-            unordered = [name for name in unordered if name not in ordered]
-        self.compound_parameters = OrderedDict((k, self.compound_parameters[k]) for k in ordered)
-        self._compound_expressions = OrderedDict((k, self._compound_expressions[k]) for k in ordered)
+        """Check every compound's symbols are defined, then put the compounds in
+        slot order (each after the compounds it reads)."""
+        known = set(self.base_parameters) | {"t"}
+        needs = {name: _symbol_names(expr) for name, expr in self._exprs.items()}
+        undefined = set().union(*needs.values()) - known - set(self._exprs)
+        if undefined:
+            raise AssertionError(f"Symbol(s) not defined: {', '.join(sorted(undefined))}")
+        order = _greedy_dependency_order(list(self.compound_parameters), needs, known)
+        self.compound_parameters = OrderedDict((name, self.compound_parameters[name]) for name in order)
+        self._exprs = OrderedDict((name, self._exprs[name]) for name in order)
 
 
 def adapt_lindblad_parameters(parameters: Any) -> LindbladParameters:

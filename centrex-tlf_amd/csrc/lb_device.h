@@ -81,7 +81,8 @@ struct DevPlan {
   int32_t n_pub;    // published elements
   int32_t K;        // Hamiltonian coefficients
   int32_t KD;       // coefficient programs: K, plus K * n_diff derivative programs
-  int32_t n_diff;   // differentiated parameters (0 for an ordinary plan)
+  int32_t n_diff;   // derivative columns per coefficient (0 for an ordinary plan)
+  int32_t diff_time; // 1: the last derivative column is d/dt (the others are parameters)
   int32_t n_slots, n_base, n_compounds;
   int32_t max_self; // self terms per slot (ELL width)
   int32_t time_dependent;
@@ -167,18 +168,22 @@ struct DevCall {
   int64_t capacity;        // rows per trajectory in trace mode
   double* out_times;       // device n_traj x capacity, or null
   int32_t* out_counts;     // device n_traj (trace mode)
-  // tape for the adjoint: accepted-step start times and sizes, and the state
-  // at the start of every ckpt_every-th accepted step (slot order, re/im)
+  // tape for the adjoint, one record per step attempt: start time, step size,
+  // (error norm, proposed next step, fac_old), LB_TAPE_* flags, and the state
+  // at the start of every ckpt_every-th attempt (slot order, re/im)
   double* tape_t;          // n_traj x tape_cap
   double* tape_h;          // n_traj x tape_cap
+  double* tape_c;          // n_traj x tape_cap x 3
+  int32_t* tape_f;         // n_traj x tape_cap
   double* tape_y;          // n_traj x ceil(tape_cap / ckpt_every) x S x 2
-  int32_t* tape_n;         // n_traj accepted steps recorded
+  int32_t* tape_n;         // n_traj attempts recorded
   int64_t tape_cap;
   int32_t ckpt_every;
+  int32_t adj_mode;        // LB_ADJOINT_*
   // adjoint workspace (per workgroup) and output
-  double* seg_y;           // workgroups x ckpt_every x 6 x S x 2: stage inputs of a segment
-  double* seg_c;           // workgroups x ckpt_every x 6 x (KD + 1) x 2: coefficient rows
-  double* out_grad;        // n_traj x n_diff: d(integral)/d(parameter)
+  double* seg_y;           // workgroups x ckpt_every x 8 x S x 2: y and k1..k7 of each attempt
+  double* seg_c;           // workgroups x ckpt_every x 7 x (KD + 1) x 2: coefficient rows
+  double* out_grad;        // n_traj x (n_diff - diff_time): d(integral)/d(parameter)
   // replay: take the accepted step sizes of a recorded tape (stride tape_cap)
   // instead of the step-size controller; every step is accepted
   const double* replay_h;

@@ -58,7 +58,8 @@ int fail(int code, const std::string& msg) {
 struct PlanSource {
   int n = 0, U = 0, K = 0, n_base = 0;
   int n_slots = 0;  // slots of the description (the program adds hidden ones after them)
-  int n_diff = 0;  // differentiated parameters (derivative programs follow the K coefficients)
+  int n_diff = 0;  // derivative columns (parameters, then time when diff_time)
+  int diff_time = 0;  // 1: the last derivative column is d/dt
   int64_t T = 0;
   std::vector<int64_t> output_ptrs, input_indices, coeff_indices;
   std::vector<uint8_t> coeff_conj, input_conj, coef_dyn;
@@ -418,6 +419,7 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   P.K = src.K;
   P.KD = src.K * (1 + src.n_diff);
   P.n_diff = src.n_diff;
+  P.diff_time = src.diff_time;
   P.n_slots = src.prog.n_slots;
   P.n_base = src.n_base;
   P.n_compounds = src.prog.n_compounds;
@@ -526,15 +528,18 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   const int n = d->n_states, U = n * (n + 1) / 2;
   if (d->n_base < 0 || d->n_base > d->n_slots) return fail(LB_ERR_INVALID, "n_base exceeds n_slots");
   if (d->n_coefficients > 254) return fail(LB_ERR_UNSUPPORTED, "more than 254 Hamiltonian coefficients");
-  if (d->n_diff_params < 0 || d->n_diff_params > LB_MAX_DIFF)
-    return fail(LB_ERR_UNSUPPORTED, "at most 8 differentiated parameters");
+  if (d->diff_time != 0 && d->diff_time != 1) return fail(LB_ERR_INVALID, "diff_time must be 0 or 1");
+  if (d->diff_time && d->n_diff_params <= 0)
+    return fail(LB_ERR_INVALID, "a time derivative column needs differentiated parameters");
+  if (d->n_diff_params < 0 || d->n_diff_params + d->diff_time > LB_MAX_DIFF)
+    return fail(LB_ERR_UNSUPPORTED, "at most 8 derivative columns (parameters plus time)");
   for (int m = 0; m < d->n_diff_params; ++m)
     if (!d->diff_param_slots || d->diff_param_slots[m] < 0 || d->diff_param_slots[m] >= d->n_base)
       return fail(LB_ERR_INVALID, "differentiated parameters must be base slots");
   // the expression list seen by the program builder: compounds, coefficients,
   // then the derivative programs of a differentiable plan
   lb_plan_desc dd = *d;
-  dd.n_coefficients = d->n_coefficients * (1 + d->n_diff_params);
+  dd.n_coefficients = d->n_coefficients * (1 + d->n_diff_params + d->diff_time);
   const int n_expr = d->n_compounds + dd.n_coefficients;
   for (int c = 0; c < d->n_compounds; ++c)
     if (d->compound_slots[c] < d->n_base || d->compound_slots[c] >= d->n_slots)
@@ -563,7 +568,8 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   src.n = n;
   src.U = U;
   src.K = d->n_coefficients;
-  src.n_diff = d->n_diff_params;
+  src.n_diff = d->n_diff_params + d->diff_time;
+  src.diff_time = d->diff_time;
   src.n_base = d->n_base;
   src.n_slots = d->n_slots;
   src.T = T;
@@ -920,10 +926,14 @@ int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out,
     c.out_counts = a->out_counts;
   }
   if (a->tape_h) {
-    if (!a->tape_t || !a->tape_y || !a->tape_n || a->tape_cap <= 0 || a->ckpt_every <= 0)
-      return fail(LB_ERR_INVALID, "an adjoint tape needs tape_t, tape_y, tape_n, tape_cap and ckpt_every");
+    if (!a->tape_t || !a->tape_c || !a->tape_f || !a->tape_y || !a->tape_n || a->tape_cap <= 0 ||
+        a->ckpt_every <= 0)
+      return fail(LB_ERR_INVALID,
+                  "an adjoint tape needs tape_t, tape_c, tape_f, tape_y, tape_n, tape_cap and ckpt_every");
     c.tape_t = a->tape_t;
     c.tape_h = a->tape_h;
+    c.tape_c = a->tape_c;
+    c.tape_f = a->tape_f;
     c.tape_y = a->tape_y;
     c.tape_n = a->tape_n;
     c.tape_cap = a->tape_cap;
@@ -1194,7 +1204,7 @@ namespace {
 // Must match carve_adj() in lb_kernels.hip.
 int64_t lds_bytes_adj(const DevPlan& P) {
   return 8 * (2 * (int64_t)P.n_pub) + 8 * 2 * (int64_t)P.S + 8 * 2 * (int64_t)P.S +
-         8 * (int64_t)((P.n + 1) & ~1) + 16 * 7 * (int64_t)(P.KD + 1) + 16 * 7 * (int64_t)P.n_slots +
+         8 * (int64_t)((P.n + 1) & ~1) + 16 * 8 * (int64_t)(P.KD + 1) + 16 * 8 * (int64_t)P.n_slots +
          8 * (int64_t)LB_MAX_DIFF * 16 + 16 * 8 * (int64_t)P.vm_depth + 16;
 }
 
@@ -1202,7 +1212,7 @@ int64_t lds_bytes_adj(const DevPlan& P) {
 int64_t lds_bytes_adj_wave(const DevPlan& P, const std::vector<int32_t>& g_rows, int t_rows, int every) {
   auto odd = [](int64_t v) { return v | 1; };
   const int64_t g = g_rows.empty() ? 0 : g_rows[0], KC = P.KD + 1;
-  return g * 64 * 16 + (int64_t)t_rows * 64 * 16 + KC * 16 + (int64_t)every * 6 * odd(KC) * 16 +
+  return g * 64 * 16 + (int64_t)t_rows * 64 * 16 + KC * 16 + (int64_t)every * 7 * odd(KC) * 16 +
          (int64_t)P.n_slots * 16 + 64 * odd(P.n_slots) * 16 + 64 * odd(P.vm_depth) * 16 + 64 * 16 +
          (int64_t)LB_MAX_DIFF * 64 * 16 +
          (int64_t)((P.n + 1) & ~1) * 8 + g * 64 * 4 + (int64_t)t_rows * 64 * 4 + 16;
@@ -1227,6 +1237,12 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   if (rc) return rc;
   c.out_grad = a->out_grad;
   if (a->ckpt_every <= 0) return fail(LB_ERR_INVALID, "ckpt_every must be positive");
+  if (a->adjoint_mode != LB_ADJOINT_ADAPTIVE && a->adjoint_mode != LB_ADJOINT_FROZEN)
+    return fail(LB_ERR_INVALID, "unknown adjoint mode");
+  if (a->adjoint_mode == LB_ADJOINT_ADAPTIVE && !p->src.diff_time)
+    return fail(LB_ERR_INVALID,
+                "the adjoint of the adaptive solve needs the time derivative programs (lb_plan_desc.diff_time = 1)");
+  c.adj_mode = a->adjoint_mode;
   // A plan of one 64-slot block whose gather rows and LDS fit takes the
   // single-wave adjoint; otherwise the multi-wave variant with the fewest
   // slots per thread that covers the plan.  args->variant = k + 1 forces
@@ -1268,9 +1284,10 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
     grid = std::max(1, per_cu) * std::max(1, p->cus);
   }
   grid = (int)std::min<int64_t>(grid, a->n_traj);
-  // workspace: ckpt_every recomputed steps per resident workgroup
-  const size_t need_y = (size_t)grid * a->ckpt_every * 6 * S * 2;
-  const size_t need_c = (size_t)grid * a->ckpt_every * 6 * (im->dev.KD + 1) * 2;
+  // workspace: ckpt_every recomputed attempts per resident workgroup (y and
+  // k1..k7, and the seven coefficient rows of each)
+  const size_t need_y = (size_t)grid * a->ckpt_every * 8 * S * 2;
+  const size_t need_c = (size_t)grid * a->ckpt_every * 7 * (im->dev.KD + 1) * 2;
   {
     std::lock_guard<std::mutex> lock(const_cast<lb_plan*>(p)->mu);
     DevImage* mi = const_cast<DevImage*>(im);

@@ -840,7 +840,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
       bad_support = L.misc[1] != 0;
     }
 
-    int64_t accepted = 0, rejected = 0, rhs_calls = 0;
+    int64_t accepted = 0, rejected = 0, rhs_calls = 0, attempts = 0;
     int status = 0;
     double fail_x = 0.0, integral = 0.0, last_t = __builtin_nan(""), last_v = 0.0;
     // stop event: g at the last accepted point, the root's time once it fired,
@@ -973,6 +973,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
 
       while (!last && (!EV || status == 0)) {
         if (ns > C.maxiters) { status = LB_TRAJ_MAXITERS; fail_x = x; break; }
+        bool clipped = false;
         if (C.replay_h) {
           // recorded schedule: step sizes of a tape, every step accepted
           const int64_t nrep = C.replay_n[traj];
@@ -981,7 +982,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
           last = accepted + 1 == nrep;
         } else {
           if (0.1 * fabs(h) <= DBL_EPSILON * fabs(x)) { status = LB_TRAJ_UNDERFLOW; fail_x = x; break; }
-          if (x + 1.01 * h - t1 > 0.0) { h = t1 - x; last = true; }
+          if (x + 1.01 * h - t1 > 0.0) { h = t1 - x; last = true; clipped = true; }
         }
         ++ns;
         // The expression machine needs registers of its own: the single-wave
@@ -1090,8 +1091,51 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         const double pw = pow(wl < 32 ? err : fac_old, wl < 32 ? alpha : -beta);
         const double fac11 = __shfl(pw, 0, 64), fo = __shfl(pw, 32, 64);
         double fac = fac11 * fo;
-        fac = fmax(facc2, fmin(facc1, fac / safety));
+        const double fac_s = fac / safety;
+        fac = fmax(facc2, fmin(facc1, fac_s));
         double hn = h / fac;
+        if (C.tape_h && !C.replay_h) {
+          // attempt tape: what the adjoint needs to differentiate this attempt
+          // and the controller's decision (one record per attempt)
+          const int64_t a = ns - 1;
+          if (a < C.tape_cap) {
+            if (tid == 0) {
+              int fl = clipped ? LB_TAPE_CLIPPED : 0;
+              double h_next;
+              if (err <= 1.0) {
+                h_next = hn;
+                fl |= LB_TAPE_ACCEPTED | (accepted == 0 ? LB_TAPE_FIRST : 0) | (err > 1.0e-4 ? LB_TAPE_ERRQ : 0);
+                if (!(fac_s > facc2 && fac_s < facc1)) fl |= LB_TAPE_CLAMPED;
+                double hc = hn;
+                if (fabs(hc) > hmax) { hc = hmax; fl |= LB_TAPE_HMAX; }
+                if (reject && fabs(h) < fabs(hc)) fl |= LB_TAPE_MINRULE;
+              } else {
+                const double fr = fac11 / safety;
+                h_next = h / fmin(facc1, fr);
+                if (!(fr < facc1)) fl |= LB_TAPE_CLAMPED;
+              }
+              const int64_t o = traj * C.tape_cap + a;
+              C.tape_t[o] = x;
+              C.tape_h[o] = h;
+              C.tape_c[3 * o] = err;
+              C.tape_c[3 * o + 1] = h_next;
+              C.tape_c[3 * o + 2] = fac_old;
+              C.tape_f[o] = fl;
+            }
+            if (a % C.ckpt_every == 0) {
+              const int64_t ncap = (C.tape_cap + C.ckpt_every - 1) / C.ckpt_every;
+              double* ck = C.tape_y + ((traj * ncap + a / C.ckpt_every) * P.S) * 2;
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const int s = tid + W * r;
+                if (s < P.S) {
+                  ck[2 * s] = y_r[r];
+                  ck[2 * s + 1] = y_i[r];
+                }
+              }
+            }
+          }
+        }
         if (err <= 1.0) {
           fac_old = fmax(err, 1.0e-4);
           if (fabs(hn) > hmax) hn = hmax;
@@ -1112,29 +1156,6 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
           }
           const double xo = x;
           x += h;
-          if (C.tape_h) {
-            // adjoint tape: step start and size, and a checkpoint of the state
-            // at the start of every ckpt_every-th step
-            const int64_t n = accepted - 1;
-            if (n < C.tape_cap) {
-              if (tid == 0) {
-                C.tape_t[traj * C.tape_cap + n] = xo;
-                C.tape_h[traj * C.tape_cap + n] = h;
-              }
-              if (n % C.ckpt_every == 0) {
-                const int64_t ncap = (C.tape_cap + C.ckpt_every - 1) / C.ckpt_every;
-                double* ck = C.tape_y + ((traj * ncap + n / C.ckpt_every) * P.S) * 2;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                  const int s = tid + W * r;
-                  if (s < P.S) {
-                    ck[2 * s] = y_r[r];
-                    ck[2 * s + 1] = y_i[r];
-                  }
-                }
-              }
-            }
-          }
           // ---- terminal stop event (dopri5.rs:280-317, tsit5.rs:274-309): g at
           // the step end (the slots of the last stage are those at x), a root
           // by 60 bisections on the dense output, its state left in yt
@@ -1251,6 +1272,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
         }
         h = hn;
       }
+      attempts = ns;
     }
 #undef LB_RHS
 #undef PUB
@@ -1264,8 +1286,8 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_solve_kernel(const DevPlan P, 
       fail_x = (double)si;  // plan points reached
     }
     if (status == 0 && C.trace && n_pushed > C.capacity) status = LB_TRAJ_CAPACITY;
-    if (status == 0 && C.tape_h && accepted > C.tape_cap) status = LB_TRAJ_CAPACITY;
-    if (C.tape_n && tid == 0) C.tape_n[traj] = (int32_t)accepted;
+    if (status == 0 && C.tape_h && !C.replay_h && attempts > C.tape_cap) status = LB_TRAJ_CAPACITY;
+    if (C.tape_n && tid == 0) C.tape_n[traj] = (int32_t)attempts;
     const bool ok = status == 0;
     const double nanv = __builtin_nan("");
     if (C.trace) {
@@ -2126,30 +2148,113 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
 }
 
 // ---------------------------------------------------------------------------
-// Discrete adjoint of the accepted Dopri5/Tsit5 steps.
+// Discrete adjoint of the adaptive Dopri5/Tsit5 solve.
 //
-// For the weighted trapezoid I over the pushed points (t0 when saving the
-// start, then every accepted step), with the accepted step sizes frozen, the
-// kernel returns dI/dp_m for the plan's differentiated base parameters.  The
-// forward solve recorded a tape (step starts and sizes, and the state at the
-// start of every ckpt_every-th step).  Segments are processed last to first:
-// the segment is recomputed from its checkpoint, keeping the stage inputs
-// Y_i and the coefficient rows (values and parameter derivatives) in the
-// workgroup's workspace, then its steps are swept backwards:
-//   kappa_i = h (b_i lambda + sum_{j>i} a_ji mu_j),   mu_i = A(t_i)^T kappa_i,
-//   dI/dp  += kappa_i^T (dA(t_i)/dp) Y_i,             lambda <- lambda + sum_i mu_i + dI/dy_n,
-// with A^T applied through the plan's transposed term tables and dA/dp through
-// the derivative coefficient programs (the RHS is linear in rho).
+// The objective is the weighted trapezoid I over the pushed points (t0 when
+// saving the start, then every accepted step; rust/src/ode/output.rs:315-330).
+// The forward solve recorded an attempt tape: for every step attempt its
+// start x, size h, error norm, the controller's proposed next step and
+// fac_old, and its flags (accepted, clipped to t1, which controller branch),
+// plus the state at the start of every ckpt_every-th attempt.  Segments of
+// ckpt_every attempts are processed last to first: the segment is recomputed
+// from its checkpoint, keeping each attempt's start state y and its stage
+// derivatives k1..k7 in the workgroup's workspace, then its attempts are
+// swept backwards.  One attempt maps (y, x, h, fac_old) to the candidate
+// yn = y + h sum_j b_j k_j, the error estimate e = h sum_j e_j k_j and, through
+// err = ||e / sc||_rms with sc = atol + rtol max(|y|, |yn|) (dopri5.rs:88-102,
+// tsit5.rs:104-119), to the next proposed step (common.rs:48-69).  With
+// lambda the cotangent of yn and errb that of err, nu = errb de/derr,
+//   kappa_7 = h e_7 nu,  mu_7 = A(x + h)^T kappa_7,  lambda' = lambda + mu_7,
+//   kappa_i = h (b_i lambda' + e_i nu + sum_{j>i} a_ji mu_j),  mu_i = A(t_i)^T kappa_i,
+//   ybar = lambda' + sum_i mu_i (+ the sc terms), dI/dp += sum_i kappa_i^T dA(t_i)/dp Y_i,
+// and the scalar cotangents of x and h collect kappa_i^T dA(t_i)/dt Y_i (time
+// derivatives of the coefficient programs) and the explicit h terms.  The
+// scalar chain -- the controller's partial derivatives, the trapezoid's
+// dependence on h, x_{n+1} = x_n + h, the clip h = t1 - x -- links the
+// attempts.  A^T comes from the plan's transposed term tables and dA/dp,
+// dA/dt from the derivative coefficient programs (the RHS is linear in rho).
+// LB_ADJOINT_FROZEN keeps only the accepted attempts with their step sizes
+// fixed (the derivative of a replay).
 // ---------------------------------------------------------------------------
 namespace {
+
+// d h_next / d (h, err, fac_old) of one attempt (common.rs:48-69): interior
+// branches h_next = h safety fac_old^beta / err^alpha (accepted) or
+// h safety / err^alpha (rejected); a clamped factor leaves h / facc; the h_max
+// cut makes it constant and the after-reject rule makes it h.
+__device__ __forceinline__ void ctrl_partials(int fl, double h, double err, double h_next, double q, double& dh,
+                                              double& de, double& dq) {
+  const double beta = 0.04, alpha = 0.2 - beta * 0.75;
+  const bool acc = (fl & LB_TAPE_ACCEPTED) != 0;
+  dh = h_next / h;
+  de = 0.0;
+  dq = 0.0;
+  if (!(fl & LB_TAPE_CLAMPED)) {
+    de = -alpha * h_next / err;
+    if (acc) dq = beta * h_next / q;
+  }
+  if (acc && (fl & LB_TAPE_MINRULE)) {
+    dh = 1.0;
+    de = 0.0;
+    dq = 0.0;
+  } else if (acc && (fl & LB_TAPE_HMAX)) {
+    dh = 0.0;
+    de = 0.0;
+    dq = 0.0;
+  }
+}
+
+// The error-norm cotangents of one packed component: nu = errb d err / d e,
+// and the scale's term, which goes to the start state or the candidate
+// (whichever max(|y|, |yn|) took).
+__device__ __forceinline__ void err_component(double errb, double err, double dimf, double atol, double rtol,
+                                              double e, double y, double yn, double& nu, double& yb, double& lb) {
+  const double sc = atol + fmax(fabs(y), fabs(yn)) * rtol;
+  const double q = e / sc, den = dimf * err * sc;
+  nu = errb * q / den;
+  const double scb = -errb * q * q / den * rtol;
+  if (fabs(y) >= fabs(yn)) yb += y > 0.0 ? scb : (y < 0.0 ? -scb : 0.0);
+  else lb += yn > 0.0 ? scb : (yn < 0.0 ? -scb : 0.0);
+}
+
+// Per-lane pieces of dI/dh that are inner products with the stage
+// derivatives: lambda'.sum b_j k_j + nu.sum e_j k_j + sum_i mu_i.sum_{j<i} a_ij k_j.
+template <int SOLVER>
+__device__ __forceinline__ double h_products(const double (&k_r)[8][1], const double (&k_i)[8][1], double lpr,
+                                             double lpi, double nur, double nui, const double (&mu_r)[8],
+                                             const double (&mu_i)[8]) {
+  using T = Tab<SOLVER>;
+  double sbr = 0.0, sbi = 0.0, ser = 0.0, sei = 0.0;
+#pragma unroll
+  for (int j = 1; j <= 7; ++j) {
+    if (j <= 6) {
+      sbr += T::a(7, j) * k_r[j][0];
+      sbi += T::a(7, j) * k_i[j][0];
+    }
+    ser += T::e(j) * k_r[j][0];
+    sei += T::e(j) * k_i[j][0];
+  }
+  double v = lpr * sbr + lpi * sbi + nur * ser + nui * sei;
+#pragma unroll
+  for (int i = 2; i <= 6; ++i) {
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int j = 1; j < i; ++j) {
+      sr += T::a(i, j) * k_r[j][0];
+      si += T::a(i, j) * k_i[j][0];
+    }
+    v += mu_r[i] * sr + mu_i[i] * si;
+  }
+  return v;
+}
 
 struct LdsAdj {
   double* pub;    // 2*n_pub: published stage input Y_i (forward publication indices)
   double* kap;    // 2*S: published kappa_i by slot
   double* selfc;  // 2*S folded self coefficients
   double* wts;    // n integral weights per state
-  dcpx* coef;     // 7 rows x (KD + 1): [0] t0, [1..6] stage times
-  dcpx* slots;    // 7 x n_slots
+  dcpx* coef;     // 8 rows x (KD + 1): [0] t0, [1..7] stage times
+  dcpx* slots;    // 8 x n_slots
   double* red;    // LB_MAX_DIFF x 16 per-wave sums
   dcpx* vm;       // 8 lanes x vm_depth
   int* misc;
@@ -2163,8 +2268,8 @@ __device__ __forceinline__ LdsAdj carve_adj(const DevPlan& P) {
   L.selfc = L.kap + 2 * (size_t)P.S;
   L.wts = L.selfc + 2 * (size_t)P.S;
   L.coef = reinterpret_cast<dcpx*>(L.wts + ((P.n + 1) & ~1));
-  L.slots = L.coef + 7 * (P.KD + 1);
-  L.red = reinterpret_cast<double*>(L.slots + 7 * P.n_slots);
+  L.slots = L.coef + 8 * (P.KD + 1);
+  L.red = reinterpret_cast<double*>(L.slots + 8 * P.n_slots);
   L.vm = reinterpret_cast<dcpx*>(L.red + LB_MAX_DIFF * 16);
   L.misc = reinterpret_cast<int*>(L.vm + 8 * P.vm_depth);
   return L;
@@ -2202,20 +2307,21 @@ __device__ __forceinline__ void rhs_adj_fwd(const DevPlan& P, const LdsAdj& L, c
   __syncthreads();
 }
 
-// mu = A(t)^T kappa for the slots of this thread, and the parameter
-// contraction g_m += kappa^T (dA/dp_m) Y over the terms of its outputs.
-// kappa and Y are published before the call (kap by slot, pub by publication
-// index) and a barrier separates the publication from this call.
+// mu = A(t)^T kappa for the slots of this thread, the parameter contraction
+// g_m += kappa^T (dA/dp_m) Y and (plans with a time column) the time
+// contraction tc += kappa^T (dA/dt) Y over the terms of its outputs.  kappa
+// and Y are published before the call (kap by slot, pub by publication index)
+// and a barrier separates the publication from this call.
 template <int R>
 __device__ __forceinline__ void adjoint_stage(const DevPlan& P, const LdsAdj& L, const Slots<R>& S,
                                               const double (&kr)[R], const double (&ki)[R],
                                               const double (&yr)[R], const double (&yi)[R],
                                               const dcpx* cf, double (&mr)[R], double (&mi)[R],
-                                              double (&g)[LB_MAX_DIFF], int tid, int W) {
+                                              double (&g)[LB_MAX_DIFF], double& tc, int tid, int W) {
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nd = P.n_diff;
-  const dcpx* dc = cf + P.K + 1;  // dc[k * nd + m] = d c_k / d p_m
+  const int nd = P.n_diff, np = P.n_diff - P.diff_time;
+  const dcpx* dc = cf + P.K + 1;  // dc[k * nd + m] = d c_k / d p_m (m = np: d c_k / d t)
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     mr[r] = 0.0;
@@ -2226,14 +2332,15 @@ __device__ __forceinline__ void adjoint_stage(const DevPlan& P, const LdsAdj& L,
     const bool diag = S.diag(r);
     // conj(kappa_own)
     const double ckr = kr[r], cki = -ki[r];
-    // self terms: transposed (conj(kappa) selfc) and their coefficient derivatives
+    // self terms: transposed (conj(kappa) selfc) and their parameter derivatives
+    // (their coefficients do not depend on time)
     const double2 a = reinterpret_cast<const double2*>(L.selfc)[s];
     {
       const double hr = ckr * a.x - cki * a.y, hi = ckr * a.y + cki * a.x;
       mr[r] += hr;
       if (!diag) mi[r] -= hi;
     }
-    if (nd > 0) {
+    if (np > 0) {
       for (int j = 0; j < P.max_self; ++j) {
         const size_t e = (size_t)s * P.max_self + j;
         const int id = P.self_coef[e];
@@ -2245,7 +2352,7 @@ __device__ __forceinline__ void adjoint_stage(const DevPlan& P, const LdsAdj& L,
         const dcpx q = cmul(mk(ckr, cki), cmul(f, x));
         const bool cj = P.self_conj[e] != 0;
         for (int m = 0; m < LB_MAX_DIFF; ++m) {
-          if (m >= nd) break;
+          if (m >= np) break;
           const dcpx d = dc[id * nd + m];
           g[m] += q.re * d.re - q.im * (cj ? -d.im : d.im);
         }
@@ -2283,13 +2390,30 @@ __device__ __forceinline__ void adjoint_stage(const DevPlan& P, const LdsAdj& L,
         const dcpx q = cmul(mk(ckr, cki), cmul(mk(f.x, f.y), mk(xr, xi)));
         const bool cj = (w & (1u << 22)) != 0;
         for (int m = 0; m < LB_MAX_DIFF; ++m) {
-          if (m >= nd) break;
+          if (m >= np) break;
           const dcpx d = dc[cid * nd + m];
           g[m] += q.re * d.re - q.im * (cj ? -d.im : d.im);
+        }
+        if (P.diff_time) {
+          const dcpx d = dc[cid * nd + np];
+          tc += q.re * d.re - q.im * (cj ? -d.im : d.im);
         }
       }
     }
   }
+}
+
+// Sum of one value over the workgroup (every thread gets the same value): a
+// wave sum, then the waves in index order.  Synchronises.
+__device__ __forceinline__ double group_sum(double v, double* red, int tid, int W) {
+  const double s = wave_sum(v);
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int b = 0; b < (W >> 6); ++b) t += red[b];
+  __syncthreads();
+  return t;
 }
 
 }  // namespace
@@ -2299,13 +2423,17 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
   using T = Tab<SOLVER>;
   const int tid = threadIdx.x, W = blockDim.x;
   const LdsAdj L = carve_adj(P);
-  const int KC = P.KD + 1;
+  const int KC = P.KD + 1, np = P.n_diff - P.diff_time;
+  const bool adaptive = C.adj_mode == LB_ADJOINT_ADAPTIVE;
+  const double dimf = (double)P.dim;
   Slots<R> S;
   load_slots<R>(P, S, tid, W);
   const int every = C.ckpt_every;
-  double* segY = C.seg_y + (size_t)blockIdx.x * every * 6 * P.S * 2;
-  dcpx* segC = reinterpret_cast<dcpx*>(C.seg_c) + (size_t)blockIdx.x * every * 6 * KC;
+  // workspace per attempt: y and k1..k7 (8 x S complex), 7 coefficient rows
+  double* segY = C.seg_y + (size_t)blockIdx.x * every * 8 * P.S * 2;
+  dcpx* segC = reinterpret_cast<dcpx*>(C.seg_c) + (size_t)blockIdx.x * every * 7 * KC;
   const int64_t ncap = (C.tape_cap + every - 1) / every;
+  auto ws = [&](int a, int j, int s) -> double* { return segY + ((size_t)(a * 8 + j) * P.S + s) * 2; };
 
   for (;;) {
     __syncthreads();
@@ -2335,8 +2463,8 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
     __syncthreads();
     // stage rows start as the t0 row: the expression machine rewrites only the
     // time-dependent entries at each stage time
-    for (int i = tid; i < 6 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
-    for (int i = tid; i < 6 * KC; i += W) L.coef[KC + i] = L.coef[i % KC];
+    for (int i = tid; i < 7 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
+    for (int i = tid; i < 7 * KC; i += W) L.coef[KC + i] = L.coef[i % KC];
     fold_self(P, tid, W, L.coef, L.selfc);
     __syncthreads();
 
@@ -2348,27 +2476,22 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
     if (ok) {
       const double* tt = C.tape_t + traj * C.tape_cap;
       const double* th = C.tape_h + traj * C.tape_cap;
-      // dI/dy at pushed point n: w * (dt_{n-1} [point n-1 pushed] + dt_n [n < N]) / 2,
-      // dt_n the time between points n and n+1
-      auto dt_of = [&](int n) { return (n + 1 < N ? tt[n + 1] : tt[N - 1] + th[N - 1]) - tt[n]; };
-      auto direct = [&](int n) {
-        double d = 0.0;
-        if (n >= 2 || (n == 1 && C.save_start)) d += dt_of(n - 1);
-        if (n < N) d += dt_of(n);
-        return 0.5 * d;
-      };
-      double lam_r[R], lam_i[R];
-      {
-        const double dN = direct(N);
+      const double* tcr = C.tape_c + traj * C.tape_cap * 3;
+      const int32_t* tfl = C.tape_f + traj * C.tape_cap;
+      double wme[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          lam_r[r] = (S.valid(r) && S.diag(r)) ? L.wts[S.state(r)] * dN : 0.0;
-          lam_i[r] = 0.0;
-        }
-      }
+      for (int r = 0; r < R; ++r) wme[r] = (S.valid(r) && S.diag(r)) ? L.wts[S.state(r)] : 0.0;
+      // cotangents: lam of the state at the start of the current step; Tb of
+      // its start time; Hb of the step size proposed into the attempt after
+      // the current one (before its clip); Qb of fac_old after the current
+      // step
+      double lam_r[R], lam_i[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) { lam_r[r] = 0.0; lam_i[r] = 0.0; }
+      double Tb = 0.0, Hb = 0.0, Qb = 0.0;
       const int nseg = (N + every - 1) / every;
       for (int sg = nseg - 1; sg >= 0; --sg) {
-        const int n0 = sg * every, n1 = min(n0 + every, N);
+        const int a0 = sg * every, a1 = min(a0 + every, N);
         // ---- recompute the segment from its checkpoint
         double y_r[R], y_i[R];
         {
@@ -2380,15 +2503,15 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
             y_i[r] = s < P.S ? ck[2 * s + 1] : 0.0;
           }
         }
-        for (int n = n0; n < n1; ++n) {
-          const double x = tt[n], h = th[n];
+        for (int a = a0; a < a1; ++a) {
+          const double x = tt[a], h = th[a];
           __syncthreads();  // previous users of the coefficient rows are done
-          if (tid < 6) {
+          if (tid < 7) {
             const int st = tid + 1;
             const double ts = st == 1 ? x : x + h * T::c(st);
             eval_coefficients(P, L.slots + st * P.n_slots, L.coef + st * KC, ts, true, L.vm + tid * P.vm_depth,
                               P.KD);
-            for (int q = 0; q < KC; ++q) segC[((n - n0) * 6 + tid) * KC + q] = L.coef[st * KC + q];
+            for (int q = 0; q < KC; ++q) segC[((a - a0) * 7 + tid) * KC + q] = L.coef[st * KC + q];
           }
           __syncthreads();
           double k_r[8][R], k_i[8][R], yt_r[R], yt_i[R];
@@ -2396,46 +2519,171 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
           for (int r = 0; r < R; ++r) {
             const int s = tid + W * r;
             if (s < P.S) {
-              segY[(((n - n0) * 6 + 0) * P.S + s) * 2] = y_r[r];
-              segY[(((n - n0) * 6 + 0) * P.S + s) * 2 + 1] = y_i[r];
+              double* w0 = ws(a - a0, 0, s);
+              w0[0] = y_r[r];
+              w0[1] = y_i[r];
             }
           }
           rhs_adj_fwd<R>(P, L, S, y_r, y_i, k_r[1], k_i[1], L.pub, L.coef + KC, tid, W);
+#define LB_ADJ_KEEP(st)                                             \
+  _Pragma("unroll") for (int r = 0; r < R; ++r) {                   \
+    const int s = tid + W * r;                                      \
+    if (s < P.S) {                                                  \
+      double* wk = ws(a - a0, st, s);                               \
+      wk[0] = k_r[st][r];                                           \
+      wk[1] = k_i[st][r];                                           \
+    }                                                               \
+  }
+          LB_ADJ_KEEP(1)
 #define LB_ADJ_FWD(st)                                                                              \
   stage_combo<R, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                                   \
-  _Pragma("unroll") for (int r = 0; r < R; ++r) {                                                  \
-    const int s = tid + W * r;                                                                     \
-    if (s < P.S) {                                                                                 \
-      segY[(((n - n0) * 6 + (st - 1)) * P.S + s) * 2] = yt_r[r];                                   \
-      segY[(((n - n0) * 6 + (st - 1)) * P.S + s) * 2 + 1] = yt_i[r];                               \
-    }                                                                                              \
-  }                                                                                                \
-  rhs_adj_fwd<R>(P, L, S, yt_r, yt_i, k_r[st], k_i[st], L.pub, L.coef + (st) * KC, tid, W);
+  rhs_adj_fwd<R>(P, L, S, yt_r, yt_i, k_r[st], k_i[st], L.pub, L.coef + (st) * KC, tid, W);        \
+  LB_ADJ_KEEP(st)
           LB_ADJ_FWD(2)
           LB_ADJ_FWD(3)
           LB_ADJ_FWD(4)
           LB_ADJ_FWD(5)
           LB_ADJ_FWD(6)
+          LB_ADJ_FWD(7)
 #undef LB_ADJ_FWD
-          stage_combo<R, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);
+#undef LB_ADJ_KEEP
+          if (tfl[a] & LB_TAPE_ACCEPTED) {
+            // stage_combo<7> left the candidate solution in yt
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            y_r[r] = yt_r[r];
-            y_i[r] = yt_i[r];
+            for (int r = 0; r < R; ++r) {
+              y_r[r] = yt_r[r];
+              y_i[r] = yt_i[r];
+            }
           }
         }
-        // ---- adjoint sweep over the segment, last step first
-        for (int n = n1 - 1; n >= n0; --n) {
-          const double h = th[n];
+        // ---- reverse sweep over the segment's attempts, last first
+        for (int a = a1 - 1; a >= a0; --a) {
+          const int fl = tfl[a];
+          const bool acc = (fl & LB_TAPE_ACCEPTED) != 0;
+          if (!adaptive && !acc) continue;
+          const double h = th[a];
+          double hb = 0.0, errb = 0.0, qnew = Qb;
+          if (adaptive) {
+            double dh, de, dq;
+            ctrl_partials(fl, h, tcr[3 * a], tcr[3 * a + 1], tcr[3 * a + 2], dh, de, dq);
+            if (acc) {
+              errb = Hb * de + ((fl & LB_TAPE_ERRQ) ? Qb : 0.0);
+              hb = Hb * dh + Tb;
+              qnew = Hb * dq;
+            } else {
+              errb = Hb * de;
+              hb = Hb * dh;
+            }
+            // a rejected attempt whose proposal nothing downstream depends on
+            if (!acc && errb == 0.0 && hb == 0.0) {
+              if (fl & LB_TAPE_CLIPPED) Hb = 0.0;
+              continue;
+            }
+          }
           __syncthreads();
-          for (int q = tid; q < 6 * KC; q += W) L.coef[KC + q] = segC[(n - n0) * 6 * KC + q];
-          __syncthreads();
-          double mu_r[7][R], mu_i[7][R];
+          for (int q = tid; q < 7 * KC; q += W) L.coef[KC + q] = segC[(a - a0) * 7 * KC + q];
+          double y_r[R], y_i[R], k_r[8][R], k_i[8][R], yn_r[R], yn_i[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int s = tid + W * r;
+            const bool in = s < P.S;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              const double* wk = in ? ws(a - a0, j, s) : nullptr;
+              const double vr = in ? wk[0] : 0.0, vi = in ? wk[1] : 0.0;
+              if (j == 0) { y_r[r] = vr; y_i[r] = vi; } else { k_r[j][r] = vr; k_i[j][r] = vi; }
+            }
+            k_r[0][r] = 0.0;
+            k_i[0][r] = 0.0;
+          }
+          stage_combo<R, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yn_r, yn_i);
+          // lambda (cotangent of the candidate), ybar (of the start state),
+          // nu (of the error estimate), per-lane h and t cotangent pieces
+          double lr[R], li[R], ybr[R], ybi[R], nur[R], nui[R];
+          double hbl = 0.0, tbl = 0.0;
+          const bool trap = acc && (!(fl & LB_TAPE_FIRST) || C.save_start);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            lr[r] = acc ? lam_r[r] : 0.0;
+            li[r] = acc ? lam_i[r] : 0.0;
+            ybr[r] = 0.0;
+            ybi[r] = 0.0;
+            nur[r] = 0.0;
+            nui[r] = 0.0;
+            if (trap) {
+              // 0.5 (v_n + v_{n+1}) h with v = w . y
+              lr[r] += 0.5 * h * wme[r];
+              ybr[r] += 0.5 * h * wme[r];
+              hbl += 0.5 * wme[r] * (y_r[r] + yn_r[r]);
+            }
+          }
+          if (errb != 0.0) {
+            const double err = tcr[3 * a];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              if (!S.valid(r)) continue;
+              double er = T::e(1) * k_r[1][r], ei = T::e(1) * k_i[1][r];
+#pragma unroll
+              for (int j = 2; j <= 7; ++j) {
+                if (T::e(j) != 0.0) {
+                  er = fma(T::e(j), k_r[j][r], er);
+                  ei = fma(T::e(j), k_i[j][r], ei);
+                }
+              }
+              err_component(errb, err, dimf, C.abstol, C.reltol, er * h, y_r[r], yn_r[r], nur[r], ybr[r], lr[r]);
+              if (!S.diag(r))
+                err_component(errb, err, dimf, C.abstol, C.reltol, ei * h, y_i[r], yn_i[r], nui[r], ybi[r], li[r]);
+            }
+          }
+          double mu_r[8][R], mu_i[8][R];
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int r = 0; r < R; ++r) { mu_r[i][r] = 0.0; mu_i[i][r] = 0.0; }
+          // one reverse stage: publish kappa and the stage input, A^T kappa
+#define LB_ADJ_REV(i, KR, KI, YR, YI)                                                              \
+  {                                                                                                \
+    _Pragma("unroll") for (int r = 0; r < R; ++r) {                                                \
+      const int s = tid + W * r;                                                                   \
+      if (S.pub[r] >= 0) {                                                                         \
+        L.pub[2 * S.pub[r]] = YR[r];                                                               \
+        L.pub[2 * S.pub[r] + 1] = YI[r];                                                           \
+      }                                                                                            \
+      if (s < P.S) {                                                                               \
+        L.kap[2 * s] = KR[r];                                                                      \
+        L.kap[2 * s + 1] = KI[r];                                                                  \
+      }                                                                                            \
+    }                                                                                              \
+    __syncthreads();                                                                               \
+    double tci = 0.0;                                                                              \
+    adjoint_stage<R>(P, L, S, KR, KI, YR, YI, L.coef + (i) * KC, mu_r[i], mu_i[i], g, tci, tid, W); \
+    tbl += tci;                                                                                    \
+    hbl += T::c(i) * tci;                                                                          \
+    __syncthreads();                                                                               \
+  }
+          double lpr[R], lpi[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) { lpr[r] = lr[r]; lpi[r] = li[r]; }
+          if (errb != 0.0) {
+            // stage 7: k7 = f(x + h, yn) enters the error estimate only
+            double kr7[R], ki7[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              kr7[r] = h * (T::e(7) * nur[r]);
+              ki7[r] = S.diag(r) ? 0.0 : h * (T::e(7) * nui[r]);
+            }
+            LB_ADJ_REV(7, kr7, ki7, yn_r, yn_i)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              lpr[r] += mu_r[7][r];
+              lpi[r] = S.diag(r) ? 0.0 : lpi[r] + mu_i[7][r];
+            }
+          }
 #define LB_ADJ_STAGE(i)                                                                             \
   {                                                                                                 \
     double kr[R], ki[R], yr[R], yi[R];                                                              \
     _Pragma("unroll") for (int r = 0; r < R; ++r) {                                                 \
-      double ar = T::a(7, i) * lam_r[r], ai = T::a(7, i) * lam_i[r];                                \
+      double ar = T::a(7, i) * lpr[r] + T::e(i) * nur[r], ai = T::a(7, i) * lpi[r] + T::e(i) * nui[r]; \
       _Pragma("unroll") for (int j = i + 1; j <= 6; ++j) {                                          \
         if (T::a(j, i) != 0.0) {                                                                    \
           ar = fma(T::a(j, i), mu_r[j][r], ar);                                                     \
@@ -2444,21 +2692,13 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
       }                                                                                             \
       kr[r] = h * ar;                                                                               \
       ki[r] = S.diag(r) ? 0.0 : h * ai;                                                             \
-      const int s = tid + W * r;                                                                    \
-      yr[r] = s < P.S ? segY[(((n - n0) * 6 + (i - 1)) * P.S + s) * 2] : 0.0;                       \
-      yi[r] = s < P.S ? segY[(((n - n0) * 6 + (i - 1)) * P.S + s) * 2 + 1] : 0.0;                   \
-      if (S.pub[r] >= 0) {                                                                          \
-        L.pub[2 * S.pub[r]] = yr[r];                                                                \
-        L.pub[2 * S.pub[r] + 1] = yi[r];                                                            \
-      }                                                                                             \
-      if (s < P.S) {                                                                                \
-        L.kap[2 * s] = kr[r];                                                                       \
-        L.kap[2 * s + 1] = ki[r];                                                                   \
-      }                                                                                             \
     }                                                                                               \
-    __syncthreads();                                                                                \
-    adjoint_stage<R>(P, L, S, kr, ki, yr, yi, L.coef + (i) * KC, mu_r[i], mu_i[i], g, tid, W);     \
-    __syncthreads();                                                                                \
+    if (i == 1) {                                                                                   \
+      _Pragma("unroll") for (int r = 0; r < R; ++r) { yr[r] = y_r[r]; yi[r] = y_i[r]; }             \
+    } else {                                                                                        \
+      stage_combo<R, SOLVER, (i > 1 ? i : 2)>(h, y_r, y_i, k_r, k_i, yr, yi);                        \
+    }                                                                                               \
+    LB_ADJ_REV(i, kr, ki, yr, yi)                                                                   \
   }
           LB_ADJ_STAGE(6)
           LB_ADJ_STAGE(5)
@@ -2467,36 +2707,63 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
           LB_ADJ_STAGE(2)
           LB_ADJ_STAGE(1)
 #undef LB_ADJ_STAGE
-          const double dn = n >= 1 ? direct(n) : 0.0;
+#undef LB_ADJ_REV
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            double sr = lam_r[r], si = lam_i[r];
+            double sr = lpr[r], si = lpi[r];
 #pragma unroll
             for (int i = 1; i <= 6; ++i) {
               sr += mu_r[i][r];
               si += mu_i[i][r];
             }
-            lam_r[r] = sr + ((S.valid(r) && S.diag(r)) ? L.wts[S.state(r)] * dn : 0.0);
-            lam_i[r] = S.diag(r) ? 0.0 : si;
+            ybr[r] += sr;
+            ybi[r] += si;
           }
+          if (adaptive) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              double kk_r[8][1], kk_i[8][1], mm_r[8], mm_i[8];
+#pragma unroll
+              for (int j = 0; j < 8; ++j) {
+                kk_r[j][0] = k_r[j][r];
+                kk_i[j][0] = k_i[j][r];
+                mm_r[j] = mu_r[j][r];
+                mm_i[j] = mu_i[j][r];
+              }
+              hbl += h_products<SOLVER>(kk_r, kk_i, lpr[r], lpi[r], nur[r], nui[r], mm_r, mm_i);
+            }
+            hb += group_sum(hbl, L.red, tid, W);
+            Tb += group_sum(tbl, L.red, tid, W);
+            if (fl & LB_TAPE_CLIPPED) {
+              Tb -= hb;
+              Hb = 0.0;
+            } else {
+              Hb = hb;
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            lam_r[r] = acc ? ybr[r] : lam_r[r] + ybr[r];
+            lam_i[r] = S.diag(r) ? 0.0 : (acc ? ybi[r] : lam_i[r] + ybi[r]);
+          }
+          if (acc) Qb = qnew;
         }
       }
     }
     // ---- dI/dp: one block reduction per parameter
-    const int nd = P.n_diff;
 #pragma unroll
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
-      if (m >= nd) break;
+      if (m >= np) break;
       const double v = wave_sum(g[m]);
       if ((tid & 63) == 0) L.red[m * 16 + (tid >> 6)] = v;
     }
     __syncthreads();
     if (tid == 0) {
       const int nw = W >> 6;
-      for (int m = 0; m < nd; ++m) {
+      for (int m = 0; m < np; ++m) {
         double tot = 0.0;
         for (int b = 0; b < nw; ++b) tot += L.red[m * 16 + b];
-        C.out_grad[traj * nd + m] = ok ? tot : __builtin_nan("");
+        C.out_grad[traj * np + m] = ok ? tot : __builtin_nan("");
       }
     }
   }
@@ -2505,18 +2772,18 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
 // ---------------------------------------------------------------------------
 // Single-wave adjoint for plans of one 64-slot block (lane = slot).
 //
-// The same discrete adjoint as lb_adjoint_kernel, arranged for latency: the
-// gather and transposed term tables live in LDS for the whole launch; the
-// coefficient rows of a checkpoint segment (every stage time of its steps is
-// on the tape) are evaluated in one pass, one (step, stage) per lane, each
-// lane with a private copy of the parameter slots and its own expression
-// stack; the recomputation uses the register/cross-lane RHS of the
-// single-wave solver (the same arithmetic, so the stage inputs are those of
-// the recorded solve); the reverse stages read kappa of other elements by
-// cross-lane permutes and fold the parameter contraction of every term into
-// the transposed pass.  The self terms of an element (time-independent
-// coefficients) and their parameter derivatives are folded once per
-// trajectory.
+// The same adjoint as lb_adjoint_kernel, arranged for latency: the gather and
+// transposed term tables live in LDS for the whole launch; the coefficient
+// rows of a checkpoint segment (every stage time of its attempts is on the
+// tape) are evaluated in one pass, one (attempt, stage) per lane, each lane
+// with a private copy of the parameter slots and its own expression stack;
+// the recomputation uses the register/cross-lane RHS of the single-wave
+// solver (the same arithmetic, so the stage derivatives are those of the
+// recorded solve); the reverse stages read kappa of other elements by
+// cross-lane permutes and fold the parameter and time contractions of every
+// term into the transposed pass.  The self terms of an element
+// (time-independent coefficients) and their parameter derivatives are folded
+// once per trajectory.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -2524,7 +2791,7 @@ struct LdsAW {
   double2* gf;     // g_rows x 64 gather factors
   double2* tf;     // t_rows x 64 transposed factors
   dcpx* coef0;     // KC: coefficient row at t0 (time-independent entries)
-  dcpx* rows;      // every * 6 rows of stride KCP: coefficient rows of a segment
+  dcpx* rows;      // every * 7 rows of stride KCP: coefficient rows of a segment
   dcpx* slots0;    // n_slots: parameter slots at t0
   dcpx* lslots;    // 64 x SSP: per-lane parameter slots
   dcpx* lvm;       // 64 x VSP: per-lane expression stacks
@@ -2547,7 +2814,7 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
   L.gf = reinterpret_cast<double2*>(q); q += (size_t)g * 64 * 16;
   L.tf = reinterpret_cast<double2*>(q); q += (size_t)t * 64 * 16;
   L.coef0 = reinterpret_cast<dcpx*>(q); q += (size_t)KC * 16;
-  L.rows = reinterpret_cast<dcpx*>(q); q += (size_t)every * 6 * odd_up(KC) * 16;
+  L.rows = reinterpret_cast<dcpx*>(q); q += (size_t)every * 7 * odd_up(KC) * 16;
   L.slots0 = reinterpret_cast<dcpx*>(q); q += (size_t)P.n_slots * 16;
   L.lslots = reinterpret_cast<dcpx*>(q); q += (size_t)64 * odd_up(P.n_slots) * 16;
   L.lvm = reinterpret_cast<dcpx*>(q); q += (size_t)64 * odd_up(P.vm_depth) * 16;
@@ -2563,17 +2830,18 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
 // Coefficients the single-wave adjoint accumulates per lane in registers.
 #define LB_AW_MAX_COEF 8
 
-// One reverse stage: mu = A(t_i)^T kappa for this lane's element and
-// g_m += Re(conj(kappa)^T (dA(t_i)/dp_m) Y_i).  `cf` is the stage's
-// coefficient row (values, the constant 1 at K, then d c_k / d p_m at
-// K + 1 + k * nd + m).  The terms' parameter contractions are first summed
-// per coefficient (Q_k, registers; K <= LB_AW_MAX_COEF), then contracted with
-// the coefficient derivatives once per stage.
+// One reverse stage: mu = A(t_i)^T kappa for this lane's element,
+// g_m += Re(conj(kappa)^T (dA(t_i)/dp_m) Y_i) and (a plan with a time column)
+// tc = Re(conj(kappa)^T (dA(t_i)/dt) Y_i).  `cf` is the stage's coefficient
+// row (values, the constant 1 at K, then d c_k / d p_m at K + 1 + k * nd + m,
+// the time column last).  The terms' contractions are first summed per
+// coefficient (Q_k, registers; K <= LB_AW_MAX_COEF), then contracted with the
+// coefficient derivatives once per stage.
 __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW& L, int trows, int lane,
                                                    double kr, double ki, double yr, double yi, bool diag,
                                                    double2 sv, const dcpx* cf, double& mr, double& mi,
-                                                   double (&g)[LB_MAX_DIFF]) {
-  const int nd = P.n_diff;
+                                                   double (&g)[LB_MAX_DIFF], double& tc) {
+  const int nd = P.n_diff, np = P.n_diff - P.diff_time;
   const dcpx* dc = cf + P.K + 1;
   const double ckr = kr, cki = -ki;  // conj(kappa)
   mr = ckr * sv.x - cki * sv.y;
@@ -2582,7 +2850,7 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
     const dcpx cx = cmul(mk(ckr, cki), mk(yr, diag ? 0.0 : yi));
 #pragma unroll
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
-      if (m >= nd) break;
+      if (m >= np) break;
       const dcpx sd = L.selfd[m * 64 + lane];
       g[m] += cx.re * sd.re - cx.im * sd.im;
     }
@@ -2641,9 +2909,13 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
     if (k >= P.K) break;
 #pragma unroll
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
-      if (m >= nd) break;
+      if (m >= np) break;
       const dcpx d = dc[k * nd + m];
       g[m] += Q[k].re * d.re - Q[k].im * d.im;
+    }
+    if (P.diff_time) {
+      const dcpx d = dc[k * nd + np];
+      tc += Q[k].re * d.re - Q[k].im * d.im;
     }
   }
 }
@@ -2656,11 +2928,13 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
   const int lane = threadIdx.x;
   const int every = C.ckpt_every;
   const LdsAW L = carve_aw(P, every);
-  const int KC = P.KD + 1, KCP = odd_up(KC), nd = P.n_diff;
+  const int KC = P.KD + 1, KCP = odd_up(KC), nd = P.n_diff, np = P.n_diff - P.diff_time;
+  const bool adaptive = C.adj_mode == LB_ADJOINT_ADAPTIVE;
+  const double dimf = (double)P.dim;
   const int grows = cst(P.blk_rows)[0], trows = cst(P.t_blk_rows)[0];
   Slots<1> S;
   load_slots<1>(P, S, lane, 64);
-  const bool diag = S.diag(0);
+  const bool diag = S.diag(0), valid = S.valid(0);
   for (int i = lane; i < grows * 64; i += 64) {
     L.gf[i] = reinterpret_cast<const double2*>(P.g_f)[i];
     L.gw[i] = P.g_word[i];
@@ -2674,7 +2948,8 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
   LV.gf = L.gf;
   dcpx* const myslots = L.lslots + lane * odd_up(P.n_slots);
   dcpx* const myvm = L.lvm + lane * odd_up(P.vm_depth);
-  double* const segY = C.seg_y + (size_t)blockIdx.x * every * 6 * 64 * 2;
+  // workspace per attempt: y and k1..k7 of this lane's element
+  double2* const segW = reinterpret_cast<double2*>(C.seg_y) + (size_t)blockIdx.x * every * 8 * 64;
   const int64_t ncap = (C.tape_cap + every - 1) / every;
 
   for (;;) {
@@ -2706,7 +2981,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
     for (int i = 0; i < P.n_slots; ++i) myslots[i] = L.slots0[i];
     fold_self(P, lane, 64, L.coef0, reinterpret_cast<double*>(L.selfc));
     // parameter derivatives of the folded self coefficient
-    for (int m = 0; m < nd; ++m) {
+    for (int m = 0; m < np; ++m) {
       dcpx acc = mk(0.0, 0.0);
       for (int j = 0; j < P.max_self; ++j) {
         const size_t e = (size_t)lane * P.max_self + j;
@@ -2730,77 +3005,155 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
     if (ok) {
       const double* tt = C.tape_t + traj * C.tape_cap;
       const double* th = C.tape_h + traj * C.tape_cap;
-      auto dt_of = [&](int n) { return (n + 1 < N ? tt[n + 1] : tt[N - 1] + th[N - 1]) - tt[n]; };
-      auto direct = [&](int n) {
-        double d = 0.0;
-        if (n >= 2 || (n == 1 && C.save_start)) d += dt_of(n - 1);
-        if (n < N) d += dt_of(n);
-        return 0.5 * d;
-      };
-      const double wme = (S.valid(0) && diag) ? L.wts[S.state(0)] : 0.0;
-      double lam_r = wme * direct(N), lam_i = 0.0;
+      const double* tcr = C.tape_c + traj * C.tape_cap * 3;
+      const int32_t* tfl = C.tape_f + traj * C.tape_cap;
+      const double wme = (valid && diag) ? L.wts[S.state(0)] : 0.0;
+      // cotangents: lam of the state at the start of the current step; Tb of
+      // its start time; Hb of the step size proposed into the attempt after
+      // the current one (before its clip); Qb of fac_old after the current step
+      double lam_r = 0.0, lam_i = 0.0, Tb = 0.0, Hb = 0.0, Qb = 0.0;
       const int nseg = (N + every - 1) / every;
       for (int sg = nseg - 1; sg >= 0; --sg) {
-        const int n0 = sg * every, n1 = min(n0 + every, N), ntask = 6 * (n1 - n0);
-        // ---- coefficient rows of the segment: task q = step (q / 6), stage (q % 6 + 1)
+        const int a0 = sg * every, a1 = min(a0 + every, N), ntask = 7 * (a1 - a0);
+        // ---- coefficient rows of the segment: task q = attempt (q / 7), stage (q % 7 + 1)
         __syncthreads();  // the previous segment's rows are no longer read
         for (int q = lane; q < ntask; q += 64) {
-          const int n = n0 + q / 6, st = q % 6 + 1;
-          const double ts = st == 1 ? tt[n] : tt[n] + th[n] * T::c(st);
+          const int a = a0 + q / 7, st = q % 7 + 1;
+          const double ts = st == 1 ? tt[a] : tt[a] + th[a] * T::c(st);
           dcpx* row = L.rows + q * KCP;
           for (int j = 0; j < KC; ++j) row[j] = L.coef0[j];
           if (P.time_dependent) eval_coefficients(P, myslots, row, ts, true, myvm, P.KD);
         }
         __syncthreads();
-        // ---- recompute the segment from its checkpoint, storing the stage inputs
+        // ---- recompute the segment from its checkpoint, keeping y and k1..k7
         double y_r[1], y_i[1];
         {
           const double2 ck = reinterpret_cast<const double2*>(C.tape_y + ((traj * ncap + sg) * P.S) * 2)[lane];
           y_r[0] = ck.x;
           y_i[0] = ck.y;
         }
-        for (int n = n0; n < n1; ++n) {
-          const double h = th[n];
-          const dcpx* rw = L.rows + (n - n0) * 6 * KCP;
-          double2* ys = reinterpret_cast<double2*>(segY) + (size_t)(n - n0) * 6 * 64 + lane;
+        for (int a = a0; a < a1; ++a) {
+          const double h = th[a];
+          const dcpx* rw = L.rows + (a - a0) * 7 * KCP;
+          double2* wk = segW + (size_t)(a - a0) * 8 * 64 + lane;
           double k_r[8][1], k_i[8][1], yt_r[1], yt_i[1];
-          ys[0] = double2{y_r[0], y_i[0]};
+          wk[0] = double2{y_r[0], y_i[0]};
           rhs_wave<G>(LV, grows, lane, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], rw);
-#define LB_AW_FWD(st)                                                                      \
-  stage_combo<1, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                           \
-  ys[(st - 1) * 64] = double2{yt_r[0], yt_i[0]};                                           \
-  rhs_wave<G>(LV, grows, lane, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0], rw + (st - 1) * KCP);
+          wk[64] = double2{k_r[1][0], k_i[1][0]};
+#define LB_AW_FWD(st)                                                                           \
+  stage_combo<1, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                                \
+  rhs_wave<G>(LV, grows, lane, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0], rw + (st - 1) * KCP); \
+  wk[(st) * 64] = double2{k_r[st][0], k_i[st][0]};
           LB_AW_FWD(2)
           LB_AW_FWD(3)
           LB_AW_FWD(4)
           LB_AW_FWD(5)
           LB_AW_FWD(6)
+          LB_AW_FWD(7)
 #undef LB_AW_FWD
-          stage_combo<1, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);
-          y_r[0] = yt_r[0];
-          y_i[0] = yt_i[0];
+          if (tfl[a] & LB_TAPE_ACCEPTED) {
+            // stage_combo<7> left the candidate solution in yt
+            y_r[0] = yt_r[0];
+            y_i[0] = yt_i[0];
+          }
         }
-        // ---- adjoint sweep over the segment, last step first
-        for (int n = n1 - 1; n >= n0; --n) {
-          const double h = th[n];
-          const dcpx* rw = L.rows + (n - n0) * 6 * KCP;
-          const double2* ys = reinterpret_cast<const double2*>(segY) + (size_t)(n - n0) * 6 * 64 + lane;
-          // stage inputs, each loaded one stage ahead of its use
-          double2 ycur = ys[5 * 64], ynext;
-          double mu_r[7], mu_i[7];
-#define LB_AW_STAGE(i)                                                                       \
-  {                                                                                          \
-    double ar = T::a(7, i) * lam_r, ai = T::a(7, i) * lam_i;                                 \
-    _Pragma("unroll") for (int j = i + 1; j <= 6; ++j) {                                     \
-      if (T::a(j, i) != 0.0) {                                                               \
-        ar = fma(T::a(j, i), mu_r[j], ar);                                                   \
-        ai = fma(T::a(j, i), mu_i[j], ai);                                                   \
-      }                                                                                      \
-    }                                                                                        \
-    if (i > 1) ynext = ys[(i - 2) * 64];                                                     \
-    adjoint_wave_stage(P, L, trows, lane, h * ar, diag ? 0.0 : h * ai, ycur.x, ycur.y, diag, \
-                       selfv, rw + (i - 1) * KCP, mu_r[i], mu_i[i], g);                      \
-    if (i > 1) ycur = ynext;                                                                 \
+        // ---- reverse sweep over the segment's attempts, last first
+        for (int a = a1 - 1; a >= a0; --a) {
+          const int fl = tfl[a];
+          const bool acc = (fl & LB_TAPE_ACCEPTED) != 0;
+          if (!adaptive && !acc) continue;
+          const double h = th[a];
+          double hb = 0.0, errb = 0.0, qnew = Qb;
+          if (adaptive) {
+            double dh, de, dq;
+            ctrl_partials(fl, h, tcr[3 * a], tcr[3 * a + 1], tcr[3 * a + 2], dh, de, dq);
+            if (acc) {
+              errb = Hb * de + ((fl & LB_TAPE_ERRQ) ? Qb : 0.0);
+              hb = Hb * dh + Tb;
+              qnew = Hb * dq;
+            } else {
+              errb = Hb * de;
+              hb = Hb * dh;
+            }
+            // a rejected attempt whose proposal nothing downstream depends on
+            if (!acc && errb == 0.0 && hb == 0.0) {
+              if (fl & LB_TAPE_CLIPPED) Hb = 0.0;
+              continue;
+            }
+          }
+          const dcpx* rw = L.rows + (a - a0) * 7 * KCP;
+          const double2* wk = segW + (size_t)(a - a0) * 8 * 64 + lane;
+          double y_r[1], y_i[1], k_r[8][1], k_i[8][1], yn_r[1], yn_i[1];
+          {
+            const double2 v = wk[0];
+            y_r[0] = v.x;
+            y_i[0] = v.y;
+          }
+#pragma unroll
+          for (int j = 1; j <= 7; ++j) {
+            const double2 v = wk[j * 64];
+            k_r[j][0] = v.x;
+            k_i[j][0] = v.y;
+          }
+          k_r[0][0] = 0.0;
+          k_i[0][0] = 0.0;
+          stage_combo<1, SOLVER, 7>(h, y_r, y_i, k_r, k_i, yn_r, yn_i);
+          double lr = acc ? lam_r : 0.0, li = acc ? lam_i : 0.0;
+          double ybr = 0.0, ybi = 0.0, nur = 0.0, nui = 0.0, hbl = 0.0, tbl = 0.0;
+          if (acc && (!(fl & LB_TAPE_FIRST) || C.save_start)) {
+            // 0.5 (v_n + v_{n+1}) h with v = w . y
+            lr += 0.5 * h * wme;
+            ybr += 0.5 * h * wme;
+            hbl += 0.5 * wme * (y_r[0] + yn_r[0]);
+          }
+          if (errb != 0.0 && valid) {
+            const double err = tcr[3 * a];
+            double er = T::e(1) * k_r[1][0], ei = T::e(1) * k_i[1][0];
+#pragma unroll
+            for (int j = 2; j <= 7; ++j) {
+              if (T::e(j) != 0.0) {
+                er = fma(T::e(j), k_r[j][0], er);
+                ei = fma(T::e(j), k_i[j][0], ei);
+              }
+            }
+            err_component(errb, err, dimf, C.abstol, C.reltol, er * h, y_r[0], yn_r[0], nur, ybr, lr);
+            if (!diag) err_component(errb, err, dimf, C.abstol, C.reltol, ei * h, y_i[0], yn_i[0], nui, ybi, li);
+          }
+          double mu_r[8], mu_i[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) { mu_r[i] = 0.0; mu_i[i] = 0.0; }
+          double lpr = lr, lpi = li;
+          if (errb != 0.0) {
+            // stage 7: k7 = f(x + h, yn) enters the error estimate only
+            double tc7 = 0.0;
+            adjoint_wave_stage(P, L, trows, lane, h * (T::e(7) * nur), diag ? 0.0 : h * (T::e(7) * nui), yn_r[0],
+                               yn_i[0], diag, selfv, rw + 6 * KCP, mu_r[7], mu_i[7], g, tc7);
+            tbl += tc7;
+            hbl += tc7;
+            lpr += mu_r[7];
+            lpi = diag ? 0.0 : lpi + mu_i[7];
+          }
+#define LB_AW_STAGE(i)                                                                                  \
+  {                                                                                                     \
+    double ar = T::a(7, i) * lpr + T::e(i) * nur, ai = T::a(7, i) * lpi + T::e(i) * nui;                \
+    _Pragma("unroll") for (int j = i + 1; j <= 6; ++j) {                                                \
+      if (T::a(j, i) != 0.0) {                                                                          \
+        ar = fma(T::a(j, i), mu_r[j], ar);                                                              \
+        ai = fma(T::a(j, i), mu_i[j], ai);                                                              \
+      }                                                                                                 \
+    }                                                                                                   \
+    double ys_r[1], ys_i[1];                                                                            \
+    if (i == 1) {                                                                                       \
+      ys_r[0] = y_r[0];                                                                                 \
+      ys_i[0] = y_i[0];                                                                                 \
+    } else {                                                                                            \
+      stage_combo<1, SOLVER, (i > 1 ? i : 2)>(h, y_r, y_i, k_r, k_i, ys_r, ys_i);                        \
+    }                                                                                                   \
+    double tci = 0.0;                                                                                   \
+    adjoint_wave_stage(P, L, trows, lane, h * ar, diag ? 0.0 : h * ai, ys_r[0], ys_i[0], diag, selfv,   \
+                       rw + (i - 1) * KCP, mu_r[i], mu_i[i], g, tci);                                   \
+    tbl += tci;                                                                                         \
+    hbl += T::c(i) * tci;                                                                               \
   }
           LB_AW_STAGE(6)
           LB_AW_STAGE(5)
@@ -2809,23 +3162,37 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
           LB_AW_STAGE(2)
           LB_AW_STAGE(1)
 #undef LB_AW_STAGE
-          double sr = lam_r, si = lam_i;
+          double sr = lpr, si = lpi;
 #pragma unroll
           for (int i = 1; i <= 6; ++i) {
             sr += mu_r[i];
             si += mu_i[i];
           }
-          lam_r = sr + (n >= 1 ? wme * direct(n) : 0.0);
-          lam_i = diag ? 0.0 : si;
+          ybr += sr;
+          ybi += si;
+          if (adaptive) {
+            hbl += h_products<SOLVER>(k_r, k_i, lpr, lpi, nur, nui, mu_r, mu_i);
+            hb += wave_sum(hbl);
+            Tb += wave_sum(tbl);
+            if (fl & LB_TAPE_CLIPPED) {
+              Tb -= hb;
+              Hb = 0.0;
+            } else {
+              Hb = hb;
+            }
+          }
+          lam_r = acc ? ybr : lam_r + ybr;
+          lam_i = diag ? 0.0 : (acc ? ybi : lam_i + ybi);
+          if (acc) Qb = qnew;
         }
       }
     }
     // ---- dI/dp: one wave sum per parameter
 #pragma unroll
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
-      if (m >= nd) break;
+      if (m >= np) break;
       const double v = wave_sum(g[m]);
-      if (lane == 0) C.out_grad[traj * nd + m] = ok ? v : __builtin_nan("");
+      if (lane == 0) C.out_grad[traj * np + m] = ok ? v : __builtin_nan("");
     }
   }
 }

@@ -119,6 +119,12 @@ typedef struct lb_plan_desc {
    * derivative compounds they read.  0 for an ordinary plan. */
   int32_t n_diff_params;
   const int32_t* diff_param_slots;
+  /* 1: the derivative programs carry one more column, d c_k / d t (the
+   * programs are then K * (n_diff_params + 1), k * (n_diff_params + 1) +
+   * n_diff_params being the time derivative), which lb_adjoint needs to
+   * differentiate the step-size controller (LB_ADJOINT_ADAPTIVE).  0 for an
+   * ordinary or frozen-step plan. */
+  int32_t diff_time;
 } lb_plan_desc;
 
 typedef struct lb_plan lb_plan;
@@ -252,12 +258,18 @@ typedef struct lb_solve_args {
                                      support through the RHS program (all others
                                      stay exactly zero) and reports
                                      LB_TRAJ_BAD_SUPPORT for a y0 outside it. */
-  /* Adjoint tape: lb_solve records it when tape_h is not NULL (the start time
-   * and size of every accepted step, and the state at the start of every
-   * ckpt_every-th step); lb_adjoint reads it.  All device arrays:
-   * tape_t, tape_h n_traj x tape_cap; tape_y n_traj x ceil(tape_cap /
-   * ckpt_every) x slots x 2 (lb_plan_support_info field 9 gives the slots);
-   * tape_n n_traj.  More accepted steps than tape_cap: LB_TRAJ_CAPACITY. */
+  /* Adjoint tape: lb_solve records it when tape_h is not NULL; lb_adjoint
+   * reads it.  One record per step ATTEMPT (accepted or rejected), in order:
+   * tape_t its start time, tape_h its step size (after the last-step clip),
+   * tape_c (err, h_next, fac_old): the error norm, the step size the
+   * controller proposed next (before the h_max and after-reject rules) and
+   * the controller's fac_old when the attempt started (common.rs:48-69),
+   * tape_f its LB_TAPE_* flags; tape_y holds the state at the start of every
+   * ckpt_every-th attempt.  All device arrays: tape_t, tape_h, tape_f
+   * n_traj x tape_cap; tape_c n_traj x tape_cap x 3; tape_y n_traj x
+   * ceil(tape_cap / ckpt_every) x slots x 2 (lb_plan_support_info field 9
+   * gives the slots); tape_n n_traj (attempts recorded).  More attempts than
+   * tape_cap: LB_TRAJ_CAPACITY. */
   double* tape_t;
   double* tape_h;
   double* tape_y;
@@ -295,7 +307,26 @@ typedef struct lb_solve_args {
   const double* event_im;
   double* out_event_t;            /* device, n_traj, or NULL: the root's time, NaN when the
                                      event did not fire */
+  /* the rest of the attempt tape (see tape_t above) */
+  double* tape_c;
+  int32_t* tape_f;
+  int32_t adjoint_mode;           /* lb_adjoint: LB_ADJOINT_ADAPTIVE or LB_ADJOINT_FROZEN */
 } lb_solve_args;
+
+/* Flags of an attempt-tape record (lb_solve_args.tape_f). */
+#define LB_TAPE_ACCEPTED 1 /* the attempt was accepted (err <= 1)                        */
+#define LB_TAPE_CLIPPED 2  /* h was clipped to t1 - t (dopri5.rs:232-235)                 */
+#define LB_TAPE_CLAMPED 4  /* the controller's factor hit facc1 or facc2 (no err/fac_old dependence) */
+#define LB_TAPE_HMAX 8     /* accepted: the next step was cut to h_max                   */
+#define LB_TAPE_MINRULE 16 /* accepted after a reject: the next step is |h| (common.rs:59-61) */
+#define LB_TAPE_ERRQ 32    /* accepted with err > 1e-4: fac_old takes err                */
+#define LB_TAPE_FIRST 64   /* the first accepted step                                    */
+
+/* lb_adjoint modes. */
+#define LB_ADJOINT_ADAPTIVE 0 /* the derivative of the adaptive solve: the accepted and rejected
+                                 attempts, the error norm and the step-size controller are
+                                 differentiated (needs a plan with diff_time = 1) */
+#define LB_ADJOINT_FROZEN 1   /* the accepted step sizes held fixed: the derivative of a replay */
 
 /*
  * Integrate a batch of trajectories.  Replaces solve_lindblad_batch_ode_py
@@ -310,12 +341,17 @@ int lb_solve(const lb_plan* plan, const lb_solve_args* args, void* stream);
  * Discrete adjoint of a recorded solve (no reference counterpart: the
  * reference has no gradients).  For the weighted trapezoid integral of each
  * trajectory (output_kind LB_OUT_INTEGRAL, pushes at t0 when save_start and
- * at every accepted step, no saveat), with the accepted step sizes of the
- * tape held fixed, writes out_grad[b, m] = d I_b / d p_m for the plan's
- * differentiated parameters (lb_plan_desc.n_diff_params).  `args` must
- * describe the same trajectories, parameters, y0 support and tape as the
- * lb_solve that recorded the tape.  The library keeps a per-plan workspace
- * of ckpt_every recomputed steps per resident workgroup.
+ * at every accepted step, no saveat), writes out_grad[b, m] = d I_b / d p_m
+ * for the plan's differentiated parameters (lb_plan_desc.n_diff_params).
+ * With adjoint_mode LB_ADJOINT_ADAPTIVE that is the derivative of the
+ * adaptive solve itself: every attempt of the tape, the error norm
+ * (dopri5.rs:88-102, tsit5.rs:104-119), the controller (common.rs:48-69),
+ * the last-step clip and the trapezoid's step lengths (output.rs:315-330)
+ * are differentiated.  LB_ADJOINT_FROZEN holds the accepted step sizes
+ * fixed.  `args` must describe the same trajectories, parameters, y0
+ * support and tape as the lb_solve that recorded the tape.  The library
+ * keeps a per-plan workspace of ckpt_every recomputed attempts per resident
+ * workgroup.
  */
 int lb_adjoint(const lb_plan* plan, const lb_solve_args* args, void* stream);
 

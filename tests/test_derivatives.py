@@ -65,3 +65,30 @@ def test_derivative_rules_reject_unsupported():
     prepared = prepare_lindblad_problem(system, params, backend="host")
     with pytest.raises(ValueError, match="base parameter"):
         differentiate_flat_plan(prepared.flat_plan, [int(prepared.flat_plan["n_slots"]) - 1])
+
+
+@pytest.mark.parametrize("builder", [tlf_n39_xy, tlf_n39, tlf_n72])
+def test_time_derivative_programs_match_central_differences(builder):
+    """The time column (with_time=True): d c_k / d t against central differences
+    in t, with the parameter columns unchanged."""
+    system, params, _, t_span, _ = builder()
+    prepared = prepare_lindblad_problem(system, params, backend="host")
+    flat = prepared.flat_plan
+    slots = [prepared.slot_names.index(n) for n in ("delta0", "Omega0")]
+    dflat = differentiate_flat_plan(flat, slots, with_time=True)
+    plain = differentiate_flat_plan(flat, slots)
+    assert dflat["diff_time"] == 1 and plain["diff_time"] == 0
+    K, P = int(flat["n_coefficients"]), len(slots)
+    base = np.asarray(flat["base_re"]) + 1j * np.asarray(flat["base_im"])
+    rng = np.random.default_rng(9)
+    for t in rng.uniform(*t_span, size=6):
+        got = _evaluate(dflat, base, t)
+        assert got.size == K + K * (P + 1)
+        ref = _evaluate(plain, base, t)
+        for m in range(P):
+            np.testing.assert_array_equal(got[K + np.arange(K) * (P + 1) + m], ref[K + np.arange(K) * P + m])
+        ht = 1e-6 * (t_span[1] - t_span[0])
+        fd = (_evaluate(flat, base, t + ht) - _evaluate(flat, base, t - ht)) / (2 * ht)
+        an = got[K + np.arange(K) * (P + 1) + P]
+        scale = np.abs(fd).max() + np.abs(an).max() + 1e-300
+        np.testing.assert_allclose(an, fd, rtol=0, atol=1e-6 * scale, err_msg=f"t={t}")

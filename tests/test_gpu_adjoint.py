@@ -1,20 +1,25 @@
 """The discrete adjoint on the GPU: gradients of photon counts with respect to
 detuning, Rabi rate, polarisation angle and pulse width on the n=39 system
-with x/y polarisation.
+with x/y polarisation (the gradient workload: delta ~ U[-3, 3] Gamma,
+Omega0 ~ U[0.3, 3] Gamma, theta in (0, pi), sigma ~ U[2, 6] / Gamma).
 
-The adjoint differentiates the discrete solution with the accepted step sizes
-held fixed.  Its exact check is therefore against central differences of the
-same discrete map: a replay of the recorded step sizes (``replay``) at
-perturbed parameters.  Against differences of the adaptive solve (GPU forward
-and CPU oracle) the two derivatives differ by the step-size controller's own
-sensitivity, which shrinks with the tolerance; those checks run at rtol 1e-12.
+Two derivatives are checked:
 
-Errors are compared as logarithmic sensitivities p_m dI/dp_m, relative to the
-largest photon count of the batch, so that a parameter whose derivative
-vanishes (theta for a z-symmetric initial state) is held to the same absolute
-standard as the others.  The initial state is a coherent superposition of the
-ground states, which breaks the symmetry and makes the theta derivative
-non-zero.
+- ``mode="adaptive"`` (the default): the derivative of the adaptive solve
+  itself, through the error norm, the step-size controller, rejected
+  attempts, the last-step clip and the trapezoid's step lengths.  It is held
+  to central differences (h = 1e-6 relative) of the GPU forward and of the
+  CPU oracle at rtol 1e-8 / atol 1e-10, within 1e-5 relative.  A trajectory
+  whose attempt count changes under +-h is a jump of the adaptive map, not a
+  derivative; such trajectories are reported and left out of the bar.
+- ``mode="frozen"``: the accepted step sizes held fixed, exactly the
+  derivative of a replay of the recorded steps; held to central differences
+  of the replay within 1e-8.
+
+Errors are relative to the largest of |dI/dp_m| over the batch for each
+parameter (so a parameter whose derivative is small for some trajectories is
+held to the batch's scale).  The initial state is a coherent superposition of
+the ground states, which makes the theta derivative non-zero.
 """
 
 from __future__ import annotations
@@ -33,7 +38,6 @@ pytestmark = pytest.mark.gpu
 
 NAMES = ["delta0", "Omega0", "theta", "sigma"]
 TOL = dict(abstol=1e-10, reltol=1e-8, dt=1e-8)
-TIGHT = dict(abstol=1e-14, reltol=1e-12, dt=1e-8)
 
 
 def _batch(n, seed=20261202 + 4):
@@ -61,42 +65,98 @@ def setup():
     return torch, prepared, y0, t_span, weights
 
 
-def _engine(prepared, t_span, weights, tol):
-    return DifferentiablePhotonCount(prepared, NAMES, weights, t_span, **tol)
+def _engine(prepared, t_span, weights, solver="dopri5", **kw):
+    return DifferentiablePhotonCount(prepared, NAMES, weights, t_span, solver=solver, **TOL, **kw)
 
 
 def _fd(fun, p, rel=1e-6):
+    """Central differences; also returns, per trajectory, whether the attempt
+    counts of the two sides and the centre agree (``fun`` returns values and
+    attempt counts)."""
     out = np.zeros_like(p)
+    _, n0 = fun(p)
+    same = np.ones(p.shape[0], dtype=bool)
     for m in range(p.shape[1]):
         h = rel * np.abs(p[:, m])
         hi, lo = p.copy(), p.copy()
         hi[:, m] += h
         lo[:, m] -= h
-        out[:, m] = (fun(hi) - fun(lo)) / (2 * h)
-    return out
+        (vh, nh), (vl, nl) = fun(hi), fun(lo)
+        out[:, m] = (vh - vl) / (2 * h)
+        same &= (nh == n0) & (nl == n0)
+    return out, same
 
 
-def _close(p, counts, got, want, rtol):
-    """|p (got - want)| <= rtol * max photon count, per parameter."""
-    scale = float(np.abs(counts).max())
-    for m in range(want.shape[1]):
-        err = np.abs(p[:, m] * (got[:, m] - want[:, m])).max()
-        print(f"{NAMES[m]}: max |p dI/dp| {np.abs(p[:, m] * want[:, m]).max():.3e}  err/I {err / scale:.2e}")
-        assert err <= rtol * scale, (NAMES[m], err / scale)
+def _rel_errors(got, want, keep):
+    """Per parameter, max over the kept trajectories of |got - want| relative to
+    the batch's largest |want|."""
+    scale = np.abs(want[keep]).max(axis=0) + 1e-300
+    return (np.abs(got - want)[keep] / scale).max(axis=0)
 
 
-def test_adjoint_matches_replay_finite_differences(setup):
+def _gpu_forward(engine, y0):
+    def fun(q):
+        counts, tape = engine.forward(q, y0)
+        return counts.cpu().numpy(), tape.n.cpu().numpy()
+    return fun
+
+
+@pytest.mark.parametrize("solver", ["dopri5", "tsit5"])
+def test_adaptive_adjoint_matches_gpu_finite_differences(setup, solver):
     torch, prepared, y0, t_span, weights = setup
-    engine = _engine(prepared, t_span, weights, TOL)
+    engine = _engine(prepared, t_span, weights, solver)
+    p = _batch(16)
+    counts, tape = engine.forward(p, y0)
+    grad = engine.gradient(p, y0, tape).cpu().numpy()
+    assert np.all(np.isfinite(grad))
+    fd, same = _fd(_gpu_forward(engine, y0), p)
+    err = _rel_errors(grad, fd, same)
+    frozen = engine.gradient(p, y0, tape, mode="frozen").cpu().numpy()
+    err_frozen = _rel_errors(frozen, fd, same)
+    print(f"{solver}: 16 trajectories, {int((~same).sum())} with attempt counts changing under +-h;"
+          f" adaptive vs FD {np.array2string(err, precision=2)}; frozen-step vs FD"
+          f" {np.array2string(err_frozen, precision=2)}")
+    assert same.sum() >= 12
+    assert np.all(err <= 1e-5), err
+
+
+def test_adaptive_adjoint_matches_oracle_finite_differences(setup):
+    torch, prepared, y0, t_span, weights = setup
+    engine = _engine(prepared, t_span, weights)
+    p = _batch(16, seed=11)
+    counts, tape = engine.forward(p, y0)
+    grad = engine.gradient(p, y0, tape).cpu().numpy()
+    oracle = Oracle(prepared)
+
+    def counts_cpu(q):
+        r = oracle.solve_batch(y0, t_span, output="photon_integral", output_when="final",
+                               integral_weights=weights, parameter_slots=NAMES,
+                               parameter_batch=q.astype(complex), **TOL)
+        return r["final"][:, 0], r["accepted"] + r["rejected"]
+
+    fd, same = _fd(counts_cpu, p)
+    np.testing.assert_allclose(counts.cpu().numpy(), counts_cpu(p)[0], rtol=0, atol=1e-7)
+    err = _rel_errors(grad, fd, same)
+    print(f"oracle: {int((~same).sum())} of 16 trajectories change step counts under +-h;"
+          f" adaptive vs oracle FD {np.array2string(err, precision=2)}")
+    assert same.sum() >= 12
+    assert np.all(err <= 1e-5), err
+
+
+def test_frozen_adjoint_matches_replay_finite_differences(setup):
+    torch, prepared, y0, t_span, weights = setup
+    engine = _engine(prepared, t_span, weights, mode="frozen")
     p = _batch(6)
     counts, tape = engine.forward(p, y0)
     grad = engine.gradient(p, y0, tape).cpu().numpy()
     assert np.all(np.isfinite(grad))
     # the replay at the recorded parameters is the recorded solve, bit for bit
     np.testing.assert_array_equal(engine.replay(p, y0, tape).cpu().numpy(), counts.cpu().numpy())
-    fd = _fd(lambda q: engine.replay(q, y0, tape).cpu().numpy(), p)
+    fd, _ = _fd(lambda q: (engine.replay(q, y0, tape).cpu().numpy(), np.zeros(len(q))), p)
     assert np.abs(p[:, 2] * fd[:, 2]).max() > 1e-3 * np.abs(counts.cpu().numpy()).max()  # theta matters
-    _close(p, counts.cpu().numpy(), grad, fd, 1e-8)
+    err = _rel_errors(grad, fd, np.ones(len(p), dtype=bool))
+    print("frozen vs replay FD", err)
+    assert np.all(err <= 1e-8), err
     # the recorded forward is the plain solve's photon count
     plan = prepared.device_plan()
     slots = [prepared.slot_names.index(n) for n in NAMES]
@@ -105,50 +165,25 @@ def test_adjoint_matches_replay_finite_differences(setup):
     np.testing.assert_allclose(counts.cpu().numpy(), ref.cpu().numpy()[:, 0], rtol=1e-14, atol=0)
 
 
-def test_adjoint_kernel_forms_agree(setup):
-    """The single-wave adjoint (variant 4, the default for this plan) and the
-    multi-wave form (variant 0) compute the same derivative."""
+@pytest.mark.parametrize("mode", ["adaptive", "frozen"])
+def test_adjoint_kernel_forms_agree(setup, mode):
+    """The single-wave adjoint (variant 5, the default for this plan) and the
+    multi-wave form (variant 1) compute the same derivative."""
     torch, prepared, y0, t_span, weights = setup
-    engine = _engine(prepared, t_span, weights, TOL)
+    engine = _engine(prepared, t_span, weights, mode=mode)
     p = _batch(5, seed=9)
     counts, tape = engine.forward(p, y0)
     wave = engine.gradient(p, y0, tape, variant=5).cpu().numpy()
     multi = engine.gradient(p, y0, tape, variant=1).cpu().numpy()
     auto = engine.gradient(p, y0, tape).cpu().numpy()
     np.testing.assert_array_equal(auto, wave)
-    _close(p, counts.cpu().numpy(), wave, multi, 1e-12)
-
-
-def test_adjoint_matches_adaptive_finite_differences(setup):
-    torch, prepared, y0, t_span, weights = setup
-    engine = _engine(prepared, t_span, weights, TIGHT)
-    p = _batch(4, seed=5)
-    counts, tape = engine.forward(p, y0)
-    grad = engine.gradient(p, y0, tape).cpu().numpy()
-    fd = _fd(lambda q: engine.forward(q, y0)[0].cpu().numpy(), p)
-    _close(p, counts.cpu().numpy(), grad, fd, 2e-5)
-
-
-def test_adjoint_matches_oracle_finite_differences(setup):
-    torch, prepared, y0, t_span, weights = setup
-    engine = _engine(prepared, t_span, weights, TIGHT)
-    p = _batch(2, seed=11)
-    counts, tape = engine.forward(p, y0)
-    grad = engine.gradient(p, y0, tape).cpu().numpy()
-    oracle = Oracle(prepared)
-
-    def counts_cpu(q):
-        r = oracle.solve_batch(y0, t_span, output="photon_integral", output_when="final",
-                               integral_weights=weights, parameter_slots=NAMES,
-                               parameter_batch=q.astype(complex), **TIGHT)
-        return r["final"][:, 0]
-
-    _close(p, counts.cpu().numpy(), grad, _fd(counts_cpu, p), 2e-5)
+    err = _rel_errors(wave, multi, np.ones(len(p), dtype=bool))
+    assert np.all(err <= 1e-12), err
 
 
 def test_autograd_function(setup):
     torch, prepared, y0, t_span, weights = setup
-    engine = _engine(prepared, t_span, weights, TOL)
+    engine = _engine(prepared, t_span, weights)
     p = torch.tensor(_batch(4, seed=3), dtype=torch.float64, device="cuda", requires_grad=True)
     target = torch.linspace(0.5, 2.0, 4, dtype=torch.float64, device="cuda")
     counts = photon_count(engine, p, y0)
@@ -160,10 +195,10 @@ def test_autograd_function(setup):
     np.testing.assert_allclose(p.grad.cpu().numpy(), want.cpu().numpy(), rtol=1e-13, atol=0)
 
 
-def test_adjoint_n72_matches_replay_finite_differences():
+def test_adjoint_n72_matches_finite_differences():
     """The multi-wave adjoint on the n=72 system (the 1 MHz polarisation square
-    wave, theta rotating the switched pair, a coherent ground-state start)
-    against central differences of the replayed discrete map."""
+    wave, theta rotating the switched pair, a coherent ground-state start):
+    frozen against the replayed map, adaptive against the forward."""
     require_gpu()
     system, params, rho0, _, weights = tlf_n72_theta()
     prepared = prepare_lindblad_problem(system, params, backend="host")
@@ -172,13 +207,15 @@ def test_adjoint_n72_matches_replay_finite_differences():
     y0 = prepared.layout.pack(_coherent_ground(system))  # breaks the symmetry that zeroes d/dtheta
     p = np.asarray([[0.4 * GAMMA, 1.2 * GAMMA, 0.3], [-0.8 * GAMMA, 2.0 * GAMMA, 1.1]])
     counts, tape = engine.forward(p, y0)
+    frozen = engine.gradient(p, y0, tape, mode="frozen").cpu().numpy()
     grad = engine.gradient(p, y0, tape).cpu().numpy()
-    assert np.all(np.isfinite(grad))
-    fd = _fd(lambda q: engine.replay(q, y0, tape).cpu().numpy(), p)
-    c = counts.cpu().numpy()
-    scale = float(np.abs(c).max())
-    for m, name in enumerate(names):
-        err = np.abs(p[:, m] * (grad[:, m] - fd[:, m])).max()
-        print(f"{name}: max |p dI/dp| {np.abs(p[:, m] * fd[:, m]).max():.3e}  err/I {err / scale:.2e}")
-        assert err <= 1e-7 * scale, (name, err / scale)
-    assert np.abs(p[:, 2] * fd[:, 2]).max() > 1e-4 * scale  # theta matters here
+    assert np.all(np.isfinite(grad)) and np.all(np.isfinite(frozen))
+    keep = np.ones(len(p), dtype=bool)
+    fd_r, _ = _fd(lambda q: (engine.replay(q, y0, tape).cpu().numpy(), np.zeros(len(q))), p)
+    err_r = _rel_errors(frozen, fd_r, keep)
+    fd, same = _fd(_gpu_forward(engine, y0), p)
+    err = _rel_errors(grad, fd, same) if same.any() else np.zeros(3)
+    print(f"n72: frozen vs replay FD {err_r}; adaptive vs FD {err} ({int(same.sum())} of 2 kept)")
+    assert np.all(err_r <= 1e-7), err_r
+    assert np.all(err <= 1e-5), err
+    assert np.abs(fd_r[:, 2]).max() > 1e-4 * np.abs(counts.cpu().numpy()).max()  # theta matters here
