@@ -35,69 +35,18 @@ import numpy as np
 
 from .ir import BuiltinFunctionId as B
 from .ir import InstructionOp as Op
+from .postfix import COMPARE as _COMPARE
+from .postfix import parse, program
 
 __all__ = ["differentiate_flat_plan"]
 
 _PIECEWISE_CONSTANT_HELPERS = {4, 7, 12, 13}
-_BINARY = {Op.ADD, Op.SUB, Op.MUL, Op.DIV, Op.POW}
-_COMPARE = set(range(14, 20))
 
 ZERO = ("zero",)
 ONE = ("const", 1.0, 0.0)
 
 
 # ---- trees -----------------------------------------------------------------
-
-def _parse(rows: Sequence[tuple]) -> tuple:
-    stack: list[tuple] = []
-    for op, index, argc, function, re, im in rows:
-        op, index, argc, function = int(op), int(index), int(argc), int(function)
-        if op == Op.CONST:
-            stack.append(("const", float(re), float(im)))
-        elif op == Op.SLOT:
-            stack.append(("slot", index))
-        elif op == Op.TIME:
-            stack.append(("time",))
-        elif op in _BINARY or op in _COMPARE:
-            b, a = stack.pop(), stack.pop()
-            stack.append(("bin", op, a, b))
-        elif op in (Op.NEG, Op.CONJ):
-            stack.append(("un", op, stack.pop()))
-        elif op in (Op.BUILTIN_FUNC, Op.HELPER_FUNC):
-            args = tuple(stack[len(stack) - argc:])
-            del stack[len(stack) - argc:]
-            stack.append(("call", op, function, args))
-        else:
-            raise NotImplementedError(f"instruction op {op} cannot be differentiated")
-    if len(stack) != 1:
-        raise ValueError("malformed expression")
-    return stack[0]
-
-
-def _emit(node: tuple, out: list[tuple]) -> None:
-    kind = node[0]
-    if kind == "zero":
-        out.append((int(Op.CONST), 0, 0, 0, 0.0, 0.0))
-    elif kind == "const":
-        out.append((int(Op.CONST), 0, 0, 0, node[1], node[2]))
-    elif kind == "slot":
-        out.append((int(Op.SLOT), node[1], 0, 0, 0.0, 0.0))
-    elif kind == "time":
-        out.append((int(Op.TIME), 0, 0, 0, 0.0, 0.0))
-    elif kind == "bin":
-        _emit(node[2], out)
-        _emit(node[3], out)
-        out.append((int(node[1]), 0, 0, 0, 0.0, 0.0))
-    elif kind == "un":
-        _emit(node[2], out)
-        out.append((int(node[1]), 0, 0, 0, 0.0, 0.0))
-    elif kind == "call":
-        for a in node[3]:
-            _emit(a, out)
-        out.append((int(node[1]), 0, len(node[3]), int(node[2]), 0.0, 0.0))
-    else:
-        raise ValueError(kind)
-
 
 def _is_zero(n: tuple) -> bool:
     return n[0] == "zero"
@@ -203,12 +152,6 @@ def _deriv(node: tuple, dslot: dict[int, tuple], wrt_time: bool = False) -> tupl
     raise ValueError(kind)
 
 
-def _program(node: tuple) -> list[tuple]:
-    out: list[tuple] = []
-    _emit(node, out)
-    return out
-
-
 def differentiate_flat_plan(flat: dict[str, Any], param_slots: Sequence[int],
                             with_time: bool = False) -> dict[str, Any]:
     """A copy of ``flat`` with derivative compounds and K*P derivative programs.
@@ -228,7 +171,7 @@ def differentiate_flat_plan(flat: dict[str, Any], param_slots: Sequence[int],
     rows = list(zip(flat["instr_op"], flat["instr_index"], flat["instr_argc"], flat["instr_function"],
                     flat["instr_re"], flat["instr_im"]))
     n_comp, K = int(flat["n_compounds"]), int(flat["n_coefficients"])
-    trees = [_parse(rows[ptrs[e]:ptrs[e + 1]]) for e in range(n_comp + K)]
+    trees = [parse(rows[ptrs[e]:ptrs[e + 1]]) for e in range(n_comp + K)]
     comp_slots = [int(s) for s in flat["compound_slots"]]
     next_slot = int(flat["n_slots"])
     deriv_compounds: list[tuple[int, list[tuple]]] = []
@@ -244,10 +187,10 @@ def differentiate_flat_plan(flat: dict[str, Any], param_slots: Sequence[int],
                 dslot[comp_slots[c]] = d
                 continue
             dslot[comp_slots[c]] = ("slot", next_slot)
-            deriv_compounds.append((next_slot, _program(d)))
+            deriv_compounds.append((next_slot, program(d)))
             next_slot += 1
         dslots.append(dslot)
-    deriv_coefs = [_program(_deriv(trees[n_comp + k], dslots[m], columns[m][1]))
+    deriv_coefs = [program(_deriv(trees[n_comp + k], dslots[m], columns[m][1]))
                    for k in range(K) for m in range(len(columns))]
 
     programs = [rows[ptrs[e]:ptrs[e + 1]] for e in range(n_comp)]

@@ -546,34 +546,28 @@ def flatten_plan(n: int, parameter_graph: dict[str, Any], hamiltonian_plan: dict
 
     Expressions are concatenated: compounds first (in evaluation order), then
     the Hamiltonian coefficients; ``expr_ptrs`` delimits them.  Tuple-valued
-    base parameters are rejected here because the device expression machine is
-    scalar-only.
+    parameters are lowered to scalar programs first (``tuples.lower_tuples``):
+    the device expression machine is scalar.
     """
-    base = parameter_graph["base_values"]
-    if any(isinstance(v, tuple) for v in base):
-        raise NotImplementedError(
-            "tuple-valued parameters (tabulated/pchip/multipass inputs) are not supported by "
-            "the GPU expression machine"
-        )
-    exprs = [e for _, e in parameter_graph["compounds"]] + [
-        c["expression"] for c in hamiltonian_plan["coefficients"]
-    ]
-    for e in exprs:
-        if not e.scalar_only:
-            raise NotImplementedError("tuple-valued expressions are not supported on the GPU")
-    rows = [ins for e in exprs for ins in e.instructions]
+    from .tuples import lower_tuples
+
+    n_slots, compounds, coefs, base = lower_tuples(
+        parameter_graph["base_values"], len(parameter_graph["slot_names"]), parameter_graph["compounds"],
+        [c["expression"] for c in hamiltonian_plan["coefficients"]])
+    exprs = [rows for _, rows in compounds] + coefs
+    rows = [ins for e in exprs for ins in e]
     ptrs = np.zeros(len(exprs) + 1, dtype=np.int32)
-    np.cumsum([len(e.instructions) for e in exprs], out=ptrs[1:])
+    np.cumsum([len(e) for e in exprs], out=ptrs[1:])
     ins = np.asarray(rows, dtype=np.float64).reshape(-1, 6) if rows else np.zeros((0, 6))
     base_arr = np.asarray([complex(v) for v in base], dtype=np.complex128)
     return {
         "n_states": int(n),
-        "n_slots": len(parameter_graph["slot_names"]),
+        "n_slots": int(n_slots),
         "n_base": len(base),
         "base_re": np.ascontiguousarray(base_arr.real),
         "base_im": np.ascontiguousarray(base_arr.imag),
-        "n_compounds": len(parameter_graph["compounds"]),
-        "compound_slots": np.asarray([s for s, _ in parameter_graph["compounds"]], dtype=np.int32),
+        "n_compounds": len(compounds),
+        "compound_slots": np.asarray([s for s, _ in compounds], dtype=np.int32),
         "n_coefficients": len(hamiltonian_plan["coefficients"]),
         "expr_ptrs": ptrs,
         "instr_op": np.ascontiguousarray(ins[:, 0].astype(np.int32)),

@@ -58,6 +58,8 @@ def library() -> C.CDLL:
     L.lb_last_error.restype = C.c_char_p
     L.lb_plan_specialize.restype = C.c_int
     L.lb_plan_specialize.argtypes = [C.c_void_p, C.c_int32]
+    L.lb_plan_specialize_status.restype = C.c_int
+    L.lb_plan_specialize_status.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
     L.lb_plan_info.restype = C.c_int
     L.lb_plan_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.lb_plan_support_info.restype = C.c_int
@@ -200,6 +202,7 @@ class DevicePlan:
         self.n = self.info["n_states"]
         self.dim = self.info["dim"]
         self.specialized = False
+        self._spec_warned = False
 
     def specialize(self, enable: bool = True) -> "DevicePlan":
         """``lb_plan_specialize``: run the paired kernels of this plan with its
@@ -209,6 +212,15 @@ class DevicePlan:
         _check(library().lb_plan_specialize(self._handle, 1 if enable else 0))
         self.specialized = bool(enable)
         return self
+
+    def specialize_failure(self) -> str | None:
+        """``lb_plan_specialize_status``: the reason a specialised compilation
+        of this plan failed (the precompiled kernels ran instead), or None."""
+        buf = C.create_string_buffer(8192)
+        rc = library().lb_plan_specialize_status(self._handle, buf, len(buf))
+        if rc < 0:
+            _check(rc)
+        return buf.value.decode(errors="replace") if rc == 1 else None
 
     def support_of(self, y0) -> np.ndarray:
         """uint8 mask (dim) of the packed entries that are nonzero in any row of ``y0``."""
@@ -496,6 +508,13 @@ class DevicePlan:
         _check(library().lb_solve(self._handle, C.byref(a), sp))
         if events is not None:
             events[1].record(stream)
+        if self.specialized and not self._spec_warned:
+            why = self.specialize_failure()
+            if why is not None:
+                import warnings
+
+                self._spec_warned = True
+                warnings.warn(f"specialised kernel not compiled, precompiled kernel used: {why}", stacklevel=2)
         return result
 
 

@@ -93,11 +93,7 @@ def _group_calls(groups: Sequence[UnevenGroup], *, t0: float, solver: str, absto
     if solver not in D.GPU_SOLVERS:
         raise NotImplementedError(f"solver {solver!r} is not available on the GPU")
     calls = []
-    seen = set()
     for g, grp in enumerate(groups):
-        if id(grp.prepared) in seen:
-            raise ValueError(f"group {g}: each prepared problem may appear in one group only")
-        seen.add(id(grp.prepared))
         t_end = np.ascontiguousarray(grp.t_end, dtype=np.float64).reshape(-1)
         count = t_end.size
         if count == 0:

@@ -37,7 +37,7 @@ typedef __hip_internal::uint64_t uint64_t;
 #define LB_MAX_AXES 8
 #define LB_MAX_WEIGHTS 64
 #define LB_MAX_SELECTED 128
-#define LB_MAX_STACK 24
+#define LB_MAX_STACK 1024
 #define LB_MAX_DIFF 8
 #define LB_STATIC_COEF 255u
 

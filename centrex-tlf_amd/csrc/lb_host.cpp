@@ -52,7 +52,65 @@ int fail(int code, const std::string& msg) {
       return fail(LB_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
   } while (0)
 
+int is_capturing(hipStream_t s, bool* capturing) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(s, &cs));
+  *capturing = cs != hipStreamCaptureStatusNone;
+  return LB_OK;
+}
+
 }  // namespace
+
+// Work-queue counters of a plan: every call takes its own counter, round
+// robin from a ring, so solves of one plan on different streams or threads
+// never share one.  Before a call resets its counter, its stream waits (on
+// the device, by stream order) for the earlier call that used the same
+// counter.  Inside a stream capture nothing is waited for or recorded: a
+// captured call keeps its counter, and a replay of that graph must not
+// overlap the call that reuses the counter kSlots calls later.
+class QueueRing {
+ public:
+  static constexpr int kSlots = 64;
+  static constexpr int kStride = 64;  // counters 256 bytes apart
+  ~QueueRing() {
+    for (hipEvent_t e : done_)
+      if (e) (void)hipEventDestroy(e);
+    if (base_) (void)hipFree(base_);
+  }
+  int init() {
+    HIP_TRY(hipMalloc(&base_, sizeof(int32_t) * kSlots * kStride));
+    HIP_TRY(hipMemset(base_, 0, sizeof(int32_t) * kSlots * kStride));
+    return LB_OK;
+  }
+  // The counter of a new call on `s` (and its slot, for retire()).
+  int acquire(hipStream_t s, int32_t** counter, int* slot) {
+    bool cap = false;
+    int rc = is_capturing(s, &cap);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(mu_);
+    const int k = (int)(next_++ % kSlots);
+    if (!cap && done_[k]) HIP_TRY(hipStreamWaitEvent(s, done_[k], 0));
+    *counter = base_ + (size_t)k * kStride;
+    *slot = k;
+    return LB_OK;
+  }
+  // Mark the end of the call's launches on `s`.
+  int retire(int slot, hipStream_t s) {
+    bool cap = false;
+    int rc = is_capturing(s, &cap);
+    if (rc || cap) return rc;
+    std::lock_guard<std::mutex> lock(mu_);
+    if (!done_[slot]) HIP_TRY(hipEventCreateWithFlags(&done_[slot], hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(done_[slot], s));
+    return LB_OK;
+  }
+
+ private:
+  int32_t* base_ = nullptr;
+  hipEvent_t done_[kSlots] = {};
+  uint64_t next_ = 0;
+  std::mutex mu_;
+};
 
 // Validated copy of a plan description plus the lowered expression program.
 struct PlanSource {
@@ -86,6 +144,7 @@ struct DevImage {
   void* ws_y = nullptr;  // adjoint workspace (grow-only)
   void* ws_c = nullptr;
   size_t ws_y_len = 0, ws_c_len = 0;
+  hipEvent_t ws_done = nullptr;  // end of the last adjoint that used the workspace
   std::vector<int32_t> g_rows_host;  // gather rows per block
   int t_rows_host = 0;               // transposed rows of block 0 (one-block plans)
 };
@@ -93,7 +152,7 @@ struct DevImage {
 struct lb_plan {
   PlanSource src;
   int device = 0, cus = 0;
-  int32_t* queue = nullptr;
+  QueueRing queues;
   std::unique_ptr<DevImage> full;
   std::map<std::vector<uint8_t>, std::unique_ptr<DevImage>> restricted;
   std::mutex mu;  // guards `restricted`, the occupancy caches and the event buffers
@@ -106,20 +165,23 @@ struct lb_plan {
   hipEvent_t ev_copied = nullptr;
   // plan-specialised paired kernels (lb_plan_specialize): the generated
   // coefficient function, and (variant | gather rows << 8, solver) -> compiled module
+  // A failed compilation leaves a null entry (the precompiled kernel runs)
+  // and its reason in spec_failure.
   bool spec = false;
-  std::string spec_source;
+  std::string spec_source, spec_failure;
   std::map<std::pair<int, int>, std::pair<hipModule_t, hipFunction_t>> spec_kernels;
   ~lb_plan() {
-    for (auto& kv : spec_kernels) (void)hipModuleUnload(kv.second.first);
+    for (auto& kv : spec_kernels)
+      if (kv.second.first) (void)hipModuleUnload(kv.second.first);
     auto release = [](DevImage* im) {
       if (!im) return;
       for (void* a : im->allocations) (void)hipFree(a);
       if (im->ws_y) (void)hipFree(im->ws_y);
       if (im->ws_c) (void)hipFree(im->ws_c);
+      if (im->ws_done) (void)hipEventDestroy(im->ws_done);
     };
     release(full.get());
     for (auto& kv : restricted) release(kv.second.get());
-    if (queue) (void)hipFree(queue);
     if (ev_copied) (void)hipEventSynchronize(ev_copied);
     if (ev_dev) (void)hipFree(ev_dev);
     if (ev_pinned) (void)hipHostFree(ev_pinned);
@@ -159,13 +221,20 @@ int check_expression(const lb_plan_desc* d, int start, int end, int n_slots, std
         sp -= d->instr_argc[p] - 1;
         break;
       case 13: {
-        int f = d->instr_function[p];
-        if (f == 8 || f == 10 || f == 14 || f == 16) {
-          why = "tuple-valued helper " + std::to_string(f) + " is not evaluated on the GPU";
-          return -2;
+        const int f = d->instr_function[p], n = d->instr_argc[p];
+        if (f < 1 || f > 17) { why = "unknown helper id"; return -1; }
+        if (n < 1) { why = "helper without arguments"; return -1; }
+        // tuple helpers take their tuples' elements as consecutive arguments
+        bool shape = true;
+        if (f == 8) shape = n >= 4 && (n - 4) % 3 == 0;
+        if (f == 10) shape = n >= 5 && ((n - 5) % 3 == 0 || (n - 6) % 3 == 0);
+        if (f == 14 || f == 16) shape = n >= 3 && n % 2 == 1;
+        if (f == 17) shape = n >= 5 && n % 2 == 1;
+        if (!shape) {
+          why = "helper " + std::to_string(f) + " with " + std::to_string(n) + " arguments";
+          return -1;
         }
-        if (f < 1 || f > 15) { why = "unknown helper id"; return -1; }
-        sp -= d->instr_argc[p] - 1;
+        sp -= n - 1;
         break;
       }
       case 14: case 15: case 16: case 17: case 18: case 19: sp -= 1; break;
@@ -639,11 +708,7 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   // ---- device side
   HIP_TRY(hipGetDevice(&p->device));
   HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device));
-  {
-    void* q = nullptr;
-    HIP_TRY(hipMalloc(&q, 64));
-    p->queue = static_cast<int32_t*>(q);
-  }
+  if (int rc0 = p->queues.init()) return rc0;
   p->full.reset(new DevImage());
   int rc = build_image(src, std::vector<uint8_t>(U, 1), p->full.get());
   if (rc != LB_OK) return rc;
@@ -673,6 +738,18 @@ extern "C" int lb_plan_specialize(lb_plan* p, int32_t enable) {
   }
   p->spec = true;
   return LB_OK;
+}
+
+extern "C" int lb_plan_specialize_status(const lb_plan* p, char* buf, int64_t cap) {
+  if (!p) return fail(LB_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lock(const_cast<lb_plan*>(p)->mu);
+  if (p->spec_failure.empty()) return 0;
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>((size_t)cap - 1, p->spec_failure.size());
+    std::memcpy(buf, p->spec_failure.data(), n);
+    buf[n] = '\0';
+  }
+  return 1;
 }
 
 extern "C" int lb_plan_info(const lb_plan* p, int64_t* info) {
@@ -946,7 +1023,6 @@ int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out,
     c.tape_cap = a->tape_cap;
   }
   c.out_event_t = a->out_event_t;
-  c.queue = p->queue;
   *im_out = im;
   return LB_OK;
 }
@@ -1054,6 +1130,8 @@ int launch_table(const lb_plan* p, const lb_solve_args* a, const DevImage* im, D
   const size_t lds = (size_t)lds_bytes_for(P);
   int rc = set_lds_limit(fn, lds);
   if (rc) return rc;
+  int slot = 0;
+  if ((rc = const_cast<lb_plan*>(p)->queues.acquire(s, &c.queue, &slot))) return rc;
   if (a->mode == LB_MODE_PLAIN) {
     for (int64_t i = 0; i < a->n_traj; ++i) {
       c.n_traj = 1;
@@ -1064,7 +1142,7 @@ int launch_table(const lb_plan* p, const lb_solve_args* a, const DevImage* im, D
       HIP_TRY(hipLaunchKernel(fn, dim3(1), dim3(W), args, lds, s));
     }
     HIP_TRY(hipGetLastError());
-    return LB_OK;
+    return const_cast<lb_plan*>(p)->queues.retire(slot, s);
   }
   int grid = a->workgroups;
   if (grid <= 0) {
@@ -1079,7 +1157,7 @@ int launch_table(const lb_plan* p, const lb_solve_args* a, const DevImage* im, D
   void* args[] = {&P, &c};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
   HIP_TRY(hipGetLastError());
-  return LB_OK;
+  return const_cast<lb_plan*>(p)->queues.retire(slot, s);
 }
 
 }  // namespace
@@ -1114,7 +1192,14 @@ int spec_kernel(lb_plan* p, int id, int solver, int rows, hipStream_t stream, hi
     hipModule_t mod = nullptr;
     hipFunction_t f = nullptr;
     std::string why;
-    if (lb_rtc_compile(p->spec_source, name, &mod, &f, why)) return fail(LB_ERR_DEVICE, why);
+    if (lb_rtc_compile(p->spec_source, name, &mod, &f, why)) {
+      // the precompiled kernel computes the same values: run it, remember
+      // the failure so that later calls neither recompile nor fail
+      (void)hipGetLastError();  // a failed module load leaves its error behind
+      mod = nullptr;
+      f = nullptr;
+      p->spec_failure = std::string(name) + ": " + why;
+    }
     it = p->spec_kernels.emplace(std::make_pair(id | (G << 8), solver), std::make_pair(mod, f)).first;
   }
   *fn = it->second.second;
@@ -1148,6 +1233,8 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   if ((rc = set_lds_limit(fn, lds))) return rc;
   if (a->mode == LB_MODE_PLAIN) {
     // one launch per trajectory: the per-trajectory GPU path
+    int slot = 0;
+    if ((rc = const_cast<lb_plan*>(p)->queues.acquire(s, &c.queue, &slot))) return rc;
     const int64_t total = a->n_traj;
     for (int64_t i = 0; i < total; ++i) {
       c.n_traj = 1;
@@ -1158,7 +1245,7 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
       HIP_TRY(hipLaunchKernel(fn, dim3(1), dim3(W), args, lds, s));
     }
     HIP_TRY(hipGetLastError());
-    return LB_OK;
+    return const_cast<lb_plan*>(p)->queues.retire(slot, s);
   }
   int grid = a->workgroups;
   if (grid <= 0) {
@@ -1188,6 +1275,8 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
                                   pair ? (a->n_traj + 1) / 2 : a->n_traj);
   }
   c.traj_offset = 0;
+  int slot = 0;
+  if ((rc = const_cast<lb_plan*>(p)->queues.acquire(s, &c.queue, &slot))) return rc;
   void* rargs[] = {&c.queue};
   HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
   void* args[] = {&P, &c};
@@ -1196,7 +1285,7 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   else
     HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
   HIP_TRY(hipGetLastError());
-  return LB_OK;
+  return const_cast<lb_plan*>(p)->queues.retire(slot, s);
 }
 
 namespace {
@@ -1285,12 +1374,18 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   }
   grid = (int)std::min<int64_t>(grid, a->n_traj);
   // workspace: ckpt_every recomputed attempts per resident workgroup (y and
-  // k1..k7, and the seven coefficient rows of each)
-  const size_t need_y = (size_t)grid * a->ckpt_every * 8 * S * 2;
-  const size_t need_c = (size_t)grid * a->ckpt_every * 7 * (im->dev.KD + 1) * 2;
+  // k1..k7, and the seven coefficient rows of each), in doubles
+  const size_t per_y = (size_t)a->ckpt_every * 8 * S * 2;
+  const size_t per_c = (size_t)a->ckpt_every * 7 * (im->dev.KD + 1) * 2;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t need_y = (size_t)grid * per_y, need_c = (size_t)grid * per_c;
+  bool cap = false;
+  if ((rc = is_capturing(s, &cap))) return rc;
+  // one adjoint at a time uses an image's workspace: a call waits (on the
+  // device) for the previous one, whatever its stream
+  std::lock_guard<std::mutex> lock(const_cast<lb_plan*>(p)->mu);
+  DevImage* mi = const_cast<DevImage*>(im);
   {
-    std::lock_guard<std::mutex> lock(const_cast<lb_plan*>(p)->mu);
-    DevImage* mi = const_cast<DevImage*>(im);
     if (mi->ws_y_len < need_y) {
       if (mi->ws_y) (void)hipFree(mi->ws_y);
       mi->ws_y = nullptr;
@@ -1308,15 +1403,21 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
     c.seg_y = static_cast<double*>(mi->ws_y);
     c.seg_c = static_cast<double*>(mi->ws_c);
   }
-  hipStream_t s = (hipStream_t)stream;
+  if (!cap && mi->ws_done) HIP_TRY(hipStreamWaitEvent(s, mi->ws_done, 0));
   DevPlan P = im->dev;
   c.traj_offset = 0;
+  int slot = 0;
+  if ((rc = const_cast<lb_plan*>(p)->queues.acquire(s, &c.queue, &slot))) return rc;
   void* rargs[] = {&c.queue};
   HIP_TRY(hipLaunchKernel(lb_dev_queue_reset_kernel(), dim3(1), dim3(1), rargs, 0, s));
   void* args[] = {&P, &c};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
   HIP_TRY(hipGetLastError());
-  return LB_OK;
+  if (!cap) {
+    if (!mi->ws_done) HIP_TRY(hipEventCreateWithFlags(&mi->ws_done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(mi->ws_done, s));
+  }
+  return const_cast<lb_plan*>(p)->queues.retire(slot, s);
 }
 
 namespace {
@@ -1342,9 +1443,6 @@ extern "C" int lb_solve_group(const lb_plan* const* plans, const lb_solve_args* 
       return fail(LB_ERR_INVALID, "grouped solves use the persistent mode");
     if (args[g].event_kind != LB_EVENT_NONE)
       return fail(LB_ERR_UNSUPPORTED, "grouped solves take no stop events (solve each plan with lb_solve)");
-    for (int h = 0; h < g; ++h)
-      if (plans[h] == plans[g])
-        return fail(LB_ERR_INVALID, "a plan may appear in one group only (it holds one work queue)");
     if (plans[g]->device != plans[0]->device) return fail(LB_ERR_INVALID, "all plans must be on one device");
   }
   if (n_groups == 1) return lb_solve(plans[0], &args[0], stream);
@@ -1419,10 +1517,9 @@ struct lb_eff_plan {
   DevEffPlan dev{};
   std::vector<void*> allocations;
   int device = 0, cus = 0;
-  int32_t* queue = nullptr;
+  QueueRing queues;
   ~lb_eff_plan() {
     for (void* a : allocations) (void)hipFree(a);
-    if (queue) (void)hipFree(queue);
   }
 };
 
@@ -1587,8 +1684,7 @@ extern "C" int lb_eff_plan_create(const lb_eff_plan_desc* d, lb_eff_plan** out_p
   if ((rc = eff_upload(p.get(), comp, &P.comp))) return rc;
   HIP_TRY(hipGetDevice(&p->device));
   HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device));
-  HIP_TRY(hipMalloc(&p->queue, sizeof(int32_t)));
-  HIP_TRY(hipMemset(p->queue, 0, sizeof(int32_t)));
+  if ((rc = p->queues.init())) return rc;
   int id, W;
   if ((rc = eff_config(P, &id, &W))) return rc;
   if (lds_bytes_eff(P) > kMaxLdsBytes) return fail(LB_ERR_UNSUPPORTED, "effective plan needs more LDS than a workgroup has");
@@ -1671,7 +1767,6 @@ extern "C" int lb_eff_solve(const lb_eff_plan* p, const lb_eff_solve_args* a, vo
   c.out_counts = a->out_counts;
   c.out_stats = a->out_stats;
   c.out_fail_t = a->out_fail_t;
-  c.queue = p->queue;
   int id, W, rc;
   if ((rc = eff_config(P, &id, &W))) return rc;
   void* fn = lb_dev_eff_kernel(id, a->solver);
@@ -1687,14 +1782,15 @@ extern "C" int lb_eff_solve(const lb_eff_plan* p, const lb_eff_solve_args* a, vo
   }
   grid = (int)std::min<int64_t>(grid, a->n_traj);
   void* reset = lb_dev_queue_reset_kernel();
-  int32_t* q = p->queue;
-  void* rargs[] = {&q};
+  int slot = 0;
+  if ((rc = const_cast<lb_eff_plan*>(p)->queues.acquire(s, &c.queue, &slot))) return rc;
+  void* rargs[] = {&c.queue};
   HIP_TRY(hipLaunchKernel(reset, dim3(1), dim3(1), rargs, 0, s));
   DevEffPlan Pc = P;
   void* args[] = {&Pc, &c};
   HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
   HIP_TRY(hipGetLastError());
-  return LB_OK;
+  return const_cast<lb_eff_plan*>(p)->queues.retire(slot, s);
 }
 
 extern "C" int lb_eff_rhs_eval(const lb_eff_plan* p, int64_t batch, const double* y, const double* t,

@@ -95,7 +95,102 @@ __device__ double rabi_from_intensity(double intensity, double coupling, double 
   return sqrt(intensity * 2.0 / (c * eps0)) * coupling * dipole / hbar;
 }
 
-// Scalar helpers of rust/src/lindblad/eval.rs:155-451, 664-789 (ids 1-7, 9, 11, 12, 13, 15).
+// Tuple helpers.  The host passes a tuple argument as its L elements, one
+// after the other (a tuple's structure is resolved when the plan is built),
+// and the real parts are used, as the reference's as_real does for real input.
+
+// Sum of L Gaussian spots at (x, y): a = x, y, amplitudes[L], xlocs[L],
+// ylocs[L], sigma_x, sigma_y (rust/src/lindblad/eval.rs:250-275).
+__device__ double multipass_intensity(const dcpx* a, int L) {
+  const double x = a[0].re, y = a[1].re, sx = a[2 + 3 * L].re, sy = a[3 + 3 * L].re;
+  double total = 0.0;
+  for (int k = 0; k < L; ++k) total += gaussian_2d(x, y, a[2 + k].re, a[2 + L + k].re, a[2 + 2 * L + k].re, sx, sy);
+  return total;
+}
+
+// Piecewise-linear interpolation, flat outside the grid; a = x, grid[L],
+// values[L] (eval.rs:360-388).  A non-increasing interval reached by the scan
+// gives NaN (the reference raises; constant grids are checked on the host).
+__device__ double linear_interp(const dcpx* a, int L) {
+  const double x = a[0].re;
+  const dcpx* g = a + 1;
+  const dcpx* v = a + 1 + L;
+  if (L == 1 || x <= g[0].re) return v[0].re;
+  for (int i = 0; i + 1 < L; ++i) {
+    const double left = g[i].re, right = g[i + 1].re;
+    if (right <= left) return __builtin_nan("");
+    if (x <= right) {
+      const double frac = (x - left) / (right - left);
+      return v[i].re + frac * (v[i + 1].re - v[i].re);
+    }
+  }
+  return v[L - 1].re;
+}
+
+// PCHIP slope at knot k of a strictly increasing grid: one-sided secants at
+// the ends, the weighted harmonic mean of the neighbouring secants inside,
+// zero at a local extremum (eval.rs:318-331, plan.rs:54-66).
+__device__ double pchip_slope(const dcpx* g, const dcpx* v, int L, int k) {
+  auto secant = [&](int i) { return (v[i + 1].re - v[i].re) / (g[i + 1].re - g[i].re); };
+  if (k == 0) return secant(0);
+  if (k == L - 1) return secant(L - 2);
+  const double d0 = secant(k - 1), d1 = secant(k);
+  const double h0 = g[k].re - g[k - 1].re, h1 = g[k + 1].re - g[k].re;
+  if ((d0 > 0.0) != (d1 > 0.0) || (d0 < 0.0) != (d1 < 0.0) || d0 == 0.0 || d1 == 0.0) return 0.0;
+  const double w1 = 2.0 * h1 + h0, w2 = h1 + 2.0 * h0;
+  return (w1 + w2) / (w1 / d0 + w2 / d1);
+}
+
+// pchip_interp in the Hermite form of eval.rs:290-351 (constant outside the
+// grid), a = x, grid[L], values[L].  A non-increasing grid gives NaN.
+__device__ double pchip_hermite(const dcpx* a, int L) {
+  const double x = a[0].re;
+  const dcpx* g = a + 1;
+  const dcpx* v = a + 1 + L;
+  if (L == 1) return v[0].re;
+  for (int i = 0; i + 1 < L; ++i)
+    if (g[i + 1].re - g[i].re <= 0.0) return __builtin_nan("");
+  if (x <= g[0].re) return v[0].re;
+  if (x >= g[L - 1].re) return v[L - 1].re;
+  int i = 0;
+  while (i < L - 2 && !(x < g[i + 1].re)) ++i;
+  const double h = g[i + 1].re - g[i].re;
+  const double t = (x - g[i].re) / h, t2 = t * t, t3 = t2 * t;
+  const double h00 = 2.0 * t3 - 3.0 * t2 + 1.0, h10 = t3 - 2.0 * t2 + t;
+  const double h01 = -2.0 * t3 + 3.0 * t2, h11 = t3 - t2;
+  return h00 * v[i].re + h10 * h * pchip_slope(g, v, L, i) + h01 * v[i + 1].re +
+         h11 * h * pchip_slope(g, v, L, i + 1);
+}
+
+// pchip_interp through the reference's precomputed table (plan.rs:33-115):
+// the cubic of interval i in Horner form, c0 + dx (c1 + dx (c2 + dx c3)),
+// the interval the last knot at or below x (the last one at or beyond the
+// grid's end), the first value at or below its start.  a = x, grid[L],
+// values[L] of the table.
+__device__ double pchip_table(const dcpx* a, int L) {
+  const double x = a[0].re;
+  const dcpx* g = a + 1;
+  const dcpx* v = a + 1 + L;
+  if (x <= g[0].re) return v[0].re;
+  int i;
+  double dx;
+  if (x >= g[L - 1].re) {
+    i = L - 2;
+    dx = g[L - 1].re - g[L - 2].re;
+  } else {
+    i = 0;
+    while (i < L - 2 && g[i + 1].re <= x) ++i;
+    dx = x - g[i].re;
+  }
+  const double h = g[i + 1].re - g[i].re, delta = (v[i + 1].re - v[i].re) / h;
+  const double d0 = pchip_slope(g, v, L, i), d1 = pchip_slope(g, v, L, i + 1);
+  const double c2 = (3.0 * delta - 2.0 * d0 - d1) / h, c3 = (d0 + d1 - 2.0 * delta) / (h * h);
+  return v[i].re + dx * (d0 + dx * (c2 + dx * c3));
+}
+
+// Scalar helpers of rust/src/lindblad/eval.rs:155-451, 664-789 (ids 1-7, 9,
+// 11, 12, 13, 15), the tuple helpers (8, 10, 14, 16) and the table form of
+// pchip_interp (17).
 __device__ __noinline__ dcpx dev_helper(int id, const dcpx* a, int argc) {
   const double ED_XTB = 0.315 * 1.602176634e-19 * 5.29177210544e-11;
 #define A(i) (a[i].re)
@@ -143,6 +238,15 @@ __device__ __noinline__ dcpx dev_helper(int id, const dcpx* a, int argc) {
       double dx = A(0) - A(1);
       return mk(exp(-dx * dx / (2.0 * A(2) * A(2))), 0.0);
     }
+    case 8: return mk(multipass_intensity(a, (argc - 4) / 3), 0.0);
+    case 10: {
+      const int L = (argc - 5) % 3 == 0 ? (argc - 5) / 3 : (argc - 6) / 3;
+      return mk(rabi_from_intensity(multipass_intensity(a, L), A(4 + 3 * L), argc == 6 + 3 * L ? A(5 + 3 * L) : ED_XTB),
+                0.0);
+    }
+    case 14: return mk(linear_interp(a, (argc - 1) / 2), 0.0);
+    case 16: return mk(pchip_hermite(a, (argc - 1) / 2), 0.0);
+    case 17: return mk(pchip_table(a, (argc - 1) / 2), 0.0);
     default: return mk(__builtin_nan(""), 0.0);  // rejected at plan creation
   }
 #undef A
@@ -250,14 +354,15 @@ __device__ __forceinline__ dcpx vm_eval(const __attribute__((address_space(4))) 
   return top;
 }
 
-// Butcher tableaus (rust/src/ode/dopri5.rs:5-37, rust/src/ode/tsit5.rs:5-38).
+// Butcher tableaus (rust/src/ode/dopri5.rs:5-37, rust/src/ode/tsit5.rs:5-38):
+// c(s) the node of stage s (c(1) = 0, c(7) = 1), a(s, j), the error weights e(j).
 template <int SOLVER>
 struct Tab;
 
 template <>
 struct Tab<0> {
   __device__ static constexpr double c(int s) {
-    return s == 2 ? 1.0 / 5.0 : s == 3 ? 3.0 / 10.0 : s == 4 ? 4.0 / 5.0 : s == 5 ? 8.0 / 9.0 : 1.0;
+    return s <= 1 ? 0.0 : s == 2 ? 1.0 / 5.0 : s == 3 ? 3.0 / 10.0 : s == 4 ? 4.0 / 5.0 : s == 5 ? 8.0 / 9.0 : 1.0;
   }
   __device__ static constexpr double a(int s, int j) {
     return s == 2 ? (j == 1 ? 1.0 / 5.0 : 0.0)
@@ -279,7 +384,7 @@ struct Tab<0> {
 template <>
 struct Tab<1> {
   __device__ static constexpr double c(int s) {
-    return s == 2 ? 0.161 : s == 3 ? 0.327 : s == 4 ? 0.9 : s == 5 ? 0.9800255409045097 : 1.0;
+    return s <= 1 ? 0.0 : s == 2 ? 0.161 : s == 3 ? 0.327 : s == 4 ? 0.9 : s == 5 ? 0.9800255409045097 : 1.0;
   }
   __device__ static constexpr double a(int s, int j) {
     return s == 2 ? (j == 1 ? 0.161 : 0.0)

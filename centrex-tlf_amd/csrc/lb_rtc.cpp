@@ -14,6 +14,7 @@
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -162,9 +163,15 @@ int compile_code(const std::string& coef_source, const std::string& name_expr, C
     return -1;
   }
   hiprtcAddNameExpression(prog, name_expr.c_str());
-  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
-                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  std::vector<std::string> words = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  if (const char* extra = std::getenv("CENTREX_LB_RTC_OPTIONS")) {  // diagnostic hook
+    std::istringstream in(extra);
+    for (std::string w; in >> w;) words.push_back(w);
+  }
+  std::vector<const char*> opts;
+  for (const auto& w : words) opts.push_back(w.c_str());
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
   if (rc != HIPRTC_SUCCESS) {
     size_t n = 0;
     hiprtcGetProgramLogSize(prog, &n);
@@ -193,7 +200,8 @@ int compile_code(const std::string& coef_source, const std::string& name_expr, C
 
 int lb_rtc_compile(const std::string& coef_source, const std::string& name_expr, hipModule_t* module,
                    hipFunction_t* function, std::string& why) {
-  const std::string key = name_expr + "\n" + coef_source;
+  const char* extra = std::getenv("CENTREX_LB_RTC_OPTIONS");
+  const std::string key = std::string(extra ? extra : "") + "\n" + name_expr + "\n" + coef_source;
   CodeObject obj;
   {
     std::lock_guard<std::mutex> lock(g_code_mu);

@@ -149,10 +149,23 @@ void lb_plan_destroy(lb_plan* plan);
  * (hiprtc, about 2 s) and later calls reuse it.  Values are bitwise those of
  * the expression machine.  A plan without time dependence is left as it is.
  * Returns LB_ERR_UNSUPPORTED (with the reason) when an instruction has no
- * generated form, LB_ERR_DEVICE from lb_solve when compilation fails.
+ * generated form.  When the run-time compilation or the module load of a
+ * variant fails, lb_solve runs the precompiled kernel instead (bitwise the
+ * same results), never tries that variant again for the plan, and records
+ * the reason for lb_plan_specialize_status.  The environment variable
+ * CENTREX_LB_RTC_OPTIONS, when set, is appended to the compile options
+ * (whitespace-separated; a diagnostic hook).
  * enable = 0 returns the plan to the precompiled kernels.
  */
 int lb_plan_specialize(lb_plan* plan, int32_t enable);
+
+/*
+ * Whether a specialised compilation of this plan has failed: returns 0 when
+ * none has, 1 when one has, with its reason copied (NUL-terminated,
+ * truncated to cap bytes) into buf when buf is not NULL.  No reference
+ * counterpart.
+ */
+int lb_plan_specialize_status(const lb_plan* plan, char* buf, int64_t cap);
 
 /* Thread-local message for the last failing call on this thread. */
 const char* lb_last_error(void);
@@ -362,8 +375,7 @@ int lb_adjoint(const lb_plan* plan, const lb_solve_args* args, void* stream);
  * streams forked from `stream` and joined back into it, so a HIP graph
  * captured on `stream` holds the whole uneven batch as one launch.  Each
  * group's args.workgroups bounds its share of the device (0 lets a group fill
- * the device, which serialises the groups).  A plan may appear in one group
- * only.  At most LB_MAX_GROUPS groups.
+ * the device, which serialises the groups).  At most LB_MAX_GROUPS groups.
  */
 #define LB_MAX_GROUPS 16
 

@@ -109,7 +109,7 @@ typedef struct {
   int uses_time;
 } expr_t;
 
-#define STACK_MAX 64
+#define STACK_MAX 1024
 static const double ED_XTB = 0.315 * 1.602176634e-19 * 5.29177210544e-11;
 
 static double rem_euclid(double x, double m) {
@@ -126,9 +126,156 @@ static double rabi_from_intensity(double intensity, double coupling, double dipo
   return e * coupling * dipole / hbar;
 }
 
+/* as_real (eval.rs:95-101): a helper's real argument. */
+static int as_real(cpx v, double* out) {
+  if (fabs(v.im) > 1e-12) return fail("complex value used where real scalar was required");
+  *out = v.re;
+  return 0;
+}
+
+/* multipass_2d_intensity (eval.rs:250-275): a = x, y, amplitudes[L],
+ * xlocs[L], ylocs[L], sigma_x, sigma_y. */
+static int multipass(const cpx* a, int L, double* out) {
+  double x, y, sx, sy, intensity = 0.0;
+  int rc = as_real(a[0], &x) || as_real(a[1], &y) || as_real(a[2 + 3 * L], &sx) || as_real(a[3 + 3 * L], &sy);
+  if (rc) return LB_ERR_INVALID;
+  for (int k = 0; k < L; ++k) {
+    double amp, xl, yl;
+    if (as_real(a[2 + k], &amp) || as_real(a[2 + L + k], &xl) || as_real(a[2 + 2 * L + k], &yl)) return LB_ERR_INVALID;
+    intensity += gaussian_2d(x, y, amp, xl, yl, sx, sy);
+  }
+  *out = intensity;
+  return 0;
+}
+
+/* linear_interp (eval.rs:360-388): a = x, grid[n], values[n]. */
+static int linear_interp(const cpx* a, int n, double* out) {
+  double x, first;
+  const cpx* grid = a + 1;
+  const cpx* values = a + 1 + n;
+  if (as_real(a[0], &x)) return LB_ERR_INVALID;
+  if (n == 1) return as_real(values[0], out);
+  if (as_real(grid[0], &first)) return LB_ERR_INVALID;
+  if (x <= first) return as_real(values[0], out);
+  for (int idx = 0; idx + 1 < n; ++idx) {
+    double left, right;
+    if (as_real(grid[idx], &left) || as_real(grid[idx + 1], &right)) return LB_ERR_INVALID;
+    if (right <= left) return fail("linear_interp grid must be strictly increasing");
+    if (x <= right) {
+      double yl, yr;
+      if (as_real(values[idx], &yl) || as_real(values[idx + 1], &yr)) return LB_ERR_INVALID;
+      double frac = (x - left) / (right - left);
+      *out = yl + frac * (yr - yl);
+      return 0;
+    }
+  }
+  return as_real(values[n - 1], out);
+}
+
+/* Slopes of a monotone cubic (pchip) on knots g, v (eval.rs:318-331,
+ * plan.rs:54-66); h and delta are the interval widths and secants. */
+static int pchip_slopes(const double* g, const double* v, int n, double* h, double* delta, double* d) {
+  for (int i = 0; i + 1 < n; ++i) {
+    h[i] = g[i + 1] - g[i];
+    if (h[i] <= 0.0) return fail("pchip_interp grid must be strictly increasing");
+    delta[i] = (v[i + 1] - v[i]) / h[i];
+  }
+  d[0] = delta[0];
+  d[n - 1] = delta[n - 2];
+  for (int i = 1; i + 1 < n; ++i) {
+    double s0 = delta[i - 1] < 0.0 ? -1.0 : 1.0, s1 = delta[i] < 0.0 ? -1.0 : 1.0;
+    if (s0 != s1 || delta[i - 1] == 0.0 || delta[i] == 0.0) {
+      d[i] = 0.0;
+    } else {
+      double w1 = 2.0 * h[i] + h[i - 1], w2 = h[i] + 2.0 * h[i - 1];
+      d[i] = (w1 + w2) / (w1 / delta[i - 1] + w2 / delta[i]);
+    }
+  }
+  return 0;
+}
+
+/* pchip_interp, Hermite form (eval.rs:290-351) when table == 0; through a
+ * precomputed table in Horner form (plan.rs:33-115) when table != 0.
+ * a = x, grid[n], values[n]. */
+static int pchip(const cpx* a, int n, int table, double* out) {
+  double x;
+  if (as_real(a[0], &x)) return LB_ERR_INVALID;
+  if (n < 2) return as_real(a[2], out);
+  double* buf = (double*)malloc(sizeof(double) * 5 * (size_t)n);
+  if (!buf) return fail("out of memory");
+  double *g = buf, *v = buf + n, *h = buf + 2 * n, *delta = buf + 3 * n, *d = buf + 4 * n;
+  for (int i = 0; i < n; ++i) {
+    g[i] = a[1 + i].re;
+    v[i] = a[1 + n + i].re;
+  }
+  int rc = pchip_slopes(g, v, n, h, delta, d);
+  if (rc == 0 && !table) {
+    if (x <= g[0]) {
+      *out = v[0];
+    } else if (x >= g[n - 1]) {
+      *out = v[n - 1];
+    } else {
+      int idx = 0;
+      for (int i = 0; i + 1 < n; ++i)
+        if (x < g[i + 1] || i == n - 2) { idx = i; break; }
+      double t = (x - g[idx]) / h[idx], t2 = t * t, t3 = t2 * t;
+      double h00 = 2.0 * t3 - 3.0 * t2 + 1.0, h10 = t3 - 2.0 * t2 + t;
+      double h01 = -2.0 * t3 + 3.0 * t2, h11 = t3 - t2;
+      *out = h00 * v[idx] + h10 * h[idx] * d[idx] + h01 * v[idx + 1] + h11 * h[idx] * d[idx + 1];
+    }
+  } else if (rc == 0) {
+    int i;
+    double dx;
+    if (x <= g[0]) {
+      i = 0;
+      dx = 0.0;
+    } else if (x >= g[n - 1]) {
+      i = n - 2;
+      dx = g[n - 1] - g[n - 2];
+    } else {
+      int pp = 0; /* partition_point(g <= x) */
+      while (pp < n && g[pp] <= x) ++pp;
+      i = pp - 1 < n - 2 ? pp - 1 : n - 2;
+      dx = x - g[i];
+    }
+    double c0 = v[i], c1 = d[i];
+    double c2 = (3.0 * delta[i] - 2.0 * d[i] - d[i + 1]) / h[i];
+    double c3 = (d[i] + d[i + 1] - 2.0 * delta[i]) / (h[i] * h[i]);
+    *out = (x <= g[0]) ? c0 : c0 + dx * (c1 + dx * (c2 + dx * c3));
+  }
+  free(buf);
+  return rc;
+}
+
 static int apply_helper(int id, const cpx* a, int argc, cpx* out) {
 #define R(i) (a[i].re)
+  double r = 0.0;
   switch (id) {
+    case 8:
+      if (argc < 4 || (argc - 4) % 3) return fail("multipass_2d_intensity with %d arguments", argc);
+      if (multipass(a, (argc - 4) / 3, &r)) return LB_ERR_INVALID;
+      *out = C(r, 0.0);
+      return 0;
+    case 10: {
+      if (argc < 5 || ((argc - 5) % 3 && (argc - 6) % 3)) return fail("multipass_2d_rabi with %d arguments", argc);
+      int L = (argc - 5) % 3 == 0 ? (argc - 5) / 3 : (argc - 6) / 3;
+      double coupling, dipole = ED_XTB;
+      if (multipass(a, L, &r) || as_real(a[4 + 3 * L], &coupling)) return LB_ERR_INVALID;
+      if (argc == 6 + 3 * L && as_real(a[5 + 3 * L], &dipole)) return LB_ERR_INVALID;
+      *out = C(rabi_from_intensity(r, coupling, dipole), 0.0);
+      return 0;
+    }
+    case 14:
+      if (argc < 3 || argc % 2 == 0) return fail("linear_interp with %d arguments", argc);
+      if (linear_interp(a, (argc - 1) / 2, &r)) return LB_ERR_INVALID;
+      *out = C(r, 0.0);
+      return 0;
+    case 16:
+    case 17:
+      if (argc < 3 || argc % 2 == 0) return fail("pchip_interp with %d arguments", argc);
+      if (pchip(a, (argc - 1) / 2, id == 17, &r)) return LB_ERR_INVALID;
+      *out = C(r, 0.0);
+      return 0;
     case 1:
       *out = C(gaussian_2d(R(0), R(1), R(2), R(3), R(4), R(5), R(6)), 0.0);
       return 0;

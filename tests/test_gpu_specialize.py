@@ -165,3 +165,37 @@ def test_first_call_inside_a_capture_takes_the_precompiled_kernel():
         torch.cuda.synchronize()
         _same(captured, want, "captured first call")
     _same(spec.solve(y0, 0.0, 6.0, **kw), want, "eager call after the capture")
+
+
+def test_failed_compilation_runs_the_precompiled_kernel(monkeypatch):
+    """A run-time compilation that fails (forced here through the compile
+    options hook: a macro redefinition made an error) leaves the solve on the
+    precompiled kernel with bitwise the same results, warns once with the
+    reason, and is not attempted again for the plan."""
+    torch = require_gpu()
+    system, params = _rich_two_level()
+    prepared = prepare_lindblad_problem(system, params, backend="gpu")
+    base = DevicePlan(prepared.flat_plan)
+    monkeypatch.setenv("CENTREX_LB_RTC_OPTIONS", "-Werror -DLB_PART=1")
+    spec = DevicePlan(prepared.flat_plan).specialize()
+    y0 = torch.as_tensor(prepared.layout.pack(np.diag([1.0, 0.0]).astype(complex)), device="cuda")
+    names = prepared.slot_names
+    ov = np.stack([np.linspace(0.3, 1.2, 24) + 0.1j, np.linspace(0.6, 2.0, 24) + 0j], axis=1)
+    kw = dict(output_kind="populations", weights=[(1, 0.3)], with_integral=True,
+              override_slots=[names.index("a"), names.index("w")], override_values=ov,
+              abstol=1e-10, reltol=1e-8, dt=1e-3)
+    want = base.solve(y0, 0.0, 6.0, **kw)
+    assert spec.specialize_failure() is None
+    with pytest.warns(UserWarning, match="precompiled kernel used"):
+        got = spec.solve(y0, 0.0, 6.0, **kw)
+    torch.cuda.synchronize()
+    _same(got, want, "failed compilation")
+    why = spec.specialize_failure()
+    assert why is not None and "lb_solve_pair_kernel" in why, why
+    import warnings
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")  # neither a second compile nor a second warning
+        again = spec.solve(y0, 0.0, 6.0, **kw)
+    torch.cuda.synchronize()
+    _same(again, want, "after the failure")
