@@ -159,15 +159,18 @@ def uneven_leg(steps: int, warmup: int, cpu_seconds: float) -> dict:
     return out
 
 
-def fit_leg(per_group: int, iterations: int, t_scale: float) -> dict:
+def fit_leg(per_group: int, iterations: int, t_scale: float, graph: bool = True) -> dict:
     """Batched gradient-based fit: per_group n=39 (x/y) + per_group n=72 (theta)
     trajectories, sigma ~ N(5, 1.5)/Gamma clipped above 1/Gamma (times t_scale),
     t_center = sigma, t_end = 2 sigma; fit per-trajectory (delta0, Omega0,
     theta) by Adam (lr 0.05 in units of Gamma, Gamma, 1) to photon counts of
     seeded true parameters, starting from perturbed ones; Dopri5, rtol 1e-8,
-    checkpoints every 16 accepted steps.  Reports gradients/s, the peak tape
-    memory of one iteration, and the final populations of a sample against the
-    CPU oracle at rtol 1e-8.
+    checkpoints every 16 accepted steps.  With ``graph`` every iteration after
+    the first is one replay of a HIP graph holding both systems' forward
+    solves, adjoints and the Adam step (statuses read once, after the last
+    replay).  Reports gradients/s, the tape memory, the loss curve, and the
+    final populations of a sample (4 trajectories per system, the shortest)
+    against the CPU oracle at rtol 1e-8.
     """
     import numpy as np
     import torch
@@ -195,7 +198,14 @@ def fit_leg(per_group: int, iterations: int, t_scale: float) -> dict:
         initial = true * np.asarray([1.0, 1.2, 1.0]) + np.asarray([0.3 * GAMMA, 0.0, 0.2])
         groups.append(FitGroup(prepared, rho0, t_end, targets.cpu().numpy(), initial, weights, fixed))
     log(f"bench: fit of {2 * per_group} trajectories x {iterations} iterations")
-    res = fit_photon_counts(groups, names, scales=[GAMMA, GAMMA, 1.0], iterations=iterations, lr=0.05, **tol)
+    def progress(it):
+        if it % 5 == 0:
+            torch.cuda.synchronize()
+            log(f"bench: fit iteration {it} done")
+
+    res = fit_photon_counts(groups, names, scales=[GAMMA, GAMMA, 1.0], iterations=iterations, lr=0.05,
+                            graph=graph, on_replay=progress, **tol)
+    log(f"bench: fit done in {res.seconds:.1f} s")
     # final populations of a sample against the oracle
     errs = []
     for grp, p in zip(groups, res.parameters):
@@ -206,7 +216,7 @@ def fit_leg(per_group: int, iterations: int, t_scale: float) -> dict:
         pops = plan.solve(y0, 0.0, grp.t_end, output_kind="populations", override_slots=slots,
                           override_values=vals.astype(complex), **tol)[0].cpu().numpy()
         oracle = Oracle(grp.prepared)
-        for b in np.argsort(grp.t_end)[:2]:
+        for b in np.argsort(grp.t_end)[:4]:
             r = oracle.solve_batch(y0, (0.0, float(grp.t_end[b])), output="populations", output_when="final",
                                    parameter_slots=names + ["sigma", "t_center"],
                                    parameter_batch=vals[b:b + 1].astype(complex), threads=1, **tol)
@@ -221,6 +231,8 @@ def fit_leg(per_group: int, iterations: int, t_scale: float) -> dict:
         "forward_seconds": res.forward_seconds,
         "adjoint_seconds": res.adjoint_seconds,
         "loss_first_last": [res.loss[0], res.loss[-1]],
+        "loss": res.loss,
+        "graph_replayed": graph,
         "tape_bytes_peak": res.tape_bytes,
         "final_population_error_vs_oracle": max(errs),
         "config": f"{per_group} tlf_n39_xy + {per_group} tlf_n72_theta, sigma scale {t_scale}, rtol 1e-8",
@@ -234,7 +246,8 @@ def adjoint_leg(steps: int, warmup: int) -> dict:
     (PX = cos theta, PY = sin theta), Gaussian envelope of width sigma.
     Parameters per trajectory (seeded): delta0 ~ U[-3, 3] Gamma, Omega0 ~
     U[0.3, 3] Gamma, theta ~ U[0, pi], sigma ~ U[2, 6] / Gamma; targets ~
-    U[0, 1].  Loss sum_b (N_b - target_b)^2, Dopri5 rtol 1e-8 atol 1e-10,
+    U[0, 1]; the initial state a seeded coherent superposition of the
+    X(J=0) levels.  Loss sum_b (N_b - target_b)^2, Dopri5 rtol 1e-8 atol 1e-10,
     checkpoints every 16 accepted steps.  One step = forward with tape +
     adjoint backward through ``torch.autograd``; the forward alone is the
     ordinary batched photon-count solve of the same trajectories.
@@ -254,7 +267,15 @@ def adjoint_leg(steps: int, warmup: int) -> dict:
                      rng.uniform(2, 6, n) / GAMMA], axis=1)
     dev = torch.device("cuda", 0)
     target = torch.as_tensor(rng.uniform(0, 1, n), device=dev)
-    system, params, rho0, t_span, weights = tlf_n39_xy()
+    system, params, _, t_span, weights = tlf_n39_xy()
+    # initial state: a seeded coherent superposition of the X(J=0) levels (an
+    # isotropic mixture makes dN/dtheta vanish by symmetry)
+    ground = system.indices(electronic="X", J=0)
+    psi = np.zeros(system.n_states, dtype=np.complex128)
+    crng = np.random.default_rng(7)
+    psi[ground] = crng.normal(size=len(ground)) + 1j * crng.normal(size=len(ground))
+    psi /= np.linalg.norm(psi)
+    rho0 = np.outer(psi, psi.conj())
     prepared = prepare_lindblad_problem(system, params, backend="gpu")
     engine = DifferentiablePhotonCount(prepared, names, weights, t_span, ckpt_every=16, **tol)
     y0 = torch.as_tensor(prepared.layout.pack(rho0), device=dev)
@@ -364,7 +385,7 @@ def adjoint_leg(steps: int, warmup: int) -> dict:
         "fd_trajectories_with_step_count_change": int((~same).sum()),
         "frozen_step_vs_fd_error": [float(e) for e in err_frozen],
         "frozen_step_vs_replay_fd_error": [float(e) for e in err_replay],
-        "config": "tlf_n39_xy, 256 seeded trajectories, Dopri5 rtol 1e-8 atol 1e-10",
+        "config": "tlf_n39_xy, 256 seeded trajectories, coherent X(J=0) initial state, Dopri5 rtol 1e-8 atol 1e-10",
     }
 
 
@@ -586,6 +607,8 @@ def main() -> int:
     ap.add_argument("--fit-trajectories", type=int, default=2048, help="trajectories per system in the fit")
     ap.add_argument("--fit-iterations", type=int, default=50)
     ap.add_argument("--fit-time-scale", type=float, default=1.0, help="scale of the transit times in the fit")
+    ap.add_argument("--fit-eager", dest="fit_graph", action="store_false",
+                    help="run the fit's iterations eagerly (default: replayed from a HIP graph)")
     ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
     ap.add_argument("--no-specialize", dest="specialize", action="store_false",
                     help="run the precompiled kernels (expression machine) instead of the plan-specialised ones")
@@ -624,7 +647,7 @@ def main() -> int:
         print(json.dumps(adjoint_leg(args.steps, args.warmup)))
         return 0
     if args.fit:
-        print(json.dumps(fit_leg(args.fit_trajectories, args.fit_iterations, args.fit_time_scale)))
+        print(json.dumps(fit_leg(args.fit_trajectories, args.fit_iterations, args.fit_time_scale, args.fit_graph)))
         return 0
     if args.uneven_only:
         print(json.dumps(uneven_leg(min(args.steps, 3), 1, args.cpu_seconds)))

@@ -125,6 +125,53 @@ class DifferentiablePhotonCount:
                     int(st[i]), float(fail_t[i].item()), maxiters=self.kw["maxiters"], n_plan=0)))
             return vals[:, 0], tape
 
+    def buffers(self, n_traj: int, y0) -> dict[str, Any]:
+        """Device buffers for :meth:`launch`: the packed initial state, its
+        support, a tape of the engine's capacity, the per-trajectory end times
+        and fixed parameter values on the device, and the gradient output."""
+        import torch
+
+        y = torch.as_tensor(np.asarray(y0, dtype=np.float64) if not torch.is_tensor(y0) else y0,
+                            dtype=torch.float64, device="cuda")
+        support = self.plan.support_of(y)
+        slots = self.plan.support_info(support)["slot_count"]
+        t1 = self.t_span[1]
+        if np.ndim(t1) > 0:
+            if len(t1) != n_traj:
+                raise ValueError("one end time per trajectory")
+            t1 = torch.as_tensor(np.asarray(t1, dtype=np.float64), device="cuda")
+        fixed = None
+        if self.fixed_values is not None:
+            if self.fixed_values.shape[0] != n_traj:
+                raise ValueError("fixed parameter values need one row per trajectory")
+            fixed = torch.as_tensor(self.fixed_values, device="cuda")
+        return {"y": y, "support": support, "t1": t1, "fixed": fixed,
+                "tape": Tape(n_traj, slots, self.capacity, self.every),
+                "grad": torch.empty((n_traj, len(self.slots)), dtype=torch.float64, device="cuda")}
+
+    def launch(self, p, buf: dict[str, Any]):
+        """Photon counts (B,), per-trajectory status (B,) and dI/dp (B, P) of
+        ``p`` (a (B, P) float64 device tensor): the forward solve recording
+        ``buf["tape"]``, then the adjoint.  Nothing is read back to the host
+        and nothing depends on a host value computed here, so a stream
+        capture of this call replays it exactly; statuses (LB_TRAJ_*, 7 when
+        the tape was too short) are for the caller to check."""
+        import torch
+
+        vals = p.to(torch.complex128)
+        slots = self.slots
+        if buf["fixed"] is not None:
+            vals = torch.cat([vals, buf["fixed"]], dim=1)
+            slots = self.slots + self.fixed_slots
+        tape = buf["tape"]
+        out, stats, _ = self.plan.solve(
+            buf["y"], self.t_span[0], buf["t1"], output_kind="integral", weights=self.weights,
+            override_slots=slots, override_values=vals, support=buf["support"], tape=tape, **self.kw)
+        self.plan.solve(buf["y"], self.t_span[0], buf["t1"], output_kind="integral", weights=self.weights,
+                        override_slots=slots, override_values=vals, support=buf["support"], tape=tape,
+                        adjoint_grad=buf["grad"], adjoint_mode=self.mode, **self.kw)
+        return out[:, 0], stats[:, 3], buf["grad"]
+
     def replay(self, params, y0, tape):
         """Photon counts (B,) integrated with the accepted step sizes recorded
         in ``tape`` (no step-size control): the map the frozen adjoint

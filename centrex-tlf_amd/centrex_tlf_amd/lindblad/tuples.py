@@ -36,7 +36,7 @@ from __future__ import annotations
 from collections.abc import Sequence
 from typing import Any
 
-from .ir import CompiledExpression
+from .ir import CompiledExpression, evaluate_expression
 from .ir import InstructionOp as Op
 from .postfix import BINARY, COMPARE, parse, program
 
@@ -70,6 +70,29 @@ def extract_pchip_tables(base_values: Sequence[Any]) -> list[tuple[tuple[float, 
 
 def _is_const(node: tuple) -> bool:
     return node[0] == "const"
+
+
+def _reads_input(node: tuple) -> bool:
+    """Whether a tree reads a slot or the time."""
+    kind = node[0]
+    if kind in ("slot", "time"):
+        return True
+    if kind == "un":
+        return _reads_input(node[2])
+    if kind == "bin":
+        return _reads_input(node[2]) or _reads_input(node[3])
+    if kind == "call":
+        return any(_reads_input(a) for a in node[3])
+    return False
+
+
+def _constant_value(node: tuple) -> float | None:
+    """The real part of a tree that reads no slot and no time, or None."""
+    if _is_const(node):
+        return node[1]
+    if _reads_input(node):
+        return None
+    return complex(evaluate_expression(CompiledExpression(tuple(program(node))), [], 0.0)).real
 
 
 def _real_const(node: tuple, what: str) -> float:
@@ -166,19 +189,22 @@ class _Lowering:
             raise ValueError(f"{name} grid and values must have the same length")
         if not grid:
             raise ValueError(f"{name} grid must contain at least one point")
-        constant = all(_is_const(e) for e in grid)
-        if constant and len(grid) > 1:
-            g = [_real_const(e, name) if function == 14 else e[1] for e in grid]
-            if any(b <= a for a, b in zip(g, g[1:])):
+        if function == 14:
+            for e in grid:
+                if _is_const(e):
+                    _real_const(e, name)
+        points = [_constant_value(e) for e in grid]  # None where the point is not constant
+        if len(grid) > 1 and all(v is not None for v in points):
+            if any(b <= a for a, b in zip(points, points[1:])):
                 raise ValueError(f"{name} grid must be strictly increasing")
         if function == 16 and self.tables:
             same_length = [t for t in self.tables if len(t[0]) == len(grid)]
-            if same_length and not constant:
+            if same_length and any(v is None for v in points):
                 raise NotImplementedError(
                     "pchip_interp with a grid that depends on time or batch parameters, where the "
                     "problem's tuples define interpolation tables, is not lowered for the GPU")
             for tgrid, tvalues in same_length:
-                if all(abs(a - e[1]) < 1e-15 for a, e in zip(tgrid, grid)):
+                if all(abs(a - b) < 1e-15 for a, b in zip(tgrid, points)):
                     return call(PCHIP_TABLE_HELPER, [x, *[("const", v, 0.0) for v in tgrid],
                                                      *[("const", v, 0.0) for v in tvalues]])
         return call(function, [x, *grid, *values])

@@ -1220,7 +1220,7 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
   const bool event = a->event_kind != LB_EVENT_NONE;
   if (event || a->solver >= LB_SOLVER_FIXED_DOPRI5) return launch_table(p, a, im, c, stream, event, ev_depth);
   int id, W;
-  const bool pair_ok = a->mode == LB_MODE_PERSISTENT && !a->trace && a->n_saveat == 0 && !a->tape_h &&
+  const bool pair_ok = a->mode == LB_MODE_PERSISTENT && !a->trace && a->n_saveat == 0 &&
                        !a->replay_h && std::isfinite(a->dt) && a->dt > 0.0 && a->output_kind != LB_OUT_RATE;
   if ((rc = resolve_config(im, a->variant, a->threads_per_traj, pair_ok, &id, &W))) return rc;
   const bool pair = lb_dev_variant_pair(id) != 0;

@@ -1839,8 +1839,9 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_fixed_kernel(const DevPlan P, 
 // inside the divergent halves).
 //
 // Covered: persistent launches with pushes at every accepted step (final
-// outputs and the weighted integral), a given initial step dt; trace, save
-// plans, tapes and replays take the other kernels.
+// outputs and the weighted integral), a given initial step dt, and the
+// adjoint's attempt tape; trace, save plans and replays take the other
+// kernels.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -1866,27 +1867,45 @@ __device__ __forceinline__ double half_sum(double x, int lane) {
 // only non-zero lane of a second wave sum.
 __device__ __forceinline__ double half_total(double x, int lane) { return half_sum(x, lane) + 0.0; }
 
+// gather_apply without branches: the coefficient is loaded for every term
+// (entry 0 for a static one) and the static term's factor selected after,
+// so the values are gather_apply's.
+__device__ __forceinline__ void gather_apply_sel(uint32_t w, double2 f, const dcpx* coef, double x_r,
+                                                 double x_raw_i, double& accr, double& acci) {
+  const uint32_t cid = w >> 24;
+  const bool stat = cid == LB_STATIC_COEF;
+  dcpx c = coef[stat ? 0u : cid];
+  c.im = (w & (1u << 22)) ? -c.im : c.im;
+  const double pr = fma(f.x, c.re, -(f.y * c.im));
+  const double pi = fma(f.x, c.im, f.y * c.re);
+  const double vr = stat ? f.x : pr, vi = stat ? f.y : pi;
+  double x_i = (w & (1u << 20)) ? 0.0 : x_raw_i;
+  x_i = (w & (1u << 21)) ? -x_i : x_i;
+  accr = fma(-vi, x_i, fma(vr, x_r, accr));
+  acci = fma(vi, x_r, fma(vr, x_i, acci));
+}
+
 // The single-wave RHS for the element in slot `sl` of this half: inputs from
-// the lanes of the same half.
+// the lanes of the same half.  The gather words come in registers (`w`);
+// a kernel compiled for one plan (LB_GEN_COEF) holds exactly its rows, so
+// its loops have no row checks.
 template <int G>
-__device__ __forceinline__ void rhs_pair(const uint32_t* gw, const double2* gf, int rows, int sl, int base,
+__device__ __forceinline__ void rhs_pair(const uint32_t (&w)[G], const double2* gf, int rows, int sl, int base,
                                          double2 selfv, bool diag, double xr, double xi, double& kr, double& ki,
                                          const dcpx* coef) {
+#ifdef LB_GEN_COEF
+  (void)rows;
+  constexpr bool exact = true;
+#else
+  constexpr bool exact = false;
+#endif
   double accr = fma(selfv.x, xr, -(selfv.y * xi));
   double acci = fma(selfv.x, xi, selfv.y * xr);
-  uint32_t w[G];
   double2 f[G];
   double sr[G], si[G];
 #pragma unroll
   for (int j = 0; j < G; ++j) {
-    if (j < rows) {
-      w[j] = gw[j * 64 + sl];
-      f[j] = gf[j * 64 + sl];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < G; ++j) {
-    if (j < rows) {
+    if (exact || j < rows) {
       const int src = (int)(w[j] & 31u) | base;
       sr[j] = __shfl(xr, src, 64);
       si[j] = __shfl(xi, src, 64);
@@ -1894,7 +1913,10 @@ __device__ __forceinline__ void rhs_pair(const uint32_t* gw, const double2* gf, 
   }
 #pragma unroll
   for (int j = 0; j < G; ++j)
-    if (j < rows) gather_apply(w[j], f[j], coef, sr[j], si[j], accr, acci);
+    if (exact || j < rows) f[j] = gf[j * 64 + sl];
+#pragma unroll
+  for (int j = 0; j < G; ++j)
+    if (exact || j < rows) gather_apply_sel(w[j], f[j], coef, sr[j], si[j], accr, acci);
   kr = accr;
   ki = diag ? 0.0 : acci;
 }
@@ -1963,6 +1985,9 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
   }
   wave_sync();
   const double my_w = (S.valid(0) && diag) ? L.wts[S.state(0)] : 0.0;
+  uint32_t gwr[G];  // this lane's gather words, for the whole launch
+#pragma unroll
+  for (int j = 0; j < G; ++j) gwr[j] = j < grows ? L.gw[j * 64 + sl] : 0u;
 
   // per-half trajectory state
   bool active = false, done = false;
@@ -2067,7 +2092,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
         if (t1 == C.t0) {
           last = true;
         } else {
-          rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], coef);
+          rhs_pair<G>(gwr, L.gf, grows, sl, base, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], coef);
           h = fmin(C.dt, t1 - C.t0);
           rhs_calls = 1;
         }
@@ -2117,7 +2142,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
       double yt_r[1], yt_i[1], yn_r[1], yn_i[1];
 #define LB_PAIR_STAGE(st)                                                                            \
   stage_combo<1, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                                    \
-  rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0],   \
+  rhs_pair<G>(gwr, L.gf, grows, sl, base, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0],   \
               coef + (st) * KK);
       LB_PAIR_STAGE(2)
       LB_PAIR_STAGE(3)
@@ -2131,7 +2156,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
         const double a0 = yn_r[0] - yt_r[0], a1 = yn_i[0] - yt_i[0];
         stiff_d = a0 * a0 + a1 * a1;
       }
-      rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, yn_r[0], yn_i[0], k_r[7][0], k_i[7][0], coef + 7 * KK);
+      rhs_pair<G>(gwr, L.gf, grows, sl, base, selfv, diag, yn_r[0], yn_i[0], k_r[7][0], k_i[7][0], coef + 7 * KK);
       rhs_calls += 6;
       // ---- error norm, integrand, stiffness (as the single-wave form)
       double er = T::e(1) * k_r[1][0], ei = T::e(1) * k_i[1][0];
@@ -2154,8 +2179,48 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
       const double pw = pow(sl < 16 ? err : fac_old, sl < 16 ? alpha : -beta);
       const double fac11 = __shfl(pw, base, 64), fo = __shfl(pw, base + 16, 64);
       double fac = fac11 * fo;
-      fac = fmax(facc2, fmin(facc1, fac / safety));
+      const double fac_s = fac / safety;
+      fac = fmax(facc2, fmin(facc1, fac_s));
       double hn = h / fac;
+      if (C.tape_h) {
+        // attempt tape, as the single-wave form records it: one record per
+        // attempt by lane 0 of the half, the state every ckpt_every-th attempt
+        const int64_t a = ns - 1;
+        if (a < C.tape_cap) {
+          if (sl == 0) {
+            int fl = finishing ? LB_TAPE_CLIPPED : 0;
+            double h_next;
+            if (err <= 1.0) {
+              h_next = hn;
+              fl |= LB_TAPE_ACCEPTED | (accepted == 0 ? LB_TAPE_FIRST : 0) | (err > 1.0e-4 ? LB_TAPE_ERRQ : 0);
+              if (!(fac_s > facc2 && fac_s < facc1)) fl |= LB_TAPE_CLAMPED;
+              double hc = hn;
+              if (fabs(hc) > hmax) {
+                hc = hmax;
+                fl |= LB_TAPE_HMAX;
+              }
+              if (reject && fabs(h) < fabs(hc)) fl |= LB_TAPE_MINRULE;
+            } else {
+              const double fr = fac11 / safety;
+              h_next = h / fmin(facc1, fr);
+              if (!(fr < facc1)) fl |= LB_TAPE_CLAMPED;
+            }
+            const int64_t o = traj * C.tape_cap + a;
+            C.tape_t[o] = x;
+            C.tape_h[o] = h;
+            C.tape_c[3 * o] = err;
+            C.tape_c[3 * o + 1] = h_next;
+            C.tape_c[3 * o + 2] = fac_old;
+            C.tape_f[o] = fl;
+          }
+          if (a % C.ckpt_every == 0 && sl < P.S) {
+            const int64_t ncap = (C.tape_cap + C.ckpt_every - 1) / C.ckpt_every;
+            double* ck = C.tape_y + ((traj * ncap + a / C.ckpt_every) * P.S) * 2;
+            ck[2 * sl] = y_r[0];
+            ck[2 * sl + 1] = y_i[0];
+          }
+        }
+      }
       if (err <= 1.0) {
         fac_old = fmax(err, 1.0e-4);
         if (fabs(hn) > hmax) hn = hmax;
@@ -2206,6 +2271,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
     }
     // ---- a finished trajectory writes its outputs (the single-wave form's)
     if (active && last) {
+      if (status == 0 && C.tape_h && ns > C.tape_cap) status = LB_TRAJ_CAPACITY;
       const bool ok = status == 0;
       if (P.restricted) {
         if (C.output_kind == LB_OUT_POPULATIONS) {
@@ -2239,6 +2305,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
         }
       }
       if (sl == 0) {
+        if (C.tape_n) C.tape_n[traj] = (int32_t)ns;
         if (C.output_kind == LB_OUT_INTEGRAL) C.out_values[traj] = ok ? integral : nanv;
         if (C.out_integral) C.out_integral[traj] = ok ? integral : nanv;
         C.out_stats[4 * traj + 0] = accepted;

@@ -104,6 +104,13 @@ std::string lb_rtc_coef_source(const LbProgram& prog, const std::vector<int32_t>
           break;
         }
         case 13: {
+          // a long tuple argument list stays on the expression machine (its
+          // array would live in the kernel's private memory)
+          if (need > 16) {
+            why = "helper " + std::to_string(q.function) + " with " + std::to_string(need) +
+                  " arguments has no generated form";
+            return "";
+          }
           const std::string arr = "a" + v;
           o << "const dcpx " << arr << "[" << need << "] = {";
           for (size_t j = st.size() - need; j < st.size(); ++j) o << (j + need > st.size() ? ", " : "") << st[j];

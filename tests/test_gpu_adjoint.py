@@ -219,3 +219,33 @@ def test_adjoint_n72_matches_finite_differences():
     assert np.all(err_r <= 1e-7), err_r
     assert np.all(err <= 1e-5), err
     assert np.abs(fd_r[:, 2]).max() > 1e-4 * np.abs(counts.cpu().numpy()).max()  # theta matters here
+
+
+def test_paired_tape_is_the_single_wave_tape(setup):
+    """The paired kernel (two trajectories per wave, the default for this
+    plan) records the attempt tape bit for bit as the single-wave kernel
+    does: records, checkpoints, attempt counts and photon counts."""
+    from centrex_tlf_amd.lindblad.native import Tape
+
+    torch, prepared, _, t_span, weights = setup
+    engine = _engine(prepared, t_span, weights)
+    rho0 = tlf_n39_xy()[2]  # the isotropic ground state: a restricted plan of <= 32 elements
+    y0 = prepared.layout.pack(rho0)
+    p = _batch(37, seed=13)  # an odd batch: one half-wave ends without a partner
+    pt, yt, support, slots = engine._common(p, y0)
+    assert engine.plan.launch_info(support)["variant"] in (16, 17)  # the paired variants, 1-based
+    ov = engine._overrides(pt)
+    tapes, outs = [], []
+    for variant in (0, 14):  # 0: the library's choice (paired); 14: the single-wave variant with 16 rows
+        tape = Tape(len(p), slots, 4096, 16)
+        out = engine.plan.solve(yt, *t_span, output_kind="integral", weights=weights, override_slots=ov[0],
+                                override_values=ov[1], support=support, tape=tape, variant=variant, **engine.kw)
+        tapes.append(tape)
+        outs.append(out)
+    torch.cuda.synchronize()
+    assert int(tapes[0].n.min()) > 20
+    for name in ("n", "t", "h", "c", "f", "y"):
+        a, b = getattr(tapes[0], name), getattr(tapes[1], name)
+        assert torch.equal(a, b), name
+    for a, b in zip(outs[0], outs[1]):
+        assert torch.equal(a.nan_to_num(7.0), b.nan_to_num(7.0))

@@ -149,3 +149,34 @@ def test_failure_names_group_and_trajectory(systems):
     groups = _groups(systems, (4, 2), seed=17)
     with pytest.raises(ValueError, match=r"group 0, trajectory \d+: Stopped at x=.*Need more than 6 steps"):
         solve_uneven_batch(groups, tuning=None, abstol=1e-10, reltol=1e-8, dt=1e-8, maxiters=5)
+
+
+def test_every_multi_wave_variant_is_bitwise_default(systems):
+    """Each multi-wave solver variant that covers the restricted n=72 plan
+    (slots per lane 1 to 12, one to sixteen waves) gives bitwise the default
+    launch's results on short transits.  The variants the autotuner may pick
+    are exactly these, so a variant that rounds or orders differently fails
+    here rather than inside a tuned batch."""
+    from centrex_tlf_amd.lindblad.native import kernel_variants
+
+    prepared, rho0, weights = systems[1]
+    plan = prepared.device_plan()
+    y0 = prepared.layout.pack(rho0)
+    pb, _ = _rows(6, 11)
+    slots = [prepared.slot_names.index(n) for n in NAMES]
+    support = plan.support_of(y0)
+    kw = dict(output_kind="populations", weights=weights, with_integral=True, override_slots=slots,
+              override_values=pb, support=support, **TOL)
+    t_end = 0.6 / GAMMA
+    want = [t.cpu().numpy() for t in plan.solve(y0, 0.0, t_end, **kw)]
+    S = plan.support_info(support)["slot_count"]
+    checked = []
+    for v in kernel_variants():  # the single-wave and paired forms cover 64 slots and are skipped
+        if v["slots_per_thread"] * v["max_threads"] < S:
+            continue
+        print(f"variant {v}", flush=True)
+        got = [t.cpu().numpy() for t in plan.solve(y0, 0.0, t_end, variant=v["variant"], **kw)]
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(np.nan_to_num(a, nan=7.0), np.nan_to_num(b, nan=7.0), err_msg=str(v))
+        checked.append(v["variant"])
+    assert len(checked) >= 8, checked
