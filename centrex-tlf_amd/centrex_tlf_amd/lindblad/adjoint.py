@@ -186,11 +186,13 @@ class DifferentiablePhotonCount:
             raise ValueError("replay of the recorded step sizes failed")
         return vals[:, 0]
 
-    def gradient(self, params, y0, tape, variant: int = 0, events=None, mode: str | None = None):
+    def gradient(self, params, y0, tape, variant: int = 0, events=None, mode: str | None = None,
+                 noalloc: bool = False):
         """dI_b/dp_b (B, P) for a tape recorded by ``forward`` with the same params.
 
         ``variant`` = k + 1 forces adjoint kernel variant k (0: the library's
-        choice); ``mode`` overrides the engine's ("adaptive" or "frozen").
+        choice); ``mode`` overrides the engine's ("adaptive" or "frozen");
+        ``noalloc`` uses ``lb_adjoint_noalloc`` (no device allocation).
         """
         import torch
 
@@ -199,7 +201,7 @@ class DifferentiablePhotonCount:
         return self.plan.solve(
             y, *self.t_span, output_kind="integral", weights=self.weights, override_slots=self._overrides(p)[0],
             override_values=self._overrides(p)[1], support=support, tape=tape, adjoint_grad=grad,
-            variant=int(variant), events=events, adjoint_mode=mode or self.mode, **self.kw)
+            variant=int(variant), events=events, adjoint_mode=mode or self.mode, noalloc=noalloc, **self.kw)
 
 
 def _autograd_function():

@@ -71,6 +71,8 @@ def library() -> C.CDLL:
     L.lb_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.lb_adjoint.restype = C.c_int
     L.lb_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.lb_adjoint_noalloc.restype = C.c_int
+    L.lb_adjoint_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.lb_launch_info.restype = C.c_int
     L.lb_launch_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     L.lb_solve_group.restype = C.c_int
@@ -291,7 +293,7 @@ class DevicePlan:
               restrict_support: bool = True, support=None, events=None,
               save_plan=None, trace: bool = False, capacity: int = 0, with_times: bool = False,
               tape=None, adjoint_grad=None, replay=None, queue_order=None, stop_event=None,
-              adjoint_mode: str = "adaptive", _launch: bool = True):
+              adjoint_mode: str = "adaptive", noalloc: bool = False, _launch: bool = True):
         """Integrate a batch; returns (values, stats, fail_t) torch tensors on the device.
 
         ``y0`` is (dim,) shared or (B, dim).  Overrides are either per
@@ -318,6 +320,8 @@ class DevicePlan:
         with_time=True)``), "frozen" holds the accepted step sizes fixed.
         ``replay`` (a :class:`Tape` or :class:`Schedule`) integrates with the
         accepted step sizes of a recorded tape instead of the controller.
+        ``noalloc`` runs the adjoint through ``lb_adjoint_noalloc`` (its
+        workspace in the library's fixed device array; no allocation).
 
         ``stop_event`` (a spec from ``events.normalize_stop_event``) ends each
         trajectory at the first root of its event function; the root times
@@ -501,7 +505,8 @@ class DevicePlan:
             events[0].record(stream)
         if adjoint_grad is not None:
             a.out_grad = _dptr(adjoint_grad)
-            _check(library().lb_adjoint(self._handle, C.byref(a), sp))
+            entry = library().lb_adjoint_noalloc if noalloc else library().lb_adjoint
+            _check(entry(self._handle, C.byref(a), sp))
             if events is not None:
                 events[1].record(stream)
             return adjoint_grad

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "lb_device.h"
+#include "lb_internal.h"
 #include "lb_program.h"
 #include "lb_rtc.h"
 
@@ -759,7 +760,7 @@ extern "C" int lb_plan_info(const lb_plan* p, int64_t* info) {
 }
 
 namespace {
-int select_image(lb_plan* p, const uint8_t* y0_support, const DevImage** out);
+int select_image(lb_plan* p, const uint8_t* y0_support, const DevImage** out, bool may_build = true);
 }
 
 extern "C" int lb_plan_support_info(const lb_plan* p, const uint8_t* y0_support, int64_t* info) {
@@ -850,8 +851,8 @@ int fill_overrides(const lb_plan* p, int32_t n, const int32_t* slots, int32_t* d
 
 // The device image for a solve: the full plan, or (with an initial-support
 // mask over packed indices) the plan restricted to that support's closure,
-// built once and cached.
-int select_image(lb_plan* p, const uint8_t* y0_support, const DevImage** out) {
+// built on first use (unless `may_build` is false) and cached.
+int select_image(lb_plan* p, const uint8_t* y0_support, const DevImage** out, bool may_build) {
   *out = p->full.get();
   if (!y0_support) return LB_OK;
   const PlanSource& S = p->src;
@@ -866,6 +867,9 @@ int select_image(lb_plan* p, const uint8_t* y0_support, const DevImage** out) {
     *out = it->second.get();
     return LB_OK;
   }
+  if (!may_build)
+    return fail(LB_ERR_INVALID,
+                "no device tables for this y0 support yet (an lb_solve with the same support builds them)");
   std::unique_ptr<DevImage> im(new DevImage());
   int rc = build_image(S, act, im.get());
   if (rc != LB_OK) {
@@ -910,7 +914,8 @@ namespace {
 
 // Validate a solve request and fill the kernel's call record; picks the
 // device image (full or restricted to the y0 support closure).
-int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out, DevCall* cp) {
+int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out, DevCall* cp,
+              bool may_build = true) {
   if (!a->y0 || !a->out_values || !a->out_stats || !a->out_fail_t)
     return fail(LB_ERR_INVALID, "y0 and output buffers are required");
   if (a->maxiters <= 0) return fail(LB_ERR_INVALID, "maxiters must be positive");
@@ -920,7 +925,7 @@ int fill_call(const lb_plan* p, const lb_solve_args* a, const DevImage** im_out,
     return fail(LB_ERR_INVALID, "t_span must be finite");
   if (a->n_traj > INT32_MAX) return fail(LB_ERR_UNSUPPORTED, "at most 2^31-1 trajectories per call");
   const DevImage* im = nullptr;
-  int rc = select_image(const_cast<lb_plan*>(p), a->y0_support, &im);
+  int rc = select_image(const_cast<lb_plan*>(p), a->y0_support, &im, may_build);
   if (rc) return rc;
   DevCall& c = *cp;
   c = DevCall{};
@@ -1309,7 +1314,14 @@ int64_t lds_bytes_adj_wave(const DevPlan& P, const std::vector<int32_t>& g_rows,
 
 }  // namespace
 
-extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream) {
+// The shared core of lb_adjoint and lb_adjoint_noalloc.  With scratch ==
+// nullptr the workspace is the image's grow-only allocation and the queue
+// counter comes from the plan's ring; otherwise `scratch` (device,
+// scratch_bytes) holds the counter in its first 256 bytes and the workspace
+// after it, the grid shrinks to the workgroups whose workspace fits, and
+// nothing is allocated or recorded.
+int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, void* scratch,
+                      size_t scratch_bytes) {
   if (!p || !a) return fail(LB_ERR_INVALID, "null argument");
   if (a->n_traj <= 0) return LB_OK;
   if (p->src.n_diff <= 0) return fail(LB_ERR_INVALID, "the plan has no differentiated parameters");
@@ -1322,7 +1334,7 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   if (a->event_kind != LB_EVENT_NONE) return fail(LB_ERR_UNSUPPORTED, "the adjoint takes no stop event");
   const DevImage* im = nullptr;
   DevCall c{};
-  int rc = fill_call(p, a, &im, &c);
+  int rc = fill_call(p, a, &im, &c, scratch == nullptr);
   if (rc) return rc;
   c.out_grad = a->out_grad;
   if (a->ckpt_every <= 0) return fail(LB_ERR_INVALID, "ckpt_every must be positive");
@@ -1378,6 +1390,27 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   const size_t per_y = (size_t)a->ckpt_every * 8 * S * 2;
   const size_t per_c = (size_t)a->ckpt_every * 7 * (im->dev.KD + 1) * 2;
   hipStream_t s = (hipStream_t)stream;
+  if (scratch) {
+    const size_t head = 256, per_wg = (per_y + per_c) * sizeof(double);
+    if (scratch_bytes < head + per_wg)
+      return fail(LB_ERR_UNSUPPORTED, "the no-allocation adjoint needs " + std::to_string(per_wg) +
+                                          " bytes of workspace per workgroup (ckpt_every " +
+                                          std::to_string(a->ckpt_every) + "); its scratch holds " +
+                                          std::to_string(scratch_bytes - head));
+    grid = (int)std::min<size_t>((size_t)grid, (scratch_bytes - head) / per_wg);
+    char* base = static_cast<char*>(scratch);
+    c.queue = reinterpret_cast<int32_t*>(base);
+    c.seg_y = reinterpret_cast<double*>(base + head);
+    c.seg_c = c.seg_y + (size_t)grid * per_y;
+    DevPlan P = im->dev;
+    c.traj_offset = 0;
+    void* rargs[] = {&c.queue};  // the counter starts at zero (a fill kernel, no memset)
+    HIP_TRY(hipLaunchKernel(lb_dev_queue_reset_kernel(), dim3(1), dim3(1), rargs, 0, s));
+    void* args[] = {&P, &c};
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(W), args, lds, s));
+    HIP_TRY(hipGetLastError());
+    return LB_OK;
+  }
   const size_t need_y = (size_t)grid * per_y, need_c = (size_t)grid * per_c;
   bool cap = false;
   if ((rc = is_capturing(s, &cap))) return rc;
@@ -1419,6 +1452,12 @@ extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream
   }
   return const_cast<lb_plan*>(p)->queues.retire(slot, s);
 }
+
+extern "C" int lb_adjoint(const lb_plan* p, const lb_solve_args* a, void* stream) {
+  return lb_adjoint_launch(p, a, stream, nullptr, 0);
+}
+
+int lb_set_error(int code, const std::string& msg) { return fail(code, msg); }
 
 namespace {
 

@@ -369,6 +369,21 @@ int lb_solve(const lb_plan* plan, const lb_solve_args* args, void* stream);
 int lb_adjoint(const lb_plan* plan, const lb_solve_args* args, void* stream);
 
 /*
+ * lb_adjoint with no device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc).  The work-queue counter and
+ * the adjoint's workspace live in a fixed device array of
+ * LB_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; the launch
+ * uses as many persistent workgroups as fit in it (fewer than lb_adjoint may
+ * use, with the same results).  The plan's tables for args->y0_support must
+ * exist already (the lb_solve that recorded the tape builds them).  Calls
+ * that overlap on one GPU share the array and race.  LB_ERR_UNSUPPORTED when
+ * one workgroup's workspace (ckpt_every attempts) exceeds the array.
+ */
+#define LB_NOALLOC_SCRATCH_BYTES (64u << 20)
+int lb_adjoint_noalloc(const lb_plan* plan, const lb_solve_args* args, void* stream);
+
+/*
  * Several plans in one call (no reference counterpart: the reference solves
  * one plan per call, batch.py:209).  Group g integrates args[g] with
  * plans[g] as lb_solve would; the groups run concurrently on library-owned
