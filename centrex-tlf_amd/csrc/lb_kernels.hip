@@ -486,16 +486,18 @@ __device__ __forceinline__ Lds carve(const DevPlan& P) {
 // the input element (x_r, x_raw_i) already fetched.
 __device__ __forceinline__ void gather_apply(uint32_t w, double2 f, const dcpx* coef, double x_r,
                                              double x_raw_i, double& accr, double& acci) {
+  // A static term uses its factor as it is.  The coefficient is loaded for
+  // every term (entry 0 for a static one) and the factor selected after, so
+  // no lane branches and the load issues with the others.
   const uint32_t cid = w >> 24;
-  double vr = f.x, vi = f.y;
-  if (cid != LB_STATIC_COEF) {
-    dcpx c = coef[cid];
-    if (w & (1u << 22)) c.im = -c.im;
-    vr = fma(f.x, c.re, -(f.y * c.im));
-    vi = fma(f.x, c.im, f.y * c.re);
-  }
+  const bool stat = cid == LB_STATIC_COEF;
+  dcpx c = coef[stat ? 0u : cid];
+  c.im = (w & (1u << 22)) ? -c.im : c.im;
+  const double pr = fma(f.x, c.re, -(f.y * c.im));
+  const double pi = fma(f.x, c.im, f.y * c.re);
+  const double vr = stat ? f.x : pr, vi = stat ? f.y : pi;
   double x_i = (w & (1u << 20)) ? 0.0 : x_raw_i;
-  if (w & (1u << 21)) x_i = -x_i;
+  x_i = (w & (1u << 21)) ? -x_i : x_i;
   // explicit fused operations: the library is built without implicit
   // contraction, so every kernel variant rounds identically
   accr = fma(-vi, x_i, fma(vr, x_r, accr));
@@ -1867,30 +1869,13 @@ __device__ __forceinline__ double half_sum(double x, int lane) {
 // only non-zero lane of a second wave sum.
 __device__ __forceinline__ double half_total(double x, int lane) { return half_sum(x, lane) + 0.0; }
 
-// gather_apply without branches: the coefficient is loaded for every term
-// (entry 0 for a static one) and the static term's factor selected after,
-// so the values are gather_apply's.
-__device__ __forceinline__ void gather_apply_sel(uint32_t w, double2 f, const dcpx* coef, double x_r,
-                                                 double x_raw_i, double& accr, double& acci) {
-  const uint32_t cid = w >> 24;
-  const bool stat = cid == LB_STATIC_COEF;
-  dcpx c = coef[stat ? 0u : cid];
-  c.im = (w & (1u << 22)) ? -c.im : c.im;
-  const double pr = fma(f.x, c.re, -(f.y * c.im));
-  const double pi = fma(f.x, c.im, f.y * c.re);
-  const double vr = stat ? f.x : pr, vi = stat ? f.y : pi;
-  double x_i = (w & (1u << 20)) ? 0.0 : x_raw_i;
-  x_i = (w & (1u << 21)) ? -x_i : x_i;
-  accr = fma(-vi, x_i, fma(vr, x_r, accr));
-  acci = fma(vi, x_r, fma(vr, x_i, acci));
-}
-
 // The single-wave RHS for the element in slot `sl` of this half: inputs from
-// the lanes of the same half.  The gather words come in registers (`w`);
-// a kernel compiled for one plan (LB_GEN_COEF) holds exactly its rows, so
-// its loops have no row checks.
+// the lanes of the same half.  A kernel compiled for one plan (LB_GEN_COEF)
+// holds exactly its rows, so its loops have no row checks; the words are
+// loaded first, then every permute, then the factors and coefficients, so
+// each group of loads issues together.
 template <int G>
-__device__ __forceinline__ void rhs_pair(const uint32_t (&w)[G], const double2* gf, int rows, int sl, int base,
+__device__ __forceinline__ void rhs_pair(const uint32_t* gw, const double2* gf, int rows, int sl, int base,
                                          double2 selfv, bool diag, double xr, double xi, double& kr, double& ki,
                                          const dcpx* coef) {
 #ifdef LB_GEN_COEF
@@ -1901,8 +1886,12 @@ __device__ __forceinline__ void rhs_pair(const uint32_t (&w)[G], const double2* 
 #endif
   double accr = fma(selfv.x, xr, -(selfv.y * xi));
   double acci = fma(selfv.x, xi, selfv.y * xr);
+  uint32_t w[G];
   double2 f[G];
   double sr[G], si[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j)
+    if (exact || j < rows) w[j] = gw[j * 64 + sl];
 #pragma unroll
   for (int j = 0; j < G; ++j) {
     if (exact || j < rows) {
@@ -1916,7 +1905,7 @@ __device__ __forceinline__ void rhs_pair(const uint32_t (&w)[G], const double2* 
     if (exact || j < rows) f[j] = gf[j * 64 + sl];
 #pragma unroll
   for (int j = 0; j < G; ++j)
-    if (exact || j < rows) gather_apply_sel(w[j], f[j], coef, sr[j], si[j], accr, acci);
+    if (exact || j < rows) gather_apply(w[j], f[j], coef, sr[j], si[j], accr, acci);
   kr = accr;
   ki = diag ? 0.0 : acci;
 }
@@ -1985,9 +1974,6 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
   }
   wave_sync();
   const double my_w = (S.valid(0) && diag) ? L.wts[S.state(0)] : 0.0;
-  uint32_t gwr[G];  // this lane's gather words, for the whole launch
-#pragma unroll
-  for (int j = 0; j < G; ++j) gwr[j] = j < grows ? L.gw[j * 64 + sl] : 0u;
 
   // per-half trajectory state
   bool active = false, done = false;
@@ -2092,7 +2078,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
         if (t1 == C.t0) {
           last = true;
         } else {
-          rhs_pair<G>(gwr, L.gf, grows, sl, base, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], coef);
+          rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], coef);
           h = fmin(C.dt, t1 - C.t0);
           rhs_calls = 1;
         }
@@ -2142,7 +2128,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
       double yt_r[1], yt_i[1], yn_r[1], yn_i[1];
 #define LB_PAIR_STAGE(st)                                                                            \
   stage_combo<1, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                                    \
-  rhs_pair<G>(gwr, L.gf, grows, sl, base, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0],   \
+  rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0],   \
               coef + (st) * KK);
       LB_PAIR_STAGE(2)
       LB_PAIR_STAGE(3)
@@ -2156,7 +2142,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_solve_pair_kernel(const DevPlan 
         const double a0 = yn_r[0] - yt_r[0], a1 = yn_i[0] - yt_i[0];
         stiff_d = a0 * a0 + a1 * a1;
       }
-      rhs_pair<G>(gwr, L.gf, grows, sl, base, selfv, diag, yn_r[0], yn_i[0], k_r[7][0], k_i[7][0], coef + 7 * KK);
+      rhs_pair<G>(L.gw, L.gf, grows, sl, base, selfv, diag, yn_r[0], yn_i[0], k_r[7][0], k_i[7][0], coef + 7 * KK);
       rhs_calls += 6;
       // ---- error norm, integrand, stiffness (as the single-wave form)
       double er = T::e(1) * k_r[1][0], ei = T::e(1) * k_i[1][0];
@@ -2538,7 +2524,8 @@ __device__ __forceinline__ void adjoint_stage(const DevPlan& P, const LdsAdj& L,
       const double2 f = reinterpret_cast<const double2*>(P.t_f)[idx];
       const uint32_t o = w & 0xFFFFFu;
       const uint32_t cid = w >> 24;
-      dcpx c = cid == LB_STATIC_COEF ? mk(1.0, 0.0) : cf[cid];
+      const dcpx cl = cf[cid == LB_STATIC_COEF ? 0u : cid];
+      dcpx c = cid == LB_STATIC_COEF ? mk(1.0, 0.0) : cl;
       if (w & (1u << 22)) c.im = -c.im;
       // H = conj(kappa_o) f C
       const dcpx ko = mk(L.kap[2 * o], -L.kap[2 * o + 1]);
@@ -3057,7 +3044,8 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
       const bool cj = (w[j] & (1u << 22)) != 0;
       // conj(kappa_o) f, then times the coefficient
       const dcpx fo = cmul(mk(kor[j], -koi[j]), mk(f[j].x, f[j].y));
-      dcpx c = cid == LB_STATIC_COEF ? mk(1.0, 0.0) : cf[cid];
+      const dcpx cl = cf[cid == LB_STATIC_COEF ? 0u : cid];
+      dcpx c = cid == LB_STATIC_COEF ? mk(1.0, 0.0) : cl;
       if (cj) c.im = -c.im;
       const dcpx hh = cmul(fo, c);
       mr += hh.re;
@@ -3069,6 +3057,7 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
       const double qi = cj ? -qx.im : qx.im;
 #pragma unroll
       for (int k = 0; k < LB_AW_MAX_COEF; ++k) {
+        if (k >= P.K) break;  // uniform: only the plan's coefficients
         if ((int)cid == k) {
           Q[k].re += qx.re;
           Q[k].im += qi;
