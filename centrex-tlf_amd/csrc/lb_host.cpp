@@ -1309,8 +1309,12 @@ int64_t lds_bytes_adj_wave(const DevPlan& P, const std::vector<int32_t>& g_rows,
   return g * 64 * 16 + (int64_t)t_rows * 64 * 16 + KC * 16 + (int64_t)every * 7 * odd(KC) * 16 +
          (int64_t)P.n_slots * 16 + 64 * odd(P.n_slots) * 16 + 64 * odd(P.vm_depth) * 16 + 64 * 16 +
          (int64_t)LB_MAX_DIFF * 64 * 16 +
-         (int64_t)((P.n + 1) & ~1) * 8 + g * 64 * 4 + (int64_t)t_rows * 64 * 4 + 16;
+         (int64_t)((P.n + 1) & ~1) * 8 + g * 64 * 4 + (int64_t)t_rows * 64 * 4 + 4 * 64 * 16 +
+         (8 + 2 * (int64_t)LB_MAX_DIFF) * 8 + 16;
 }
+
+// Threads of the single-block adjoint: two waves per trajectory.
+constexpr int kAdjWaveThreads = 128;
 
 }  // namespace
 
@@ -1345,8 +1349,9 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
                 "the adjoint of the adaptive solve needs the time derivative programs (lb_plan_desc.diff_time = 1)");
   c.adj_mode = a->adjoint_mode;
   // A plan of one 64-slot block whose gather rows and LDS fit takes the
-  // single-wave adjoint; otherwise the multi-wave variant with the fewest
-  // slots per thread that covers the plan.  args->variant = k + 1 forces
+  // single-block adjoint (lane = slot, two waves per trajectory); otherwise
+  // the multi-wave variant with the fewest slots per thread that covers the
+  // plan.  args->variant = k + 1 forces
   // adjoint variant k (both forms compute the same derivative).
   const int S = im->dev.S;
   const int64_t lds_wave = lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every);
@@ -1360,13 +1365,13 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
     if (!lb_dev_adjoint_variant(a->variant - 1, &R, &maxw, &rows))
       return fail(LB_ERR_INVALID, "no such adjoint variant");
     wave = rows > 0;
-    W = wave ? 64 : std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
+    W = wave ? kAdjWaveThreads : std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
     if (wave ? !wave_fits(rows) : W > maxw)
       return fail(LB_ERR_INVALID, "the adjoint variant does not cover this plan");
     id = a->variant - 1;
   } else {
     for (int v = 0; lb_dev_adjoint_variant(v, &R, &maxw, &rows); ++v)
-      if (rows > 0 && wave_fits(rows)) { id = v; W = 64; wave = true; break; }
+      if (rows > 0 && wave_fits(rows)) { id = v; W = kAdjWaveThreads; wave = true; break; }
     for (int v = 0; id < 0 && lb_dev_adjoint_variant(v, &R, &maxw, &rows); ++v) {
       if (rows > 0) continue;
       const int w = std::max(64, ((S + R - 1) / R + 63) / 64 * 64);

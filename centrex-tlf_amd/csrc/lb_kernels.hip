@@ -2929,20 +2929,25 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
 }
 
 // ---------------------------------------------------------------------------
-// Single-wave adjoint for plans of one 64-slot block (lane = slot).
+// Adjoint for plans of one 64-slot block (lane = slot), two waves per
+// trajectory.
 //
 // The same adjoint as lb_adjoint_kernel, arranged for latency: the gather and
 // transposed term tables live in LDS for the whole launch; the coefficient
 // rows of a checkpoint segment (every stage time of its attempts is on the
-// tape) are evaluated in one pass, one (attempt, stage) per lane, each lane
-// with a private copy of the parameter slots and its own expression stack;
-// the recomputation uses the register/cross-lane RHS of the single-wave
-// solver (the same arithmetic, so the stage derivatives are those of the
-// recorded solve); the reverse stages read kappa of other elements by
-// cross-lane permutes and fold the parameter and time contractions of every
-// term into the transposed pass.  The self terms of an element
+// tape) are evaluated in one pass, one (attempt, stage) per lane of the first
+// wave, each lane with a private copy of the parameter slots and its own
+// expression stack; the first wave recomputes the segment with the
+// register/cross-lane RHS of the single-wave solver (the same arithmetic, so
+// the stage derivatives are those of the recorded solve); the reverse stages
+// read kappa of other elements by cross-lane permutes and fold the parameter
+// and time contractions of every term into the transposed pass.  Both waves
+// sweep every attempt with the same per-element state; each takes half of the
+// transposed rows of every reverse stage, and the two partial sums of mu and
+// of the step-size and time cotangents meet in LDS (first wave's part first),
+// so both waves continue with the same values.  The self terms of an element
 // (time-independent coefficients) and their parameter derivatives are folded
-// once per trajectory.
+// once per trajectory and counted by the first wave.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -2959,6 +2964,8 @@ struct LdsAW {
   double* wts;     // n integral weights
   uint32_t* gw;    // g_rows x 64 gather words
   uint32_t* tw;    // t_rows x 64 transposed words
+  double2* part;   // 2 x 2 x 64: per-wave partial mu of a reverse stage (two buffers)
+  double* xs;      // 8 + 2 * LB_MAX_DIFF: per-wave partial step/time cotangents, gradient sums
   int* misc;
 };
 
@@ -2982,6 +2989,8 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
   L.wts = reinterpret_cast<double*>(q); q += (size_t)((P.n + 1) & ~1) * 8;
   L.gw = reinterpret_cast<uint32_t*>(q); q += (size_t)g * 64 * 4;
   L.tw = reinterpret_cast<uint32_t*>(q); q += (size_t)t * 64 * 4;
+  L.part = reinterpret_cast<double2*>(q); q += (size_t)4 * 64 * 16;
+  L.xs = reinterpret_cast<double*>(q); q += (size_t)(8 + 2 * LB_MAX_DIFF) * 8;
   L.misc = reinterpret_cast<int*>(q);
   return L;
 }
@@ -2989,23 +2998,28 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
 // Coefficients the single-wave adjoint accumulates per lane in registers.
 #define LB_AW_MAX_COEF 8
 
-// One reverse stage: mu = A(t_i)^T kappa for this lane's element,
-// g_m += Re(conj(kappa)^T (dA(t_i)/dp_m) Y_i) and (a plan with a time column)
-// tc = Re(conj(kappa)^T (dA(t_i)/dt) Y_i).  `cf` is the stage's coefficient
-// row (values, the constant 1 at K, then d c_k / d p_m at K + 1 + k * nd + m,
-// the time column last).  The terms' contractions are first summed per
+// One reverse stage over the transposed rows [r0, r1) of this lane's element:
+// its part of mu = A(t_i)^T kappa, of g_m += Re(conj(kappa)^T (dA(t_i)/dp_m)
+// Y_i) and (a plan with a time column) of tc = Re(conj(kappa)^T (dA(t_i)/dt)
+// Y_i); `own` adds the self terms.  `cf` is the stage's coefficient row
+// (values, the constant 1 at K, then d c_k / d p_m at K + 1 + k * nd + m, the
+// time column last).  The terms' contractions are first summed per
 // coefficient (Q_k, registers; K <= LB_AW_MAX_COEF), then contracted with the
 // coefficient derivatives once per stage.
-__device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW& L, int trows, int lane,
-                                                   double kr, double ki, double yr, double yi, bool diag,
+constexpr int kAwChunk = 4;  // transposed rows fetched together
+
+__device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW& L, int r0, int r1, bool own,
+                                                   int lane, double kr, double ki, double yr, double yi, bool diag,
                                                    double2 sv, const dcpx* cf, double& mr, double& mi,
                                                    double (&g)[LB_MAX_DIFF], double& tc) {
   const int nd = P.n_diff, np = P.n_diff - P.diff_time;
   const dcpx* dc = cf + P.K + 1;
   const double ckr = kr, cki = -ki;  // conj(kappa)
-  mr = ckr * sv.x - cki * sv.y;
-  mi = diag ? 0.0 : -(ckr * sv.y + cki * sv.x);
-  {
+  mr = 0.0;
+  mi = 0.0;
+  if (own) {
+    mr = ckr * sv.x - cki * sv.y;
+    mi = diag ? 0.0 : -(ckr * sv.y + cki * sv.x);
     const dcpx cx = cmul(mk(ckr, cki), mk(yr, diag ? 0.0 : yi));
 #pragma unroll
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
@@ -3017,21 +3031,21 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
   dcpx Q[LB_AW_MAX_COEF];
 #pragma unroll
   for (int k = 0; k < LB_AW_MAX_COEF; ++k) Q[k] = mk(0.0, 0.0);
-  constexpr int CH = 4;
-  for (int c0 = 0; c0 < trows; c0 += CH) {
+  constexpr int CH = kAwChunk;
+  for (int c0 = r0; c0 < r1; c0 += CH) {
     uint32_t w[CH];
     double2 f[CH];
     double kor[CH], koi[CH];
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-      if (c0 + j < trows) {
+      if (c0 + j < r1) {
         w[j] = L.tw[(c0 + j) * 64 + lane];
         f[j] = L.tf[(c0 + j) * 64 + lane];
       }
     }
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-      if (c0 + j < trows) {
+      if (c0 + j < r1) {
         const int o = (int)(w[j] & 63u);
         kor[j] = __shfl(kr, o, 64);
         koi[j] = __shfl(ki, o, 64);
@@ -3039,7 +3053,7 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
     }
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-      if (c0 + j >= trows) continue;
+      if (c0 + j >= r1) continue;
       const uint32_t cid = w[j] >> 24;
       const bool cj = (w[j] & (1u << 22)) != 0;
       // conj(kappa_o) f, then times the coefficient
@@ -3084,26 +3098,61 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
 }  // namespace
 
 template <int SOLVER, int G, int WAVES>
-__global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPlan P, const DevCall C) {
+__global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPlan P, const DevCall C) {
   using T = Tab<SOLVER>;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // wave index (uniform) and waves per trajectory (1 or 2)
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = (int)(blockDim.x >> 6);
+  const bool lead = wv == 0;
   const int every = C.ckpt_every;
   const LdsAW L = carve_aw(P, every);
   const int KC = P.KD + 1, KCP = odd_up(KC), nd = P.n_diff, np = P.n_diff - P.diff_time;
   const bool adaptive = C.adj_mode == LB_ADJOINT_ADAPTIVE;
   const double dimf = (double)P.dim;
   const int grows = cst(P.blk_rows)[0], trows = cst(P.t_blk_rows)[0];
+  // this wave's transposed rows: the first wave takes the first half of the
+  // chunks, the second the rest
+  const int nch = (trows + kAwChunk - 1) / kAwChunk;
+  const int cut = nw > 1 ? min(trows, (nch + 1) / 2 * kAwChunk) : trows;
+  const int tr0 = lead ? 0 : cut, tr1 = lead ? cut : trows;
   Slots<1> S;
   load_slots<1>(P, S, lane, 64);
   const bool diag = S.diag(0), valid = S.valid(0);
-  for (int i = lane; i < grows * 64; i += 64) {
+  for (int i = tid; i < grows * 64; i += (int)blockDim.x) {
     L.gf[i] = reinterpret_cast<const double2*>(P.g_f)[i];
     L.gw[i] = P.g_word[i];
   }
-  for (int i = lane; i < trows * 64; i += 64) {
+  for (int i = tid; i < trows * 64; i += (int)blockDim.x) {
     L.tf[i] = reinterpret_cast<const double2*>(P.t_f)[i];
     L.tw[i] = P.t_word[i];
   }
+  int par = 0;  // exchange buffer of the next partial-sum exchange
+  // mu (or a scalar pair) summed over the waves, first wave's part first;
+  // every exchange has one barrier and alternates between two buffers, so a
+  // buffer is rewritten only after both waves have read it
+  auto meet = [&](double a, double b, double& sa, double& sb, bool per_lane) {
+    if (nw == 1) {
+      sa = a;
+      sb = b;
+      return;
+    }
+    if (per_lane) {
+      L.part[(par * 2 + wv) * 64 + lane] = double2{a, b};
+    } else if (lane == 0) {
+      L.xs[par * 4 + wv * 2] = a;
+      L.xs[par * 4 + wv * 2 + 1] = b;
+    }
+    __syncthreads();
+    if (per_lane) {
+      const double2 u = L.part[par * 2 * 64 + lane], v = L.part[(par * 2 + 1) * 64 + lane];
+      sa = u.x + v.x;
+      sb = u.y + v.y;
+    } else {
+      sa = L.xs[par * 4] + L.xs[par * 4 + 2];
+      sb = L.xs[par * 4 + 1] + L.xs[par * 4 + 3];
+    }
+    par ^= 1;
+  };
   Lds LV{};  // the gather tables as the single-wave RHS reads them
   LV.gw = L.gw;
   LV.gf = L.gf;
@@ -3115,7 +3164,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
 
   for (;;) {
     __syncthreads();
-    if (lane == 0) L.misc[0] = atomicAdd(C.queue, 1);
+    if (tid == 0) L.misc[0] = atomicAdd(C.queue, 1);
     __syncthreads();
     const int64_t local = L.misc[0];
     if (local >= C.n_traj) break;
@@ -3124,25 +3173,29 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
     const int64_t traj = C.queue_order ? (int64_t)C.queue_order[local + C.traj_offset] : local + C.traj_offset;
     if (traj < 0 || traj >= C.n_total) continue;
 
-    // ---- parameters and coefficients at t0, weights
-    for (int i = lane; i < P.n_slots; i += 64)
-      L.slots0[i] = i < P.n_base ? mk(P.base[2 * i], P.base[2 * i + 1]) : mk(0.0, 0.0);
+    // ---- parameters and coefficients at t0, weights (the first wave)
+    if (lead)
+      for (int i = lane; i < P.n_slots; i += 64)
+        L.slots0[i] = i < P.n_base ? mk(P.base[2 * i], P.base[2 * i + 1]) : mk(0.0, 0.0);
     __syncthreads();
-    if (lane == 0) {
+    if (tid == 0) {
       apply_overrides(C, traj, L.slots0);
       eval_coefficients(P, L.slots0, L.coef0, C.t0, false, myvm, P.KD);
     }
-    for (int i = lane; i < P.n; i += 64) {
-      double w = 0.0;
-      for (int q = 0; q < C.n_weights; ++q)
-        if (C.weight_index[q] == i) w += C.weight_value[q];
-      L.wts[i] = w;
-    }
+    if (lead)
+      for (int i = lane; i < P.n; i += 64) {
+        double w = 0.0;
+        for (int q = 0; q < C.n_weights; ++q)
+          if (C.weight_index[q] == i) w += C.weight_value[q];
+        L.wts[i] = w;
+      }
     __syncthreads();
-    for (int i = 0; i < P.n_slots; ++i) myslots[i] = L.slots0[i];
-    fold_self(P, lane, 64, L.coef0, reinterpret_cast<double*>(L.selfc));
+    if (lead) {
+      for (int i = 0; i < P.n_slots; ++i) myslots[i] = L.slots0[i];
+      fold_self(P, lane, 64, L.coef0, reinterpret_cast<double*>(L.selfc));
+    }
     // parameter derivatives of the folded self coefficient
-    for (int m = 0; m < np; ++m) {
+    for (int m = 0; lead && m < np; ++m) {
       dcpx acc = mk(0.0, 0.0);
       for (int j = 0; j < P.max_self; ++j) {
         const size_t e = (size_t)lane * P.max_self + j;
@@ -3178,7 +3231,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
         const int a0 = sg * every, a1 = min(a0 + every, N), ntask = 7 * (a1 - a0);
         // ---- coefficient rows of the segment: task q = attempt (q / 7), stage (q % 7 + 1)
         __syncthreads();  // the previous segment's rows are no longer read
-        for (int q = lane; q < ntask; q += 64) {
+        for (int q = lane; lead && q < ntask; q += 64) {
           const int a = a0 + q / 7, st = q % 7 + 1;
           const double ts = st == 1 ? tt[a] : tt[a] + th[a] * T::c(st);
           dcpx* row = L.rows + q * KCP;
@@ -3187,38 +3240,44 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
         }
         __syncthreads();
         // ---- recompute the segment from its checkpoint, keeping y and k1..k7
+        // (the first wave; the barrier after it publishes the workspace)
         double y_r[1], y_i[1];
-        {
-          const double2 ck = reinterpret_cast<const double2*>(C.tape_y + ((traj * ncap + sg) * P.S) * 2)[lane];
-          y_r[0] = ck.x;
-          y_i[0] = ck.y;
-        }
-        for (int a = a0; a < a1; ++a) {
-          const double h = th[a];
-          const dcpx* rw = L.rows + (a - a0) * 7 * KCP;
-          double2* wk = segW + (size_t)(a - a0) * 8 * 64 + lane;
-          double k_r[8][1], k_i[8][1], yt_r[1], yt_i[1];
-          wk[0] = double2{y_r[0], y_i[0]};
-          rhs_wave<G>(LV, grows, lane, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], rw);
-          wk[64] = double2{k_r[1][0], k_i[1][0]};
+        if (lead) {
+          {
+            const double2 ck = reinterpret_cast<const double2*>(C.tape_y + ((traj * ncap + sg) * P.S) * 2)[lane];
+            y_r[0] = ck.x;
+            y_i[0] = ck.y;
+          }
+          for (int a = a0; a < a1; ++a) {
+            const double h = th[a];
+            const dcpx* rw = L.rows + (a - a0) * 7 * KCP;
+            double2* wk = segW + (size_t)(a - a0) * 8 * 64 + lane;
+            double k_r[8][1], k_i[8][1], yt_r[1], yt_i[1];
+            wk[0] = double2{y_r[0], y_i[0]};
+            rhs_wave<G>(LV, grows, lane, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], rw);
+            wk[64] = double2{k_r[1][0], k_i[1][0]};
 #define LB_AW_FWD(st)                                                                           \
   stage_combo<1, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                                \
   rhs_wave<G>(LV, grows, lane, selfv, diag, yt_r[0], yt_i[0], k_r[st][0], k_i[st][0], rw + (st - 1) * KCP); \
   wk[(st) * 64] = double2{k_r[st][0], k_i[st][0]};
-          LB_AW_FWD(2)
-          LB_AW_FWD(3)
-          LB_AW_FWD(4)
-          LB_AW_FWD(5)
-          LB_AW_FWD(6)
-          LB_AW_FWD(7)
+            LB_AW_FWD(2)
+            LB_AW_FWD(3)
+            LB_AW_FWD(4)
+            LB_AW_FWD(5)
+            LB_AW_FWD(6)
+            LB_AW_FWD(7)
 #undef LB_AW_FWD
-          if (tfl[a] & LB_TAPE_ACCEPTED) {
-            // stage_combo<7> left the candidate solution in yt
-            y_r[0] = yt_r[0];
-            y_i[0] = yt_i[0];
+            if (tfl[a] & LB_TAPE_ACCEPTED) {
+              // stage_combo<7> left the candidate solution in yt
+              y_r[0] = yt_r[0];
+              y_i[0] = yt_i[0];
+            }
           }
         }
-        // ---- reverse sweep over the segment's attempts, last first
+        __syncthreads();
+        // ---- reverse sweep over the segment's attempts, last first (both
+        // waves, the same control flow: every branch below depends on the
+        // tape and on cotangents both waves hold)
         for (int a = a1 - 1; a >= a0; --a) {
           const int fl = tfl[a];
           const bool acc = (fl & LB_TAPE_ACCEPTED) != 0;
@@ -3265,7 +3324,7 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
             // 0.5 (v_n + v_{n+1}) h with v = w . y
             lr += 0.5 * h * wme;
             ybr += 0.5 * h * wme;
-            hbl += 0.5 * wme * (y_r[0] + yn_r[0]);
+            if (lead) hbl += 0.5 * wme * (y_r[0] + yn_r[0]);
           }
           if (errb != 0.0 && valid) {
             const double err = tcr[3 * a];
@@ -3286,9 +3345,10 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
           double lpr = lr, lpi = li;
           if (errb != 0.0) {
             // stage 7: k7 = f(x + h, yn) enters the error estimate only
-            double tc7 = 0.0;
-            adjoint_wave_stage(P, L, trows, lane, h * (T::e(7) * nur), diag ? 0.0 : h * (T::e(7) * nui), yn_r[0],
-                               yn_i[0], diag, selfv, rw + 6 * KCP, mu_r[7], mu_i[7], g, tc7);
+            double tc7 = 0.0, pr, pi;
+            adjoint_wave_stage(P, L, tr0, tr1, lead, lane, h * (T::e(7) * nur), diag ? 0.0 : h * (T::e(7) * nui),
+                               yn_r[0], yn_i[0], diag, selfv, rw + 6 * KCP, pr, pi, g, tc7);
+            meet(pr, pi, mu_r[7], mu_i[7], true);
             tbl += tc7;
             hbl += tc7;
             lpr += mu_r[7];
@@ -3310,9 +3370,10 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
     } else {                                                                                            \
       stage_combo<1, SOLVER, (i > 1 ? i : 2)>(h, y_r, y_i, k_r, k_i, ys_r, ys_i);                        \
     }                                                                                                   \
-    double tci = 0.0;                                                                                   \
-    adjoint_wave_stage(P, L, trows, lane, h * ar, diag ? 0.0 : h * ai, ys_r[0], ys_i[0], diag, selfv,   \
-                       rw + (i - 1) * KCP, mu_r[i], mu_i[i], g, tci);                                   \
+    double tci = 0.0, pr, pi;                                                                           \
+    adjoint_wave_stage(P, L, tr0, tr1, lead, lane, h * ar, diag ? 0.0 : h * ai, ys_r[0], ys_i[0], diag, \
+                       selfv, rw + (i - 1) * KCP, pr, pi, g, tci);                                      \
+    meet(pr, pi, mu_r[i], mu_i[i], true);                                                               \
     tbl += tci;                                                                                         \
     hbl += T::c(i) * tci;                                                                               \
   }
@@ -3332,9 +3393,11 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
           ybr += sr;
           ybi += si;
           if (adaptive) {
-            hbl += h_products<SOLVER>(k_r, k_i, lpr, lpi, nur, nui, mu_r, mu_i);
-            hb += wave_sum(hbl);
-            Tb += wave_sum(tbl);
+            if (lead) hbl += h_products<SOLVER>(k_r, k_i, lpr, lpi, nur, nui, mu_r, mu_i);
+            double sh, st;
+            meet(wave_sum(hbl), wave_sum(tbl), sh, st, false);
+            hb += sh;
+            Tb += st;
             if (fl & LB_TAPE_CLIPPED) {
               Tb -= hb;
               Hb = 0.0;
@@ -3348,13 +3411,20 @@ __global__ __launch_bounds__(64, WAVES) void lb_adjoint_wave_kernel(const DevPla
         }
       }
     }
-    // ---- dI/dp: one wave sum per parameter
+    // ---- dI/dp: one wave sum per parameter and wave, the waves in order
 #pragma unroll
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
       if (m >= np) break;
       const double v = wave_sum(g[m]);
-      if (lane == 0) C.out_grad[traj * np + m] = ok ? v : __builtin_nan("");
+      if (lane == 0) L.xs[8 + wv * LB_MAX_DIFF + m] = v;
     }
+    __syncthreads();
+    if (tid == 0)
+      for (int m = 0; m < np; ++m) {
+        double v = L.xs[8 + m];
+        if (nw > 1) v += L.xs[8 + LB_MAX_DIFF + m];
+        C.out_grad[traj * np + m] = ok ? v : __builtin_nan("");
+      }
   }
 }
 
