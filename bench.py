@@ -228,8 +228,9 @@ def fit_leg(per_group: int, iterations: int, t_scale: float, graph: bool = True)
         "trajectories": 2 * per_group,
         "iterations": iterations,
         "seconds": res.seconds,
-        "forward_seconds": res.forward_seconds,
-        "adjoint_seconds": res.adjoint_seconds,
+        # the replayed iterations run forward and adjoint in one graph: no split
+        "forward_seconds": None if graph else res.forward_seconds,
+        "adjoint_seconds": None if graph else res.adjoint_seconds,
         "loss_first_last": [res.loss[0], res.loss[-1]],
         "loss": res.loss,
         "graph_replayed": graph,
