@@ -126,6 +126,7 @@ struct PlanSource {
   std::vector<int32_t> up_re, up_im;  // upper index -> packed indices (-1 imag on the diagonal)
   LbProgram prog;
   std::vector<int32_t> expr_dyn;
+  std::vector<uint8_t> fwd_compound;  // 0: a compound only the derivative programs read
   std::vector<int32_t> compound_slot;
   bool time_dependent = false;
   int n_dyn_coef = 0;
@@ -530,10 +531,24 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   up(src.compound_slot, &P.compound_slot);
   up(src.expr_dyn, &P.expr_dyn);
   {
+    // Evaluation order: the compounds the solve needs, the K coefficients,
+    // then the compounds only the derivative programs read and the
+    // derivative programs.  Each part keeps the program's order, so every
+    // expression follows the compounds it reads; a solve evaluates the first
+    // two parts only.
     const int nc = src.prog.n_compounds, K = src.K, KD = src.K * (1 + src.n_diff);
+    std::vector<int> order;
+    for (int e = 0; e < nc; ++e)
+      if (src.fwd_compound[e]) order.push_back(e);
+    for (int e = nc; e < nc + K; ++e) order.push_back(e);
+    const int n_fwd = (int)order.size();
+    for (int e = 0; e < nc; ++e)
+      if (!src.fwd_compound[e]) order.push_back(e);
+    for (int e = nc + K; e < nc + KD; ++e) order.push_back(e);
     std::vector<DevExpr> all, dyn;
     int n_dyn_fwd = 0;
-    for (int e = 0; e < nc + KD; ++e) {
+    for (int i = 0; i < (int)order.size(); ++i) {
+      const int e = order[i];
       DevExpr r{};
       r.start = src.prog.expr_start[e];
       r.end = src.prog.expr_start[e + 1];
@@ -546,10 +561,10 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
       all.push_back(r);
       if (src.expr_dyn[e]) {
         dyn.push_back(r);
-        if (e < nc + K) ++n_dyn_fwd;
+        if (i < n_fwd) ++n_dyn_fwd;
       }
     }
-    P.n_all_fwd = nc + K;
+    P.n_all_fwd = n_fwd;
     P.n_all = nc + KD;
     P.n_dyn_fwd = n_dyn_fwd;
     P.n_dyn = (int32_t)dyn.size();
@@ -700,6 +715,49 @@ extern "C" int lb_plan_create(const lb_plan_desc* d, lb_plan** out_plan) {
   }
   src.expr_dyn.assign(src.prog.compound_dyn.begin(), src.prog.compound_dyn.end());
   src.expr_dyn.insert(src.expr_dyn.end(), src.prog.coef_dyn.begin(), src.prog.coef_dyn.end());
+  {
+    // The compounds a solve leaves out: those only the derivative programs of
+    // a differentiable plan read (directly or through other compounds).
+    // Every other compound -- read by a coefficient value, or by nothing the
+    // plan holds (a stop-event expression may read it) -- is evaluated by the
+    // solve as before.
+    const int nc = src.prog.n_compounds, KD = src.K * (1 + src.n_diff);
+    std::vector<int> comp_of_slot(std::max(src.prog.n_slots, 1), -1);
+    for (int c = 0; c < nc; ++c) comp_of_slot[src.prog.compound_slot[c]] = c;
+    // the compounds read, directly or through other compounds, by expressions `from`
+    auto reach = [&](const std::vector<int>& from) {
+      std::vector<uint8_t> seen(nc, 0);
+      std::vector<int> todo = from;
+      while (!todo.empty()) {
+        const int e = todo.back();
+        todo.pop_back();
+        for (int q = src.prog.expr_start[e]; q < src.prog.expr_start[e + 1]; ++q) {
+          const DevInstr& ins = src.prog.instr[q];
+          if (ins.op != 2 || ins.index < 0 || ins.index >= (int)comp_of_slot.size()) continue;
+          const int c = comp_of_slot[ins.index];
+          if (c >= 0 && !seen[c]) {
+            seen[c] = 1;
+            todo.push_back(c);
+          }
+        }
+      }
+      return seen;
+    };
+    std::vector<int> values, derivs;
+    for (int e = nc; e < nc + src.K; ++e) values.push_back(e);
+    for (int e = nc + src.K; e < nc + KD; ++e) derivs.push_back(e);
+    const std::vector<uint8_t> by_values = reach(values), by_derivs = reach(derivs);
+    src.fwd_compound.assign(nc, 1);
+    for (int c = 0; c < nc; ++c)
+      if (by_derivs[c] && !by_values[c]) src.fwd_compound[c] = 0;
+    // what a kept compound reads is kept too
+    std::vector<int> kept;
+    for (int c = 0; c < nc; ++c)
+      if (src.fwd_compound[c]) kept.push_back(c);
+    const std::vector<uint8_t> by_kept = reach(kept);
+    for (int c = 0; c < nc; ++c)
+      if (by_kept[c]) src.fwd_compound[c] = 1;
+  }
   if (src.expr_dyn.empty()) src.expr_dyn.push_back(0);
   src.compound_slot = src.prog.compound_slot;
   if (src.compound_slot.empty()) src.compound_slot.push_back(0);
@@ -734,7 +792,12 @@ extern "C" int lb_plan_specialize(lb_plan* p, int32_t enable) {
   }
   if (p->spec_source.empty()) {
     std::string why;
-    p->spec_source = lb_rtc_coef_source(p->src.prog, p->src.expr_dyn, p->src.K, why);
+    // only the programs a solve evaluates (the derivative-only compounds of a
+    // differentiable plan stay out)
+    std::vector<int32_t> dyn = p->src.expr_dyn;
+    for (int c = 0; c < p->src.prog.n_compounds; ++c)
+      if (!p->src.fwd_compound[c]) dyn[c] = 0;
+    p->spec_source = lb_rtc_coef_source(p->src.prog, dyn, p->src.K, why);
     if (p->spec_source.empty()) return fail(LB_ERR_UNSUPPORTED, why);
   }
   p->spec = true;
