@@ -1365,19 +1365,22 @@ int64_t lds_bytes_adj(const DevPlan& P) {
          8 * (int64_t)LB_MAX_DIFF * 16 + 16 * 8 * (int64_t)P.vm_depth + 16;
 }
 
-// LDS of the single-wave adjoint (carve_aw in lb_kernels.hip).
-int64_t lds_bytes_adj_wave(const DevPlan& P, const std::vector<int32_t>& g_rows, int t_rows, int every) {
+// LDS of the single-block adjoint with `nbuf` segment buffers (carve_aw in
+// lb_kernels.hip).
+int64_t lds_bytes_adj_wave(const DevPlan& P, const std::vector<int32_t>& g_rows, int t_rows, int every, int nbuf) {
   auto odd = [](int64_t v) { return v | 1; };
   const int64_t g = g_rows.empty() ? 0 : g_rows[0], KC = P.KD + 1;
-  return g * 64 * 16 + (int64_t)t_rows * 64 * 16 + KC * 16 + (int64_t)every * 7 * odd(KC) * 16 +
+  return g * 64 * 16 + (int64_t)t_rows * 64 * 16 + KC * 16 + (int64_t)nbuf * every * 7 * odd(KC) * 16 +
          (int64_t)P.n_slots * 16 + 64 * odd(P.n_slots) * 16 + 64 * odd(P.vm_depth) * 16 + 64 * 16 +
          (int64_t)LB_MAX_DIFF * 64 * 16 +
          (int64_t)((P.n + 1) & ~1) * 8 + g * 64 * 4 + (int64_t)t_rows * 64 * 4 + 4 * 64 * 16 +
          (8 + 2 * (int64_t)LB_MAX_DIFF) * 8 + 16;
 }
 
-// Threads of the single-block adjoint: two waves per trajectory.
-constexpr int kAdjWaveThreads = 128;
+// The single-block adjoint runs three waves per trajectory when two segment
+// buffers fit in LDS (two sweep a segment while the third prepares the next),
+// two otherwise (one buffer; the first wave prepares each segment before the
+// sweep).
 
 }  // namespace
 
@@ -1417,7 +1420,10 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
   // plan.  args->variant = k + 1 forces
   // adjoint variant k (both forms compute the same derivative).
   const int S = im->dev.S;
-  const int64_t lds_wave = lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every);
+  const int wave_bufs =
+      lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every, 2) <= kMaxLdsBytes ? 2 : 1;
+  const int wave_threads = wave_bufs == 2 ? 192 : 128;
+  const int64_t lds_wave = lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every, wave_bufs);
   auto wave_fits = [&](int rows) {
     return im->dev.nblk == 1 && !im->g_rows_host.empty() && im->g_rows_host[0] <= rows &&
            im->dev.K <= 8 && lds_wave <= kMaxLdsBytes;  // 8: LB_AW_MAX_COEF
@@ -1428,13 +1434,13 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
     if (!lb_dev_adjoint_variant(a->variant - 1, &R, &maxw, &rows))
       return fail(LB_ERR_INVALID, "no such adjoint variant");
     wave = rows > 0;
-    W = wave ? kAdjWaveThreads : std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
+    W = wave ? wave_threads : std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
     if (wave ? !wave_fits(rows) : W > maxw)
       return fail(LB_ERR_INVALID, "the adjoint variant does not cover this plan");
     id = a->variant - 1;
   } else {
     for (int v = 0; lb_dev_adjoint_variant(v, &R, &maxw, &rows); ++v)
-      if (rows > 0 && wave_fits(rows)) { id = v; W = kAdjWaveThreads; wave = true; break; }
+      if (rows > 0 && wave_fits(rows)) { id = v; W = wave_threads; wave = true; break; }
     for (int v = 0; id < 0 && lb_dev_adjoint_variant(v, &R, &maxw, &rows); ++v) {
       if (rows > 0) continue;
       const int w = std::max(64, ((S + R - 1) / R + 63) / 64 * 64);
@@ -1454,8 +1460,9 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
   }
   grid = (int)std::min<int64_t>(grid, a->n_traj);
   // workspace: ckpt_every recomputed attempts per resident workgroup (y and
-  // k1..k7, and the seven coefficient rows of each), in doubles
-  const size_t per_y = (size_t)a->ckpt_every * 8 * S * 2;
+  // k1..k7, per segment buffer, and the seven coefficient rows of each), in
+  // doubles
+  const size_t per_y = (size_t)a->ckpt_every * 8 * S * 2 * (wave ? wave_bufs : 1);
   const size_t per_c = (size_t)a->ckpt_every * 7 * (im->dev.KD + 1) * 2;
   hipStream_t s = (hipStream_t)stream;
   if (scratch) {

@@ -2955,7 +2955,7 @@ struct LdsAW {
   double2* gf;     // g_rows x 64 gather factors
   double2* tf;     // t_rows x 64 transposed factors
   dcpx* coef0;     // KC: coefficient row at t0 (time-independent entries)
-  dcpx* rows;      // every * 7 rows of stride KCP: coefficient rows of a segment
+  dcpx* rows;      // nbuf x every * 7 rows of stride KCP: coefficient rows of a segment
   dcpx* slots0;    // n_slots: parameter slots at t0
   dcpx* lslots;    // 64 x SSP: per-lane parameter slots
   dcpx* lvm;       // 64 x VSP: per-lane expression stacks
@@ -2972,7 +2972,7 @@ struct LdsAW {
 // Odd strides (in 16-byte units) spread the lanes' rows over the LDS banks.
 __device__ __forceinline__ int odd_up(int v) { return v | 1; }
 
-__device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
+__device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every, int nbuf) {
   extern __shared__ double smem[];
   const int g = cst(P.blk_rows)[0], t = cst(P.t_blk_rows)[0], KC = P.KD + 1;
   char* q = reinterpret_cast<char*>(smem);
@@ -2980,7 +2980,7 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
   L.gf = reinterpret_cast<double2*>(q); q += (size_t)g * 64 * 16;
   L.tf = reinterpret_cast<double2*>(q); q += (size_t)t * 64 * 16;
   L.coef0 = reinterpret_cast<dcpx*>(q); q += (size_t)KC * 16;
-  L.rows = reinterpret_cast<dcpx*>(q); q += (size_t)every * 7 * odd_up(KC) * 16;
+  L.rows = reinterpret_cast<dcpx*>(q); q += (size_t)nbuf * every * 7 * odd_up(KC) * 16;
   L.slots0 = reinterpret_cast<dcpx*>(q); q += (size_t)P.n_slots * 16;
   L.lslots = reinterpret_cast<dcpx*>(q); q += (size_t)64 * odd_up(P.n_slots) * 16;
   L.lvm = reinterpret_cast<dcpx*>(q); q += (size_t)64 * odd_up(P.vm_depth) * 16;
@@ -3007,6 +3007,10 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every) {
 // coefficient (Q_k, registers; K <= LB_AW_MAX_COEF), then contracted with the
 // coefficient derivatives once per stage.
 constexpr int kAwChunk = 4;  // transposed rows fetched together
+// Polls of one exchange wait before the launch is marked broken: about a
+// second, far beyond the longest legitimate wait (one segment's rows and
+// recomputation, well under a millisecond).
+constexpr int kAwSpinLimit = 1 << 22;
 
 __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW& L, int r0, int r1, bool own,
                                                    int lane, double kr, double ki, double yr, double yi, bool diag,
@@ -3098,22 +3102,26 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
 }  // namespace
 
 template <int SOLVER, int G, int WAVES>
-__global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPlan P, const DevCall C) {
+__global__ __launch_bounds__(192, WAVES) void lb_adjoint_wave_kernel(const DevPlan P, const DevCall C) {
   using T = Tab<SOLVER>;
   const int tid = threadIdx.x, lane = tid & 63;
-  // wave index (uniform) and waves per trajectory (1 or 2)
+  // wave index (uniform) and waves per trajectory: with three, waves 0 and 1
+  // sweep a segment's attempts while wave 2 evaluates the coefficient rows of
+  // the next segment and recomputes it into the other of two buffers; with
+  // two, wave 0 does that work before the sweep, into one buffer
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = (int)(blockDim.x >> 6);
-  const bool lead = wv == 0;
+  const int nrev = nw >= 2 ? 2 : 1, prod = nw >= 3 ? 2 : 0, nbuf = nw >= 3 ? 2 : 1;
+  const bool rev = wv < nrev, lead = wv == 0;
   const int every = C.ckpt_every;
-  const LdsAW L = carve_aw(P, every);
+  const LdsAW L = carve_aw(P, every, nbuf);
   const int KC = P.KD + 1, KCP = odd_up(KC), nd = P.n_diff, np = P.n_diff - P.diff_time;
   const bool adaptive = C.adj_mode == LB_ADJOINT_ADAPTIVE;
   const double dimf = (double)P.dim;
   const int grows = cst(P.blk_rows)[0], trows = cst(P.t_blk_rows)[0];
-  // this wave's transposed rows: the first wave takes the first half of the
-  // chunks, the second the rest
+  // this wave's transposed rows: the first sweeping wave takes the first half
+  // of the chunks, the second the rest
   const int nch = (trows + kAwChunk - 1) / kAwChunk;
-  const int cut = nw > 1 ? min(trows, (nch + 1) / 2 * kAwChunk) : trows;
+  const int cut = nrev > 1 ? min(trows, (nch + 1) / 2 * kAwChunk) : trows;
   const int tr0 = lead ? 0 : cut, tr1 = lead ? cut : trows;
   Slots<1> S;
   load_slots<1>(P, S, lane, 64);
@@ -3126,12 +3134,23 @@ __global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPl
     L.tf[i] = reinterpret_cast<const double2*>(P.t_f)[i];
     L.tw[i] = P.t_word[i];
   }
-  int par = 0;  // exchange buffer of the next partial-sum exchange
-  // mu (or a scalar pair) summed over the waves, first wave's part first;
-  // every exchange has one barrier and alternates between two buffers, so a
-  // buffer is rewritten only after both waves have read it
+  if (tid == 0) {
+    L.misc[1] = 0;  // exchanges posted by sweeping wave 0
+    L.misc[2] = 0;  // ... and by sweeping wave 1
+    L.misc[3] = 0;  // 1: an exchange wait ran out (the gradients become NaN)
+  }
+  int par = 0, posted = 0;  // exchange buffer and count of this wave's exchanges
+  bool broken = false;
+  // mu (or a scalar pair) summed over the two sweeping waves, wave 0's part
+  // first.  The waves meet through LDS flags, not a workgroup barrier (the
+  // third wave works on the next segment meanwhile): each posts its part and
+  // its exchange count, then waits until the other has posted the same count.
+  // Two buffers alternate, so a wave rewrites one only after the other has
+  // passed the following exchange, that is after it has read it.  The wait
+  // is bounded: a wave that waits too long marks the launch broken and every
+  // later wait returns at once, so the kernel always ends.
   auto meet = [&](double a, double b, double& sa, double& sb, bool per_lane) {
-    if (nw == 1) {
+    if (nrev == 1) {
       sa = a;
       sb = b;
       return;
@@ -3142,7 +3161,20 @@ __global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPl
       L.xs[par * 4 + wv * 2] = a;
       L.xs[par * 4 + wv * 2 + 1] = b;
     }
-    __syncthreads();
+    ++posted;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(&L.misc[1 + wv], posted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (int it = 0; !broken; ++it) {
+      if (__hip_atomic_load(&L.misc[2 - wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= posted) break;
+      if (__hip_atomic_load(&L.misc[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0 ||
+          it >= kAwSpinLimit) {
+        broken = true;
+        if (lane == 0) __hip_atomic_store(&L.misc[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     if (per_lane) {
       const double2 u = L.part[par * 2 * 64 + lane], v = L.part[(par * 2 + 1) * 64 + lane];
       sa = u.x + v.x;
@@ -3158,8 +3190,8 @@ __global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPl
   LV.gf = L.gf;
   dcpx* const myslots = L.lslots + lane * odd_up(P.n_slots);
   dcpx* const myvm = L.lvm + lane * odd_up(P.vm_depth);
-  // workspace per attempt: y and k1..k7 of this lane's element
-  double2* const segW = reinterpret_cast<double2*>(C.seg_y) + (size_t)blockIdx.x * every * 8 * 64;
+  // workspace per attempt: y and k1..k7 of this lane's element, per buffer
+  double2* const segW = reinterpret_cast<double2*>(C.seg_y) + (size_t)blockIdx.x * nbuf * every * 8 * 64;
   const int64_t ncap = (C.tape_cap + every - 1) / every;
 
   for (;;) {
@@ -3190,10 +3222,9 @@ __global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPl
         L.wts[i] = w;
       }
     __syncthreads();
-    if (lead) {
+    if (wv == prod)
       for (int i = 0; i < P.n_slots; ++i) myslots[i] = L.slots0[i];
-      fold_self(P, lane, 64, L.coef0, reinterpret_cast<double*>(L.selfc));
-    }
+    if (lead) fold_self(P, lane, 64, L.coef0, reinterpret_cast<double*>(L.selfc));
     // parameter derivatives of the folded self coefficient
     for (int m = 0; lead && m < np; ++m) {
       dcpx acc = mk(0.0, 0.0);
@@ -3227,31 +3258,35 @@ __global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPl
       // the current one (before its clip); Qb of fac_old after the current step
       double lam_r = 0.0, lam_i = 0.0, Tb = 0.0, Hb = 0.0, Qb = 0.0;
       const int nseg = (N + every - 1) / every;
-      for (int sg = nseg - 1; sg >= 0; --sg) {
-        const int a0 = sg * every, a1 = min(a0 + every, N), ntask = 7 * (a1 - a0);
-        // ---- coefficient rows of the segment: task q = attempt (q / 7), stage (q % 7 + 1)
-        __syncthreads();  // the previous segment's rows are no longer read
-        for (int q = lane; lead && q < ntask; q += 64) {
-          const int a = a0 + q / 7, st = q % 7 + 1;
-          const double ts = st == 1 ? tt[a] : tt[a] + th[a] * T::c(st);
-          dcpx* row = L.rows + q * KCP;
-          for (int j = 0; j < KC; ++j) row[j] = L.coef0[j];
-          if (P.time_dependent) eval_coefficients(P, myslots, row, ts, true, myvm, P.KD);
-        }
-        __syncthreads();
-        // ---- recompute the segment from its checkpoint, keeping y and k1..k7
-        // (the first wave; the barrier after it publishes the workspace)
-        double y_r[1], y_i[1];
-        if (lead) {
+      // Phase ph produces segment nseg - 1 - ph and sweeps the one produced
+      // before it (two buffers), or produces and sweeps the same segment (one).
+      const int lag = nbuf - 1;
+      for (int ph = 0; ph < nseg + lag; ++ph) {
+        const int sp = nseg - 1 - ph, sr = sp + lag;
+        if (wv == prod && sp >= 0) {
+          const int a0 = sp * every, a1 = min(a0 + every, N), ntask = 7 * (a1 - a0);
+          dcpx* const rows = L.rows + (size_t)(sp % nbuf) * every * 7 * KCP;
+          double2* const wseg = segW + (size_t)(sp % nbuf) * every * 8 * 64;
+          // ---- coefficient rows of the segment: task q = attempt (q / 7), stage (q % 7 + 1)
+          for (int q = lane; q < ntask; q += 64) {
+            const int a = a0 + q / 7, st = q % 7 + 1;
+            const double ts = st == 1 ? tt[a] : tt[a] + th[a] * T::c(st);
+            dcpx* row = rows + q * KCP;
+            for (int j = 0; j < KC; ++j) row[j] = L.coef0[j];
+            if (P.time_dependent) eval_coefficients(P, myslots, row, ts, true, myvm, P.KD);
+          }
+          wave_sync();
+          // ---- recompute the segment from its checkpoint, keeping y and k1..k7
+          double y_r[1], y_i[1];
           {
-            const double2 ck = reinterpret_cast<const double2*>(C.tape_y + ((traj * ncap + sg) * P.S) * 2)[lane];
+            const double2 ck = reinterpret_cast<const double2*>(C.tape_y + ((traj * ncap + sp) * P.S) * 2)[lane];
             y_r[0] = ck.x;
             y_i[0] = ck.y;
           }
           for (int a = a0; a < a1; ++a) {
             const double h = th[a];
-            const dcpx* rw = L.rows + (a - a0) * 7 * KCP;
-            double2* wk = segW + (size_t)(a - a0) * 8 * 64 + lane;
+            const dcpx* rw = rows + (a - a0) * 7 * KCP;
+            double2* wk = wseg + (size_t)(a - a0) * 8 * 64 + lane;
             double k_r[8][1], k_i[8][1], yt_r[1], yt_i[1];
             wk[0] = double2{y_r[0], y_i[0]};
             rhs_wave<G>(LV, grows, lane, selfv, diag, y_r[0], y_i[0], k_r[1][0], k_i[1][0], rw);
@@ -3274,10 +3309,17 @@ __global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPl
             }
           }
         }
-        __syncthreads();
+        if (lag == 0) __syncthreads();  // one buffer: the sweep reads what was just written
+        if (!rev || sr < 0 || sr >= nseg) {
+          __syncthreads();
+          continue;
+        }
+        const int a0 = sr * every, a1 = min(a0 + every, N);
+        const dcpx* const rows = L.rows + (size_t)(sr % nbuf) * every * 7 * KCP;
+        const double2* const wseg = segW + (size_t)(sr % nbuf) * every * 8 * 64;
         // ---- reverse sweep over the segment's attempts, last first (both
-        // waves, the same control flow: every branch below depends on the
-        // tape and on cotangents both waves hold)
+        // sweeping waves, the same control flow: every branch below depends
+        // on the tape and on cotangents both waves hold)
         for (int a = a1 - 1; a >= a0; --a) {
           const int fl = tfl[a];
           const bool acc = (fl & LB_TAPE_ACCEPTED) != 0;
@@ -3301,8 +3343,8 @@ __global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPl
               continue;
             }
           }
-          const dcpx* rw = L.rows + (a - a0) * 7 * KCP;
-          const double2* wk = segW + (size_t)(a - a0) * 8 * 64 + lane;
+          const dcpx* rw = rows + (a - a0) * 7 * KCP;
+          const double2* wk = wseg + (size_t)(a - a0) * 8 * 64 + lane;
           double y_r[1], y_i[1], k_r[8][1], k_i[8][1], yn_r[1], yn_i[1];
           {
             const double2 v = wk[0];
@@ -3409,24 +3451,28 @@ __global__ __launch_bounds__(128, WAVES) void lb_adjoint_wave_kernel(const DevPl
           lam_i = diag ? 0.0 : (acc ? ybi : lam_i + ybi);
           if (acc) Qb = qnew;
         }
+        __syncthreads();
       }
     }
-    // ---- dI/dp: one wave sum per parameter and wave, the waves in order
+    // ---- dI/dp: one wave sum per parameter and sweeping wave, the waves in order
 #pragma unroll
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
       if (m >= np) break;
       const double v = wave_sum(g[m]);
-      if (lane == 0) L.xs[8 + wv * LB_MAX_DIFF + m] = v;
+      if (rev && lane == 0) L.xs[8 + wv * LB_MAX_DIFF + m] = v;
     }
     __syncthreads();
-    if (tid == 0)
+    if (tid == 0) {
+      const bool fine = ok && L.misc[3] == 0;
       for (int m = 0; m < np; ++m) {
         double v = L.xs[8 + m];
-        if (nw > 1) v += L.xs[8 + LB_MAX_DIFF + m];
-        C.out_grad[traj * np + m] = ok ? v : __builtin_nan("");
+        if (nrev > 1) v += L.xs[8 + LB_MAX_DIFF + m];
+        C.out_grad[traj * np + m] = fine ? v : __builtin_nan("");
       }
+    }
   }
 }
+
 
 // ---------------------------------------------------------------------------
 // Batched RHS (one evaluation per workgroup).
