@@ -1426,7 +1426,7 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
   const int64_t lds_wave = lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every, wave_bufs);
   auto wave_fits = [&](int rows) {
     return im->dev.nblk == 1 && !im->g_rows_host.empty() && im->g_rows_host[0] <= rows &&
-           im->dev.K <= 8 && lds_wave <= kMaxLdsBytes;  // 8: LB_AW_MAX_COEF
+           lds_wave <= kMaxLdsBytes;
   };
   int id = -1, W = 0, R = 0, maxw = 0, rows = 0;
   bool wave = false;

@@ -2995,17 +2995,13 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every, int nbuf)
   return L;
 }
 
-// Coefficients the single-wave adjoint accumulates per lane in registers.
-#define LB_AW_MAX_COEF 8
-
 // One reverse stage over the transposed rows [r0, r1) of this lane's element:
 // its part of mu = A(t_i)^T kappa, of g_m += Re(conj(kappa)^T (dA(t_i)/dp_m)
 // Y_i) and (a plan with a time column) of tc = Re(conj(kappa)^T (dA(t_i)/dt)
 // Y_i); `own` adds the self terms.  `cf` is the stage's coefficient row
 // (values, the constant 1 at K, then d c_k / d p_m at K + 1 + k * nd + m, the
-// time column last).  The terms' contractions are first summed per
-// coefficient (Q_k, registers; K <= LB_AW_MAX_COEF), then contracted with the
-// coefficient derivatives once per stage.
+// time column last); each term is contracted with its coefficient's
+// derivatives as it is read.
 constexpr int kAwChunk = 4;  // transposed rows fetched together
 // Polls of one exchange wait before the launch is marked broken: about a
 // second, far beyond the longest legitimate wait (one segment's rows and
@@ -3032,9 +3028,6 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
       g[m] += cx.re * sd.re - cx.im * sd.im;
     }
   }
-  dcpx Q[LB_AW_MAX_COEF];
-#pragma unroll
-  for (int k = 0; k < LB_AW_MAX_COEF; ++k) Q[k] = mk(0.0, 0.0);
   constexpr int CH = kAwChunk;
   for (int c0 = r0; c0 < r1; c0 += CH) {
     uint32_t w[CH];
@@ -3058,43 +3051,36 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
       if (c0 + j >= r1) continue;
-      const uint32_t cid = w[j] >> 24;
-      const bool cj = (w[j] & (1u << 22)) != 0;
-      // conj(kappa_o) f, then times the coefficient
+      const uint32_t wj = w[j], cid = wj >> 24;
+      const bool stat = cid == LB_STATIC_COEF, cj = (wj & (1u << 22)) != 0;
+      const uint32_t k = stat ? 0u : cid;
+      // conj(kappa_o) f, then times the coefficient (1 for a static term)
       const dcpx fo = cmul(mk(kor[j], -koi[j]), mk(f[j].x, f[j].y));
-      const dcpx cl = cf[cid == LB_STATIC_COEF ? 0u : cid];
-      dcpx c = cid == LB_STATIC_COEF ? mk(1.0, 0.0) : cl;
-      if (cj) c.im = -c.im;
+      dcpx c = cf[k];
+      c.re = stat ? 1.0 : c.re;
+      c.im = stat ? 0.0 : (cj ? -c.im : c.im);
       const dcpx hh = cmul(fo, c);
       mr += hh.re;
-      if (!(w[j] & (1u << 20))) mi += (w[j] & (1u << 21)) ? hh.im : -hh.im;
-      // Re(qx d') with d' = conj(d) for a conjugated coefficient is Re(conj(qx) d)
-      double xi = (w[j] & (1u << 20)) ? 0.0 : yi;
-      if (w[j] & (1u << 21)) xi = -xi;
-      const dcpx qx = cmul(fo, mk(yr, xi));
+      mi += (wj & (1u << 20)) ? 0.0 : ((wj & (1u << 21)) ? hh.im : -hh.im);
+      // the term's contraction with its coefficient's derivatives, read from
+      // the stage row (weight zero for a static term); Re(qx d') with
+      // d' = conj(d) for a conjugated coefficient is Re(conj(qx) d)
+      double xi = (wj & (1u << 20)) ? 0.0 : yi;
+      xi = (wj & (1u << 21)) ? -xi : xi;
+      const double zw = stat ? 0.0 : 1.0;
+      const dcpx qx = cmul(fo, mk(yr * zw, xi * zw));
       const double qi = cj ? -qx.im : qx.im;
+      const dcpx* dk = dc + k * nd;
 #pragma unroll
-      for (int k = 0; k < LB_AW_MAX_COEF; ++k) {
-        if (k >= P.K) break;  // uniform: only the plan's coefficients
-        if ((int)cid == k) {
-          Q[k].re += qx.re;
-          Q[k].im += qi;
-        }
+      for (int m = 0; m < LB_MAX_DIFF; ++m) {
+        if (m >= np) break;
+        const dcpx d = dk[m];
+        g[m] += qx.re * d.re - qi * d.im;
       }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < LB_AW_MAX_COEF; ++k) {
-    if (k >= P.K) break;
-#pragma unroll
-    for (int m = 0; m < LB_MAX_DIFF; ++m) {
-      if (m >= np) break;
-      const dcpx d = dc[k * nd + m];
-      g[m] += Q[k].re * d.re - Q[k].im * d.im;
-    }
-    if (P.diff_time) {
-      const dcpx d = dc[k * nd + np];
-      tc += Q[k].re * d.re - Q[k].im * d.im;
+      if (P.diff_time) {
+        const dcpx d = dk[np];
+        tc += qx.re * d.re - qi * d.im;
+      }
     }
   }
 }
