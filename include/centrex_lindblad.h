@@ -364,7 +364,9 @@ int lb_solve(const lb_plan* plan, const lb_solve_args* args, void* stream);
  * fixed.  `args` must describe the same trajectories, parameters, y0
  * support and tape as the lb_solve that recorded the tape.  The library
  * keeps a per-plan workspace of ckpt_every recomputed attempts per resident
- * workgroup.
+ * workgroup.  A trajectory whose tape is empty or longer than tape_cap gets
+ * NaN gradients; so does every trajectory a workgroup of the single-block
+ * adjoint handles after one of its bounded waits between waves ran out.
  */
 int lb_adjoint(const lb_plan* plan, const lb_solve_args* args, void* stream);
 
