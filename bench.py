@@ -131,6 +131,28 @@ def uneven_leg(steps: int, warmup: int, cpu_seconds: float) -> dict:
         if solver == "dopri5":
             out["value"] = n_traj / tuned_s
             pops = [r.populations for r in res]
+            # roofline of the batch: the reference algorithm's FLOPs (full
+            # plans) and the FLOPs of the restricted tables the kernels run,
+            # over the grouped call's time; the n=72 group sets that time
+            roof = {"bound": "fp64_valu", "latency_bound": True, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "seconds": tuned_s, "groups": {}}
+            alg_all = exe_all = 0.0
+            for grp, r, label in zip(groups, res, ("n39", "n72")):
+                plan = grp.prepared.device_plan()
+                y0 = grp.prepared.layout.pack(grp.rho0)
+                info, rinfo = plan.info, plan.support_info(plan.support_of(y0))
+                st = r.stats.cpu().numpy() if hasattr(r.stats, "cpu") else np.asarray(r.stats)
+                alg = algorithmic_flops(st, info["flops_per_rhs"], info["dim"], solver)
+                exe = algorithmic_flops(st, rinfo["flops_per_rhs"], rinfo["dim"], solver)
+                roof["groups"][label] = {"flops": alg, "executed_flops": exe, "elements": int(rinfo["upper_len"]),
+                                         "f_rhs_full": int(info["flops_per_rhs"]),
+                                         "f_rhs_restricted": int(rinfo["flops_per_rhs"])}
+                alg_all += alg
+                exe_all += exe
+            roof["achieved"] = alg_all / tuned_s / 1e12
+            roof["frac"] = roof["achieved"] / FP64_PEAK_TFLOPS
+            roof["executed_frac"] = exe_all / tuned_s / 1e12 / FP64_PEAK_TFLOPS
+            out["roofline"] = roof
     # ---- CPU oracle on all host cores: a seeded sample of each system solved
     # as one batch (per-trajectory end times), Dopri5; the mixed batch's rate
     # follows from the two measured rates
