@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1420,8 +1421,18 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
   // plan.  args->variant = k + 1 forces
   // adjoint variant k (both forms compute the same derivative).
   const int S = im->dev.S;
-  const int wave_bufs =
+  int wave_bufs =
       lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every, 2) <= kMaxLdsBytes ? 2 : 1;
+  // Polls of one wait between the single-block adjoint's waves: about a
+  // second, far beyond the longest legitimate wait (one segment's rows and
+  // recomputation, well under a millisecond).  Diagnostic hooks for tests:
+  // CENTREX_LB_ADJOINT_BUFFERS=1 forces one segment buffer (two waves),
+  // CENTREX_LB_ADJOINT_WAIT_LIMIT sets the bound (negative: every wait counts
+  // as run out, so the launch ends with NaN gradients).
+  c.adj_wait_limit = 1 << 22;
+  if (const char* e = std::getenv("CENTREX_LB_ADJOINT_BUFFERS"))
+    if (std::atoi(e) == 1) wave_bufs = 1;
+  if (const char* e = std::getenv("CENTREX_LB_ADJOINT_WAIT_LIMIT")) c.adj_wait_limit = std::atoi(e);
   const int wave_threads = wave_bufs == 2 ? 192 : 128;
   const int64_t lds_wave = lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every, wave_bufs);
   auto wave_fits = [&](int rows) {

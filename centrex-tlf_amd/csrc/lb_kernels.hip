@@ -3003,10 +3003,6 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every, int nbuf)
 // time column last); each term is contracted with its coefficient's
 // derivatives as it is read.
 constexpr int kAwChunk = 4;  // transposed rows fetched together
-// Polls of one exchange wait before the launch is marked broken: about a
-// second, far beyond the longest legitimate wait (one segment's rows and
-// recomputation, well under a millisecond).
-constexpr int kAwSpinLimit = 1 << 22;
 
 __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW& L, int r0, int r1, bool own,
                                                    int lane, double kr, double ki, double yr, double yi, bool diag,
@@ -3123,8 +3119,9 @@ __global__ __launch_bounds__(192, WAVES) void lb_adjoint_wave_kernel(const DevPl
   if (tid == 0) {
     L.misc[1] = 0;  // exchanges posted by sweeping wave 0
     L.misc[2] = 0;  // ... and by sweeping wave 1
-    L.misc[3] = 0;  // 1: an exchange wait ran out (the gradients become NaN)
+    L.misc[3] = C.adj_wait_limit < 0 ? 1 : 0;  // 1: an exchange wait ran out (the gradients become NaN)
   }
+  const int wait_limit = max(C.adj_wait_limit, 0);
   int par = 0, posted = 0;  // exchange buffer and count of this wave's exchanges
   bool broken = false;
   // mu (or a scalar pair) summed over the two sweeping waves, wave 0's part
@@ -3153,7 +3150,7 @@ __global__ __launch_bounds__(192, WAVES) void lb_adjoint_wave_kernel(const DevPl
     for (int it = 0; !broken; ++it) {
       if (__hip_atomic_load(&L.misc[2 - wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= posted) break;
       if (__hip_atomic_load(&L.misc[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0 ||
-          it >= kAwSpinLimit) {
+          it >= wait_limit) {
         broken = true;
         if (lane == 0) __hip_atomic_store(&L.misc[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         break;

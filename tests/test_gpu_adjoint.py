@@ -181,6 +181,40 @@ def test_adjoint_kernel_forms_agree(setup, mode):
     assert np.all(err <= 1e-12), err
 
 
+@pytest.mark.parametrize("mode", ["adaptive", "frozen"])
+def test_single_block_adjoint_on_two_and_three_waves_is_bitwise_identical(setup, monkeypatch, mode):
+    """The single-block adjoint prepares each checkpoint segment on a third
+    wave into a second buffer; with one buffer (two waves) the first wave
+    prepares it before the sweep.  The sweep's arithmetic and its split over
+    the two sweeping waves are the same, so the gradients are the same bits."""
+    torch, prepared, y0, t_span, weights = setup
+    engine = _engine(prepared, t_span, weights, mode=mode)
+    p = _batch(37, seed=11)
+    _, tape = engine.forward(p, y0)
+    three = engine.gradient(p, y0, tape).cpu().numpy()
+    monkeypatch.setenv("CENTREX_LB_ADJOINT_BUFFERS", "1")
+    two = engine.gradient(p, y0, tape).cpu().numpy()
+    monkeypatch.delenv("CENTREX_LB_ADJOINT_BUFFERS")
+    assert np.all(np.isfinite(three))
+    np.testing.assert_array_equal(two, three)
+
+
+def test_single_block_adjoint_wait_that_runs_out_gives_nan(setup, monkeypatch):
+    """A wait between the sweeping waves that runs out (forced from the host:
+    every wait counts as run out) ends the launch with NaN gradients rather
+    than holding the GPU, and leaves the plan usable."""
+    torch, prepared, y0, t_span, weights = setup
+    engine = _engine(prepared, t_span, weights)
+    p = _batch(9, seed=12)
+    _, tape = engine.forward(p, y0)
+    want = engine.gradient(p, y0, tape).cpu().numpy()
+    monkeypatch.setenv("CENTREX_LB_ADJOINT_WAIT_LIMIT", "-1")
+    broken = engine.gradient(p, y0, tape).cpu().numpy()
+    monkeypatch.delenv("CENTREX_LB_ADJOINT_WAIT_LIMIT")
+    assert np.all(np.isnan(broken))
+    np.testing.assert_array_equal(engine.gradient(p, y0, tape).cpu().numpy(), want)
+
+
 def test_autograd_function(setup):
     torch, prepared, y0, t_span, weights = setup
     engine = _engine(prepared, t_span, weights)
