@@ -128,6 +128,10 @@ struct DevPlan {
   const double* base;           // n_base*2
 };
 
+// Waves that sweep a checkpoint segment of the single-block adjoint (each
+// applies its share of the transposed rows of every reverse stage).
+#define LB_AW_SWEEPERS 4
+
 struct DevCall {
   int64_t n_traj;
   double t0, t1, abstol, reltol, dt;

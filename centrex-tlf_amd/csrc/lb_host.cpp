@@ -1374,14 +1374,14 @@ int64_t lds_bytes_adj_wave(const DevPlan& P, const std::vector<int32_t>& g_rows,
   return g * 64 * 16 + (int64_t)t_rows * 64 * 16 + KC * 16 + (int64_t)nbuf * every * 7 * odd(KC) * 16 +
          (int64_t)P.n_slots * 16 + 64 * odd(P.n_slots) * 16 + 64 * odd(P.vm_depth) * 16 + 64 * 16 +
          (int64_t)LB_MAX_DIFF * 64 * 16 +
-         (int64_t)((P.n + 1) & ~1) * 8 + g * 64 * 4 + (int64_t)t_rows * 64 * 4 + 4 * 64 * 16 +
-         (8 + 2 * (int64_t)LB_MAX_DIFF) * 8 + 16;
+         (int64_t)((P.n + 1) & ~1) * 8 + g * 64 * 4 + (int64_t)t_rows * 64 * 4 + 2 * LB_AW_SWEEPERS * 64 * 16 +
+         (4 + (int64_t)LB_MAX_DIFF) * LB_AW_SWEEPERS * 8 + 8 * 4;
 }
 
-// The single-block adjoint runs three waves per trajectory when two segment
-// buffers fit in LDS (two sweep a segment while the third prepares the next),
-// two otherwise (one buffer; the first wave prepares each segment before the
-// sweep).
+// The single-block adjoint runs LB_AW_SWEEPERS + 1 waves per trajectory when
+// two segment buffers fit in LDS (LB_AW_SWEEPERS sweep a segment while the
+// last prepares the next), LB_AW_SWEEPERS otherwise (one buffer; the first
+// wave prepares each segment before the sweep).
 
 }  // namespace
 
@@ -1416,24 +1416,24 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
                 "the adjoint of the adaptive solve needs the time derivative programs (lb_plan_desc.diff_time = 1)");
   c.adj_mode = a->adjoint_mode;
   // A plan of one 64-slot block whose gather rows and LDS fit takes the
-  // single-block adjoint (lane = slot, two waves per trajectory); otherwise
-  // the multi-wave variant with the fewest slots per thread that covers the
-  // plan.  args->variant = k + 1 forces
-  // adjoint variant k (both forms compute the same derivative).
+  // single-block adjoint (lane = slot, several waves per trajectory);
+  // otherwise the multi-wave variant with the fewest slots per thread that
+  // covers the plan.  args->variant = k + 1 forces adjoint variant k (both
+  // forms compute the same derivative).
   const int S = im->dev.S;
   int wave_bufs =
       lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every, 2) <= kMaxLdsBytes ? 2 : 1;
   // Polls of one wait between the single-block adjoint's waves: about a
   // second, far beyond the longest legitimate wait (one segment's rows and
   // recomputation, well under a millisecond).  Diagnostic hooks for tests:
-  // CENTREX_LB_ADJOINT_BUFFERS=1 forces one segment buffer (two waves),
+  // CENTREX_LB_ADJOINT_BUFFERS=1 forces one segment buffer (no preparing wave),
   // CENTREX_LB_ADJOINT_WAIT_LIMIT sets the bound (negative: every wait counts
   // as run out, so the launch ends with NaN gradients).
   c.adj_wait_limit = 1 << 22;
   if (const char* e = std::getenv("CENTREX_LB_ADJOINT_BUFFERS"))
     if (std::atoi(e) == 1) wave_bufs = 1;
   if (const char* e = std::getenv("CENTREX_LB_ADJOINT_WAIT_LIMIT")) c.adj_wait_limit = std::atoi(e);
-  const int wave_threads = wave_bufs == 2 ? 192 : 128;
+  const int wave_threads = 64 * (wave_bufs == 2 ? LB_AW_SWEEPERS + 1 : LB_AW_SWEEPERS);
   const int64_t lds_wave = lds_bytes_adj_wave(im->dev, im->g_rows_host, im->t_rows_host, a->ckpt_every, wave_bufs);
   auto wave_fits = [&](int rows) {
     return im->dev.nblk == 1 && !im->g_rows_host.empty() && im->g_rows_host[0] <= rows &&

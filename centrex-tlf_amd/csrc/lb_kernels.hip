@@ -2929,25 +2929,25 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint for plans of one 64-slot block (lane = slot), two waves per
+// Adjoint for plans of one 64-slot block (lane = slot), several waves per
 // trajectory.
 //
 // The same adjoint as lb_adjoint_kernel, arranged for latency: the gather and
-// transposed term tables live in LDS for the whole launch; the coefficient
-// rows of a checkpoint segment (every stage time of its attempts is on the
-// tape) are evaluated in one pass, one (attempt, stage) per lane of the first
-// wave, each lane with a private copy of the parameter slots and its own
-// expression stack; the first wave recomputes the segment with the
+// transposed term tables live in LDS for the whole launch.  One wave prepares
+// checkpoint segments: it evaluates the coefficient rows of a segment (every
+// stage time of its attempts is on the tape) in one pass, one (attempt,
+// stage) per lane, each lane with a private copy of the parameter slots and
+// its own expression stack, and recomputes the segment with the
 // register/cross-lane RHS of the single-wave solver (the same arithmetic, so
-// the stage derivatives are those of the recorded solve); the reverse stages
-// read kappa of other elements by cross-lane permutes and fold the parameter
-// and time contractions of every term into the transposed pass.  Both waves
-// sweep every attempt with the same per-element state; each takes half of the
-// transposed rows of every reverse stage, and the two partial sums of mu and
-// of the step-size and time cotangents meet in LDS (first wave's part first),
-// so both waves continue with the same values.  The self terms of an element
-// (time-independent coefficients) and their parameter derivatives are folded
-// once per trajectory and counted by the first wave.
+// the stage derivatives are those of the recorded solve).  LB_AW_SWEEPERS
+// waves sweep the attempts with the same per-element state; each applies its
+// share of the transposed rows of every reverse stage (kappa of other
+// elements by cross-lane permutes, each term contracted with its
+// coefficient's parameter and time derivatives), and the partial sums of mu
+// and of the step-size and time cotangents meet in LDS in a fixed order, so
+// every sweeping wave continues with the same values.  The self terms of an
+// element (time-independent coefficients) and their parameter derivatives
+// are folded once per trajectory and counted by the first wave.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -2964,8 +2964,9 @@ struct LdsAW {
   double* wts;     // n integral weights
   uint32_t* gw;    // g_rows x 64 gather words
   uint32_t* tw;    // t_rows x 64 transposed words
-  double2* part;   // 2 x 2 x 64: per-wave partial mu of a reverse stage (two buffers)
-  double* xs;      // 8 + 2 * LB_MAX_DIFF: per-wave partial step/time cotangents, gradient sums
+  double2* part;   // 2 x LB_AW_SWEEPERS x 64: per-wave partial mu of a reverse stage (two buffers)
+  double* xs;      // 4 * LB_AW_SWEEPERS: per-wave partial step/time cotangents (two buffers),
+                   // then LB_AW_SWEEPERS * LB_MAX_DIFF per-wave gradient sums
   int* misc;
 };
 
@@ -2989,8 +2990,8 @@ __device__ __forceinline__ LdsAW carve_aw(const DevPlan& P, int every, int nbuf)
   L.wts = reinterpret_cast<double*>(q); q += (size_t)((P.n + 1) & ~1) * 8;
   L.gw = reinterpret_cast<uint32_t*>(q); q += (size_t)g * 64 * 4;
   L.tw = reinterpret_cast<uint32_t*>(q); q += (size_t)t * 64 * 4;
-  L.part = reinterpret_cast<double2*>(q); q += (size_t)4 * 64 * 16;
-  L.xs = reinterpret_cast<double*>(q); q += (size_t)(8 + 2 * LB_MAX_DIFF) * 8;
+  L.part = reinterpret_cast<double2*>(q); q += (size_t)2 * LB_AW_SWEEPERS * 64 * 16;
+  L.xs = reinterpret_cast<double*>(q); q += (size_t)(4 + LB_MAX_DIFF) * LB_AW_SWEEPERS * 8;
   L.misc = reinterpret_cast<int*>(q);
   return L;
 }
@@ -3084,15 +3085,18 @@ __device__ __forceinline__ void adjoint_wave_stage(const DevPlan& P, const LdsAW
 }  // namespace
 
 template <int SOLVER, int G, int WAVES>
-__global__ __launch_bounds__(192, WAVES) void lb_adjoint_wave_kernel(const DevPlan P, const DevCall C) {
+__global__ __launch_bounds__(64 * (LB_AW_SWEEPERS + 1), WAVES) void lb_adjoint_wave_kernel(const DevPlan P,
+                                                                                          const DevCall C) {
   using T = Tab<SOLVER>;
   const int tid = threadIdx.x, lane = tid & 63;
-  // wave index (uniform) and waves per trajectory: with three, waves 0 and 1
-  // sweep a segment's attempts while wave 2 evaluates the coefficient rows of
-  // the next segment and recomputes it into the other of two buffers; with
-  // two, wave 0 does that work before the sweep, into one buffer
+  // wave index (uniform) and waves per trajectory: with LB_AW_SWEEPERS + 1,
+  // waves 0 .. LB_AW_SWEEPERS - 1 sweep a segment's attempts while the last
+  // evaluates the coefficient rows of the next segment and recomputes it into
+  // the other of two buffers; with LB_AW_SWEEPERS (or fewer), wave 0 does
+  // that work before the sweep, into one buffer
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = (int)(blockDim.x >> 6);
-  const int nrev = nw >= 2 ? 2 : 1, prod = nw >= 3 ? 2 : 0, nbuf = nw >= 3 ? 2 : 1;
+  const bool ahead = nw > LB_AW_SWEEPERS;
+  const int nrev = ahead ? LB_AW_SWEEPERS : nw, prod = ahead ? LB_AW_SWEEPERS : 0, nbuf = ahead ? 2 : 1;
   const bool rev = wv < nrev, lead = wv == 0;
   const int every = C.ckpt_every;
   const LdsAW L = carve_aw(P, every, nbuf);
@@ -3100,11 +3104,10 @@ __global__ __launch_bounds__(192, WAVES) void lb_adjoint_wave_kernel(const DevPl
   const bool adaptive = C.adj_mode == LB_ADJOINT_ADAPTIVE;
   const double dimf = (double)P.dim;
   const int grows = cst(P.blk_rows)[0], trows = cst(P.t_blk_rows)[0];
-  // this wave's transposed rows: the first sweeping wave takes the first half
-  // of the chunks, the second the rest
-  const int nch = (trows + kAwChunk - 1) / kAwChunk;
-  const int cut = nrev > 1 ? min(trows, (nch + 1) / 2 * kAwChunk) : trows;
-  const int tr0 = lead ? 0 : cut, tr1 = lead ? cut : trows;
+  // this wave's transposed rows: the chunks of kAwChunk rows in equal runs,
+  // the first sweeping wave's first
+  const int nch = (trows + kAwChunk - 1) / kAwChunk, cpw = (nch + nrev - 1) / nrev;
+  const int tr0 = min(trows, wv * cpw * kAwChunk), tr1 = min(trows, (wv + 1) * cpw * kAwChunk);
   Slots<1> S;
   load_slots<1>(P, S, lane, 64);
   const bool diag = S.diag(0), valid = S.valid(0);
@@ -3116,21 +3119,25 @@ __global__ __launch_bounds__(192, WAVES) void lb_adjoint_wave_kernel(const DevPl
     L.tf[i] = reinterpret_cast<const double2*>(P.t_f)[i];
     L.tw[i] = P.t_word[i];
   }
+  // misc[1 + w]: exchanges posted by sweeping wave w; misc[LB_AW_SWEEPERS + 1]:
+  // 1 when an exchange wait ran out (the gradients become NaN)
+  int* const posts = L.misc + 1;
+  int* const broke = L.misc + 1 + LB_AW_SWEEPERS;
   if (tid == 0) {
-    L.misc[1] = 0;  // exchanges posted by sweeping wave 0
-    L.misc[2] = 0;  // ... and by sweeping wave 1
-    L.misc[3] = C.adj_wait_limit < 0 ? 1 : 0;  // 1: an exchange wait ran out (the gradients become NaN)
+    for (int w = 0; w < LB_AW_SWEEPERS; ++w) posts[w] = 0;
+    *broke = C.adj_wait_limit < 0 ? 1 : 0;
   }
   const int wait_limit = max(C.adj_wait_limit, 0);
   int par = 0, posted = 0;  // exchange buffer and count of this wave's exchanges
   bool broken = false;
-  // mu (or a scalar pair) summed over the two sweeping waves, wave 0's part
-  // first.  The waves meet through LDS flags, not a workgroup barrier (the
-  // third wave works on the next segment meanwhile): each posts its part and
-  // its exchange count, then waits until the other has posted the same count.
-  // Two buffers alternate, so a wave rewrites one only after the other has
-  // passed the following exchange, that is after it has read it.  The wait
-  // is bounded: a wave that waits too long marks the launch broken and every
+  // mu (or a scalar pair) summed over the sweeping waves in a fixed tree,
+  // ((w0 + w1) + (w2 + w3)).  The waves meet through LDS flags, not a
+  // workgroup barrier (the preparing wave works on the next segment
+  // meanwhile): each posts its part and its exchange count, then waits until
+  // every other sweeping wave has posted the same count.  Two buffers
+  // alternate, so a wave rewrites one only after the others have passed the
+  // following exchange, that is after they have read it.  The wait is
+  // bounded: a wave that waits too long marks the launch broken and every
   // later wait returns at once, so the kernel always ends.
   auto meet = [&](double a, double b, double& sa, double& sb, bool per_lane) {
     if (nrev == 1) {
@@ -3138,34 +3145,50 @@ __global__ __launch_bounds__(192, WAVES) void lb_adjoint_wave_kernel(const DevPl
       sb = b;
       return;
     }
+    double2* const pp = L.part + (size_t)par * LB_AW_SWEEPERS * 64;
+    double* const xp = L.xs + par * 2 * LB_AW_SWEEPERS;
     if (per_lane) {
-      L.part[(par * 2 + wv) * 64 + lane] = double2{a, b};
+      pp[wv * 64 + lane] = double2{a, b};
     } else if (lane == 0) {
-      L.xs[par * 4 + wv * 2] = a;
-      L.xs[par * 4 + wv * 2 + 1] = b;
+      xp[wv * 2] = a;
+      xp[wv * 2 + 1] = b;
     }
     ++posted;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (lane == 0) __hip_atomic_store(&L.misc[1 + wv], posted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    for (int it = 0; !broken; ++it) {
-      if (__hip_atomic_load(&L.misc[2 - wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= posted) break;
-      if (__hip_atomic_load(&L.misc[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0 ||
-          it >= wait_limit) {
-        broken = true;
-        if (lane == 0) __hip_atomic_store(&L.misc[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        break;
+    if (lane == 0) __hip_atomic_store(&posts[wv], posted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (int w = 0; w < nrev; ++w) {
+      if (w == wv) continue;
+      for (int it = 0; !broken; ++it) {
+        if (__hip_atomic_load(&posts[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= posted) break;
+        if (__hip_atomic_load(broke, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0 || it >= wait_limit) {
+          broken = true;
+          if (lane == 0) __hip_atomic_store(broke, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
       }
-      __builtin_amdgcn_s_sleep(2);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    double ra = 0.0, rb = 0.0;
     if (per_lane) {
-      const double2 u = L.part[par * 2 * 64 + lane], v = L.part[(par * 2 + 1) * 64 + lane];
-      sa = u.x + v.x;
-      sb = u.y + v.y;
+      const double2 u0 = pp[lane], u1 = pp[64 + lane];
+      ra = u0.x + u1.x;
+      rb = u0.y + u1.y;
+      if (nrev > 2) {
+        const double2 u2 = pp[128 + lane], u3 = nrev > 3 ? pp[192 + lane] : double2{0.0, 0.0};
+        ra = ra + (u2.x + u3.x);
+        rb = rb + (u2.y + u3.y);
+      }
     } else {
-      sa = L.xs[par * 4] + L.xs[par * 4 + 2];
-      sb = L.xs[par * 4 + 1] + L.xs[par * 4 + 3];
+      ra = xp[0] + xp[2];
+      rb = xp[1] + xp[3];
+      if (nrev > 2) {
+        ra = ra + (xp[4] + (nrev > 3 ? xp[6] : 0.0));
+        rb = rb + (xp[5] + (nrev > 3 ? xp[7] : 0.0));
+      }
     }
+    sa = ra;
+    sb = rb;
     par ^= 1;
   };
   Lds LV{};  // the gather tables as the single-wave RHS reads them
@@ -3442,14 +3465,15 @@ __global__ __launch_bounds__(192, WAVES) void lb_adjoint_wave_kernel(const DevPl
     for (int m = 0; m < LB_MAX_DIFF; ++m) {
       if (m >= np) break;
       const double v = wave_sum(g[m]);
-      if (rev && lane == 0) L.xs[8 + wv * LB_MAX_DIFF + m] = v;
+      if (rev && lane == 0) L.xs[4 * LB_AW_SWEEPERS + wv * LB_MAX_DIFF + m] = v;
     }
     __syncthreads();
     if (tid == 0) {
-      const bool fine = ok && L.misc[3] == 0;
+      const bool fine = ok && *broke == 0;
+      const double* gs = L.xs + 4 * LB_AW_SWEEPERS;
       for (int m = 0; m < np; ++m) {
-        double v = L.xs[8 + m];
-        if (nrev > 1) v += L.xs[8 + LB_MAX_DIFF + m];
+        double v = gs[m];
+        for (int w = 1; w < nrev; ++w) v += gs[w * LB_MAX_DIFF + m];
         C.out_grad[traj * np + m] = fine ? v : __builtin_nan("");
       }
     }

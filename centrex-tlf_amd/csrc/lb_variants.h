@@ -38,7 +38,7 @@
 #define LB_ADJ_CONFIGS(Z) Z(0, 1, 1024, 2) Z(1, 2, 512, 2) Z(2, 4, 256, 2) Z(3, 8, 256, 2)
 
 // Single-block adjoint variants for plans of one 64-slot block (lane = slot,
-// two or three waves per trajectory): V(id, gather rows held in registers by
+// several waves per trajectory): V(id, gather rows held in registers by
 // the recomputation, waves-per-SIMD hint), in increasing rows (the host takes
 // the first that covers the plan).
 #define LB_ADJ_WAVE_CONFIGS(V) V(4, 12, 2) V(5, 16, 2)

@@ -182,11 +182,11 @@ def test_adjoint_kernel_forms_agree(setup, mode):
 
 
 @pytest.mark.parametrize("mode", ["adaptive", "frozen"])
-def test_single_block_adjoint_on_two_and_three_waves_is_bitwise_identical(setup, monkeypatch, mode):
-    """The single-block adjoint prepares each checkpoint segment on a third
-    wave into a second buffer; with one buffer (two waves) the first wave
+def test_single_block_adjoint_with_and_without_preparing_wave_is_bitwise_identical(setup, monkeypatch, mode):
+    """The single-block adjoint prepares each checkpoint segment on its own
+    wave into a second buffer; with one buffer the first sweeping wave
     prepares it before the sweep.  The sweep's arithmetic and its split over
-    the two sweeping waves are the same, so the gradients are the same bits."""
+    the sweeping waves are the same, so the gradients are the same bits."""
     torch, prepared, y0, t_span, weights = setup
     engine = _engine(prepared, t_span, weights, mode=mode)
     p = _batch(37, seed=11)
