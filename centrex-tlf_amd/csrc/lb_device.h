@@ -186,6 +186,7 @@ struct DevCall {
   int32_t adj_mode;        // LB_ADJOINT_*
   int32_t adj_wait_limit;  // polls of one exchange wait between the waves of the single-block
                            // adjoint before the launch is marked broken (< 0: broken from the start)
+  int32_t adj_row_batch;   // multi-wave adjoint: attempts whose coefficient rows one pass evaluates
   // adjoint workspace (per workgroup) and output
   double* seg_y;           // workgroups x ckpt_every x 8 x S x 2: y and k1..k7 of each attempt
   double* seg_c;           // workgroups x ckpt_every x 7 x (KD + 1) x 2: coefficient rows

@@ -1360,10 +1360,11 @@ extern "C" int lb_solve(const lb_plan* p, const lb_solve_args* a, void* stream) 
 namespace {
 
 // Must match carve_adj() in lb_kernels.hip.
-int64_t lds_bytes_adj(const DevPlan& P) {
+int64_t lds_bytes_adj(const DevPlan& P, int E) {
+  const int64_t rows = 1 + 7 * (int64_t)E;
   return 8 * (2 * (int64_t)P.n_pub) + 8 * 2 * (int64_t)P.S + 8 * 2 * (int64_t)P.S +
-         8 * (int64_t)((P.n + 1) & ~1) + 16 * 8 * (int64_t)(P.KD + 1) + 16 * 8 * (int64_t)P.n_slots +
-         8 * (int64_t)LB_MAX_DIFF * 16 + 16 * 8 * (int64_t)P.vm_depth + 16;
+         8 * (int64_t)((P.n + 1) & ~1) + 16 * rows * (int64_t)(P.KD + 1) + 16 * rows * (int64_t)P.n_slots +
+         8 * (int64_t)LB_MAX_DIFF * 16 + 16 * std::max<int64_t>(8, 7 * (int64_t)E) * (int64_t)P.vm_depth + 16;
 }
 
 // LDS of the single-block adjoint with `nbuf` segment buffers (carve_aw in
@@ -1461,7 +1462,13 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
   if (id < 0) return fail(LB_ERR_UNSUPPORTED, "plan is too large for the adjoint kernel variants");
   void* fn = lb_dev_adjoint_kernel(id, a->solver);
   if (!fn) return fail(LB_ERR_UNSUPPORTED, "adjoint kernel variant is not built");
-  const size_t lds = (size_t)(wave ? lds_wave : lds_bytes_adj(im->dev));
+  // the multi-wave form evaluates the coefficient rows of E attempts per pass:
+  // the largest E of 4, 2 whose LDS leaves room for two workgroups per CU, else 1
+  int row_batch = 1;
+  for (int e : {4, 2})
+    if (lds_bytes_adj(im->dev, e) <= kMaxLdsBytes / 2) { row_batch = e; break; }
+  c.adj_row_batch = row_batch;
+  const size_t lds = (size_t)(wave ? lds_wave : lds_bytes_adj(im->dev, row_batch));
   if ((rc = set_lds_limit(fn, lds))) return rc;
   int grid = a->workgroups;
   if (grid <= 0) {

@@ -2411,25 +2411,26 @@ struct LdsAdj {
   double* kap;    // 2*S: published kappa_i by slot
   double* selfc;  // 2*S folded self coefficients
   double* wts;    // n integral weights per state
-  dcpx* coef;     // 8 rows x (KD + 1): [0] t0, [1..7] stage times
-  dcpx* slots;    // 8 x n_slots
+  dcpx* coef;     // 1 + 7 E rows x (KD + 1): [0] t0, then the stage times of E attempts
+  dcpx* slots;    // 1 + 7 E slot copies (n_slots each)
   double* red;    // LB_MAX_DIFF x 16 per-wave sums
   dcpx* vm;       // 8 lanes x vm_depth
   int* misc;
 };
 
-__device__ __forceinline__ LdsAdj carve_adj(const DevPlan& P) {
+__device__ __forceinline__ LdsAdj carve_adj(const DevPlan& P, int E) {
   extern __shared__ double smem[];
+  const size_t rows = 1 + 7 * (size_t)E;
   LdsAdj L;
   L.pub = smem;
   L.kap = L.pub + 2 * (size_t)P.n_pub;
   L.selfc = L.kap + 2 * (size_t)P.S;
   L.wts = L.selfc + 2 * (size_t)P.S;
   L.coef = reinterpret_cast<dcpx*>(L.wts + ((P.n + 1) & ~1));
-  L.slots = L.coef + 8 * (P.KD + 1);
-  L.red = reinterpret_cast<double*>(L.slots + 8 * P.n_slots);
+  L.slots = L.coef + rows * (P.KD + 1);
+  L.red = reinterpret_cast<double*>(L.slots + rows * P.n_slots);
   L.vm = reinterpret_cast<dcpx*>(L.red + LB_MAX_DIFF * 16);
-  L.misc = reinterpret_cast<int*>(L.vm + 8 * P.vm_depth);
+  L.misc = reinterpret_cast<int*>(L.vm + max(8, 7 * E) * (size_t)P.vm_depth);
   return L;
 }
 
@@ -2581,7 +2582,9 @@ template <int R, int SOLVER, int MAXW, int WAVES>
 __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P, const DevCall C) {
   using T = Tab<SOLVER>;
   const int tid = threadIdx.x, W = blockDim.x;
-  const LdsAdj L = carve_adj(P);
+  // the coefficient rows of E attempts are evaluated in one pass (7 E lanes)
+  const int E = max(1, C.adj_row_batch);
+  const LdsAdj L = carve_adj(P, E);
   const int KC = P.KD + 1, np = P.n_diff - P.diff_time;
   const bool adaptive = C.adj_mode == LB_ADJOINT_ADAPTIVE;
   const double dimf = (double)P.dim;
@@ -2622,8 +2625,8 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
     __syncthreads();
     // stage rows start as the t0 row: the expression machine rewrites only the
     // time-dependent entries at each stage time
-    for (int i = tid; i < 7 * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
-    for (int i = tid; i < 7 * KC; i += W) L.coef[KC + i] = L.coef[i % KC];
+    for (int i = tid; i < 7 * E * P.n_slots; i += W) L.slots[P.n_slots + i] = L.slots[i % P.n_slots];
+    for (int i = tid; i < 7 * E * KC; i += W) L.coef[KC + i] = L.coef[i % KC];
     fold_self(P, tid, W, L.coef, L.selfc);
     __syncthreads();
 
@@ -2663,16 +2666,24 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
           }
         }
         for (int a = a0; a < a1; ++a) {
-          const double x = tt[a], h = th[a];
-          __syncthreads();  // previous users of the coefficient rows are done
-          if (tid < 7) {
-            const int st = tid + 1;
-            const double ts = st == 1 ? x : x + h * T::c(st);
-            eval_coefficients(P, L.slots + st * P.n_slots, L.coef + st * KC, ts, true, L.vm + tid * P.vm_depth,
-                              P.KD);
-            for (int q = 0; q < KC; ++q) segC[((a - a0) * 7 + tid) * KC + q] = L.coef[st * KC + q];
+          const double h = th[a];
+          const int jb = (a - a0) % E;  // this attempt's place in its batch of E
+          if (jb == 0) {
+            // the coefficient rows of the next E attempts, lane 7 j + s - 1
+            // for attempt a + j, stage s; kept for the sweep in segC
+            __syncthreads();  // previous users of the coefficient rows are done
+            const int nb = min(E, a1 - a);
+            if (tid < 7 * nb) {
+              const int aa = a + tid / 7, st = tid % 7 + 1;
+              const double ts = st == 1 ? tt[aa] : tt[aa] + th[aa] * T::c(st);
+              dcpx* const row = L.coef + (1 + tid) * KC;
+              eval_coefficients(P, L.slots + (1 + tid) * P.n_slots, row, ts, true, L.vm + tid * P.vm_depth, P.KD);
+              for (int q = 0; q < KC; ++q) segC[((aa - a0) * 7 + st - 1) * KC + q] = row[q];
+            }
+            __syncthreads();
           }
-          __syncthreads();
+          // stage s of this attempt at rows[s] (row 0 of the block is the t0 row)
+          const dcpx* const rows = L.coef + 7 * jb * KC;
           double k_r[8][R], k_i[8][R], yt_r[R], yt_i[R];
 #pragma unroll
           for (int r = 0; r < R; ++r) {
@@ -2683,7 +2694,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
               w0[1] = y_i[r];
             }
           }
-          rhs_adj_fwd<R>(P, L, S, y_r, y_i, k_r[1], k_i[1], L.pub, L.coef + KC, tid, W);
+          rhs_adj_fwd<R>(P, L, S, y_r, y_i, k_r[1], k_i[1], L.pub, rows + KC, tid, W);
 #define LB_ADJ_KEEP(st)                                             \
   _Pragma("unroll") for (int r = 0; r < R; ++r) {                   \
     const int s = tid + W * r;                                      \
@@ -2696,7 +2707,7 @@ __global__ __launch_bounds__(MAXW, WAVES) void lb_adjoint_kernel(const DevPlan P
           LB_ADJ_KEEP(1)
 #define LB_ADJ_FWD(st)                                                                              \
   stage_combo<R, SOLVER, st>(h, y_r, y_i, k_r, k_i, yt_r, yt_i);                                   \
-  rhs_adj_fwd<R>(P, L, S, yt_r, yt_i, k_r[st], k_i[st], L.pub, L.coef + (st) * KC, tid, W);        \
+  rhs_adj_fwd<R>(P, L, S, yt_r, yt_i, k_r[st], k_i[st], L.pub, rows + (st) * KC, tid, W);          \
   LB_ADJ_KEEP(st)
           LB_ADJ_FWD(2)
           LB_ADJ_FWD(3)
