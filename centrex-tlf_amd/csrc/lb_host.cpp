@@ -374,6 +374,36 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
     if (act[o]) order.push_back(o);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n_gather[a] > n_gather[b]; });
   const int S = std::max(64, (n_act + 63) / 64 * 64), nblk = S / 64;
+  // Blocks of 64 consecutive elements of that order, placed so that the
+  // default launch's waves get about equal gather rows: wave w of nw holds
+  // blocks w, w + nw, w + 2 nw, ..., so the blocks go, longest first, to the
+  // wave with the fewest rows so far (at its next free position).  A block's
+  // elements and their terms are unchanged; only its position moves.
+  std::vector<int> slot_el(S, -1);  // slot -> element, or -1
+  {
+    std::vector<int> brows(nblk, 0);
+    int rows_sum = 0;
+    for (int b = 0; b < nblk; ++b) {
+      for (int l = 0; l < 64 && 64 * b + l < n_act; ++l) brows[b] = std::max(brows[b], n_gather[order[64 * b + l]]);
+      rows_sum += brows[b];
+    }
+    std::vector<int> pos_of(nblk);
+    std::iota(pos_of.begin(), pos_of.end(), 0);
+    int id = -1, W = 0, R = 1, maxw = 0, waves = 0;
+    pick_config(S, rows_sum, &id, &W);
+    if (nblk > 1 && id >= 0 && lb_dev_variant(id, &R, &maxw, &waves) && !lb_dev_variant_rows(id) && R > 1) {
+      const int nw = W / 64;
+      std::vector<int> load(nw, 0), next_r(nw, 0);
+      for (int b = 0; b < nblk; ++b) {  // brows is non-increasing in b
+        int best = -1;
+        for (int w = 0; w < nw; ++w)
+          if (w + nw * next_r[w] < nblk && (best < 0 || load[w] < load[best])) best = w;
+        pos_of[b] = best + nw * next_r[best]++;
+        load[best] += brows[b];
+      }
+    }
+    for (int s0 = 0; s0 < n_act; ++s0) slot_el[64 * pos_of[s0 / 64] + s0 % 64] = order[s0];
+  }
   std::vector<int32_t> slot_up(S, -1), slot_re(S, -1), slot_im(S, -1), slot_pub(S, -1);
   std::vector<int32_t> pub_of_upper(U, -1);
   int n_pub = 0;
@@ -381,8 +411,9 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   // gather words name the owning lane (the single-wave kernels read inputs
   // by cross-lane permutes).
   const bool one_block = S == 64;
-  for (int s = 0; s < n_act; ++s) {
-    const int o = order[s];
+  for (int s = 0; s < S; ++s) {
+    const int o = slot_el[s];
+    if (o < 0) continue;
     slot_up[s] = o;
     slot_re[s] = src.up_re[o];
     slot_im[s] = src.up_im[o];
@@ -400,8 +431,8 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   for (int b = 0; b < nblk; ++b) {
     int rows = 0;
     for (int l = 0; l < 64; ++l) {
-      int s = 64 * b + l;
-      if (s < n_act) rows = std::max(rows, n_gather[order[s]]);
+      const int o = slot_el[64 * b + l];
+      if (o >= 0) rows = std::max(rows, n_gather[o]);
     }
     blk_row0[b] = rows_total;
     blk_rows[b] = rows;
@@ -409,8 +440,9 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   }
   std::vector<uint32_t> g_word((size_t)rows_total * 64, lb_gword(0, true, false, false, LB_STATIC_COEF));
   std::vector<double> g_f((size_t)rows_total * 64 * 2, 0.0);
-  for (int s = 0; s < n_act; ++s) {
-    const int o = order[s], b = s / 64, l = s % 64;
+  for (int s = 0; s < S; ++s) {
+    const int o = slot_el[s], b = s / 64, l = s % 64;
+    if (o < 0) continue;
     int js = 0, jg = 0;
     for (int64_t k = src.output_ptrs[o]; k < src.output_ptrs[o + 1]; ++k) {
       const int in = (int)src.input_indices[k];
@@ -438,7 +470,8 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
   int t_total = 0;
   if (src.n_diff > 0) {
     std::vector<int32_t> slot_of(U, -1);
-    for (int s = 0; s < n_act; ++s) slot_of[order[s]] = s;
+    for (int s = 0; s < S; ++s)
+      if (slot_el[s] >= 0) slot_of[slot_el[s]] = s;
     std::vector<std::vector<int64_t>> readers(S);
     for (int o = 0; o < U; ++o) {
       if (!act[o]) continue;
@@ -460,8 +493,9 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
     std::vector<int32_t> out_of(src.T, -1);
     for (int o = 0; o < U; ++o)
       for (int64_t k = src.output_ptrs[o]; k < src.output_ptrs[o + 1]; ++k) out_of[k] = o;
-    for (int s = 0; s < n_act; ++s) {
-      const int b = s / 64, l = s % 64, in = order[s];
+    for (int s = 0; s < S; ++s) {
+      const int b = s / 64, l = s % 64, in = slot_el[s];
+      if (in < 0) continue;
       int j = 0;
       for (int64_t k : readers[s]) {
         const int64_t ci = src.coeff_indices[k];
