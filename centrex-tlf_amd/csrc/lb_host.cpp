@@ -393,11 +393,19 @@ int build_image(const PlanSource& src, const std::vector<uint8_t>& act, DevImage
     pick_config(S, rows_sum, &id, &W);
     if (nblk > 1 && id >= 0 && lb_dev_variant(id, &R, &maxw, &waves) && !lb_dev_variant_rows(id) && R > 1) {
       const int nw = W / 64;
-      std::vector<int> load(nw, 0), next_r(nw, 0);
+      std::vector<int> load(nw, 0), next_r(nw, 0), cap(nw, 0);
+      for (int w = 0; w < nw; ++w)
+        while (w + nw * cap[w] < nblk) ++cap[w];
       for (int b = 0; b < nblk; ++b) {  // brows is non-increasing in b
+        // the least loaded wave with a free position; of equal loads, the
+        // one with the fewest free positions (it can take the fewest blocks)
         int best = -1;
-        for (int w = 0; w < nw; ++w)
-          if (w + nw * next_r[w] < nblk && (best < 0 || load[w] < load[best])) best = w;
+        for (int w = 0; w < nw; ++w) {
+          if (next_r[w] >= cap[w]) continue;
+          if (best < 0 || load[w] < load[best] ||
+              (load[w] == load[best] && cap[w] - next_r[w] < cap[best] - next_r[best]))
+            best = w;
+        }
         pos_of[b] = best + nw * next_r[best]++;
         load[best] += brows[b];
       }
