@@ -41,6 +41,19 @@ def tlf_n39():
     return system, params, rho0, (0.0, 20.0 / GAMMA), weights
 
 
+def coherent_ground(system, seed: int = 7):
+    """rho0 = |psi><psi| with psi a seeded random superposition of the X(J=0)
+    states.  Unlike the uniform mixture it is not invariant under rotations
+    about the quantisation axis, so photon counts of the x/y system depend on
+    the polarisation angle."""
+    rng = np.random.default_rng(seed)
+    ground = system.indices(electronic="X", J=0)
+    psi = np.zeros(system.n_states, dtype=np.complex128)
+    psi[ground] = rng.normal(size=len(ground)) + 1j * rng.normal(size=len(ground))
+    psi /= np.linalg.norm(psi)
+    return np.outer(psi, psi.conj())
+
+
 def tlf_n39_xy(theta: float = 0.25 * math.pi, sigma: float = 3.0 / GAMMA):
     """The n=39 system with the laser polarisation split into x and y:
     PX0 = cos(theta), PY0 = sin(theta), Gaussian envelope of width sigma.

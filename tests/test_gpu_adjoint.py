@@ -32,7 +32,7 @@ from centrex_tlf_amd.lindblad.adjoint import DifferentiablePhotonCount, photon_c
 from oracle.lb_oracle import Oracle
 
 from ._gpu import require_gpu
-from .systems import GAMMA, tlf_n39_xy, tlf_n72_theta
+from .systems import GAMMA, coherent_ground, tlf_n39_xy, tlf_n72_theta
 
 pytestmark = pytest.mark.gpu
 
@@ -46,22 +46,12 @@ def _batch(n, seed=20261202 + 4):
                      rng.uniform(2, 6, n) / GAMMA], axis=1)
 
 
-def _coherent_ground(system, seed=7):
-    """rho0 = |psi><psi| with psi a random superposition of the X(J=0) states."""
-    rng = np.random.default_rng(seed)
-    ground = system.indices(electronic="X", J=0)
-    psi = np.zeros(system.n_states, dtype=np.complex128)
-    psi[ground] = rng.normal(size=len(ground)) + 1j * rng.normal(size=len(ground))
-    psi /= np.linalg.norm(psi)
-    return np.outer(psi, psi.conj())
-
-
 @pytest.fixture(scope="module")
 def setup():
     torch = require_gpu()
     system, params, rho0, t_span, weights = tlf_n39_xy()
     prepared = prepare_lindblad_problem(system, params, backend="host")
-    y0 = prepared.layout.pack(_coherent_ground(system))
+    y0 = prepared.layout.pack(coherent_ground(system))
     return torch, prepared, y0, t_span, weights
 
 
@@ -165,6 +155,43 @@ def test_frozen_adjoint_matches_replay_finite_differences(setup):
     np.testing.assert_allclose(counts.cpu().numpy(), ref.cpu().numpy()[:, 0], rtol=1e-14, atol=0)
 
 
+def test_adjoint_from_uniform_mixture_paired_tape(setup):
+    """From the uniform mixture of the X(J=0) levels (the restricted plan fits
+    two trajectories per wave, so the tape comes from the paired forward) the
+    counts of a replay do not depend on theta: the frozen-step theta
+    derivative vanishes to rounding, and the other three derivatives meet the
+    bars above."""
+    torch, prepared, _, t_span, weights = setup
+    system, _, rho0, _, _ = tlf_n39_xy()
+    y0 = prepared.layout.pack(rho0)
+    keep3 = [0, 1, 3]
+    p = _batch(8, seed=5)
+    engine = _engine(prepared, t_span, weights)
+    counts, tape = engine.forward(p, y0)
+    scale = np.abs(counts.cpu().numpy()).max()
+    grad = engine.gradient(p, y0, tape).cpu().numpy()
+    frozen = engine.gradient(p, y0, tape, mode="frozen").cpu().numpy()
+    assert np.all(np.isfinite(grad)) and np.all(np.isfinite(frozen))
+    fd, same = _fd(_gpu_forward(engine, y0), p)
+    fd_r, _ = _fd(lambda q: (engine.replay(q, y0, tape).cpu().numpy(), np.zeros(len(q))), p)
+    err = _rel_errors(grad, fd, same)
+    err_r = _rel_errors(frozen[:, keep3], fd_r[:, keep3], np.ones(len(p), dtype=bool))
+    theta_frozen = np.abs(p[:, 2] * frozen[:, 2]).max() / scale
+    theta_adaptive = np.abs(p[:, 2] * grad[:, 2]).max() / scale
+    print(f"uniform mixture: adaptive vs FD {np.array2string(err, precision=2)} ({int(same.sum())} of 8 kept); "
+          f"frozen vs replay FD {np.array2string(err_r, precision=2)}; theta |p dN/dp| / N: frozen "
+          f"{theta_frozen:.2e}, adaptive {theta_adaptive:.2e}")
+    assert same.sum() >= 6
+    assert np.all(err[keep3] <= 1e-5), err
+    assert np.all(err_r <= 1e-8), err_r
+    assert theta_frozen <= 1e-10
+    # the adaptive map depends on theta only through the step-size control (the
+    # error norm weighs real and imaginary parts of the coherences apart): a
+    # derivative about 1e-4 of the count, whose central differences carry
+    # rounding of about 1e-10 / h relative to it, so it is held to 1e-4
+    assert err[2] <= 1e-4, err
+
+
 @pytest.mark.parametrize("mode", ["adaptive", "frozen"])
 def test_adjoint_kernel_forms_agree(setup, mode):
     """The single-wave adjoint (variant 5, the default for this plan) and the
@@ -238,7 +265,7 @@ def test_adjoint_n72_matches_finite_differences():
     prepared = prepare_lindblad_problem(system, params, backend="host")
     names = ["delta0", "Omega0", "theta"]
     engine = DifferentiablePhotonCount(prepared, names, weights, (0.0, 2.0 / GAMMA), tape_capacity=4096, **TOL)
-    y0 = prepared.layout.pack(_coherent_ground(system))  # breaks the symmetry that zeroes d/dtheta
+    y0 = prepared.layout.pack(coherent_ground(system))  # breaks the symmetry that zeroes d/dtheta
     p = np.asarray([[0.4 * GAMMA, 1.2 * GAMMA, 0.3], [-0.8 * GAMMA, 2.0 * GAMMA, 1.1]])
     counts, tape = engine.forward(p, y0)
     frozen = engine.gradient(p, y0, tape, mode="frozen").cpu().numpy()
