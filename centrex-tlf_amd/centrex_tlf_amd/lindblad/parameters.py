@@ -20,9 +20,9 @@ from __future__ import annotations
 
 import functools
 import numbers
+import operator
 from collections import OrderedDict
 from collections.abc import Mapping, Sequence
-from dataclasses import dataclass, field
 from typing import Any
 
 import numpy as np
@@ -75,6 +75,9 @@ def _normalize_symbol_name(value: str) -> str:
     return value.replace("ϕ", "φ")
 
 
+# ---- stored base values
+
+
 @functools.singledispatch
 def _slot_value(value: Any) -> complex | tuple[complex, ...]:
     """A base parameter's stored value: a complex scalar, or a tuple of them for
@@ -112,186 +115,180 @@ def _infer_kind(default: Any) -> str:
     return "complex" if any(part.imag != 0.0 for part in parts) else "real"
 
 
-def _parse_expr(expr: str) -> smp.Expr:
-    local_dict = {name: smp.Function(name) for name in HELPER_FUNCTIONS}
-    return sympy_parser.parse_expr(_normalize_symbol_name(expr), local_dict=local_dict)
+# helper names parse as undefined functions, so the lowering sees the call
+_HELPER_CALLS = {name: smp.Function(name) for name in HELPER_FUNCTIONS}
 
 
-def _merge(left: Mapping[str, "Parameter"], right: Mapping[str, "Parameter"]) -> dict:
-    merged = dict(left)
-    for name, parameter in right.items():
-        existing = merged.get(name)
-        if existing is not None and existing is not parameter:
-            raise ValueError(f"duplicate Parameter name {name!r}")
-        merged[name] = parameter
-    return merged
+def _parse_expr(text: str) -> smp.Expr:
+    return sympy_parser.parse_expr(_normalize_symbol_name(text), local_dict=dict(_HELPER_CALLS))
 
 
-class _Arith:
-    """Operator overloads shared by Parameter and RuntimeExpression."""
-
-    def _rt(self) -> "RuntimeExpression":
-        raise NotImplementedError
-
-    def __neg__(self):
-        rt = self._rt()
-        return RuntimeExpression(-rt.expr, rt.parameters)
-
-    def __add__(self, other):
-        return _binary(self._rt(), other, lambda a, b: a + b)
-
-    def __radd__(self, other):
-        return _binary(_to_runtime_expression(other), self._rt(), lambda a, b: a + b)
-
-    def __sub__(self, other):
-        return _binary(self._rt(), other, lambda a, b: a - b)
-
-    def __rsub__(self, other):
-        return _binary(_to_runtime_expression(other), self._rt(), lambda a, b: a - b)
-
-    def __mul__(self, other):
-        return _binary(self._rt(), other, lambda a, b: a * b)
-
-    def __rmul__(self, other):
-        return _binary(_to_runtime_expression(other), self._rt(), lambda a, b: a * b)
-
-    def __truediv__(self, other):
-        return _binary(self._rt(), other, lambda a, b: a / b)
-
-    def __rtruediv__(self, other):
-        return _binary(_to_runtime_expression(other), self._rt(), lambda a, b: a / b)
-
-    def __pow__(self, other):
-        return _binary(self._rt(), other, lambda a, b: a**b)
-
-    def __rpow__(self, other):
-        return _binary(_to_runtime_expression(other), self._rt(), lambda a, b: a**b)
+# ---- expression objects
 
 
-@dataclass(frozen=True, eq=False)
-class Parameter(_Arith):
-    """A registered base parameter that may be scanned or used in expressions."""
+def _freeze(obj: Any, **attributes: Any) -> None:
+    for key, value in attributes.items():
+        object.__setattr__(obj, key, value)
 
-    name: str
-    default: Any = 0.0
-    kind: str = "real"
-    symbol: smp.Symbol = field(init=False)
 
-    def __post_init__(self) -> None:
-        normalized = _normalize_symbol_name(self.name)
-        if self.kind not in {"real", "complex"}:
+def _joined(*maps: Mapping[str, "Parameter"]) -> dict[str, "Parameter"]:
+    """The union of the parameter maps of several operands; a name must stand
+    for one Parameter object throughout an expression."""
+    union: dict[str, Parameter] = {}
+    for mapping in maps:
+        for key, param in mapping.items():
+            if union.setdefault(key, param) is not param:
+                raise ValueError(f"duplicate Parameter name {key!r}")
+    return union
+
+
+def _combine(build, *operands: Any) -> "RuntimeExpression":
+    """``build`` applied to the sympy forms of ``operands`` (each coerced to a
+    RuntimeExpression), carrying the union of their parameters."""
+    terms = [_to_runtime_expression(item) for item in operands]
+    return RuntimeExpression(build(*(term.expr for term in terms)), _joined(*(term.parameters for term in terms)))
+
+
+class _Immutable:
+    __slots__ = ()
+
+    def __setattr__(self, key: str, value: Any) -> None:
+        raise AttributeError(f"cannot assign to field {key!r} of an immutable {type(self).__name__}")
+
+    def __delattr__(self, key: str) -> None:
+        raise AttributeError(f"cannot delete field {key!r} of an immutable {type(self).__name__}")
+
+
+def _with_arithmetic(cls):
+    """Install +, -, *, /, ** (both operand orders) and unary minus; every
+    result is a RuntimeExpression over the joined parameters."""
+    for dunder, op in (("add", operator.add), ("sub", operator.sub), ("mul", operator.mul),
+                       ("truediv", operator.truediv), ("pow", operator.pow)):
+        setattr(cls, f"__{dunder}__", lambda self, other, op=op: _combine(op, self, other))
+        setattr(cls, f"__r{dunder}__", lambda self, other, op=op: _combine(op, other, self))
+    cls.__neg__ = lambda self: _combine(operator.neg, self)
+    return cls
+
+
+@_with_arithmetic
+class Parameter(_Immutable):
+    """A registered base parameter that may be scanned or used in expressions.
+
+    ``Parameter(name, default=0.0, kind="real")``; ``symbol`` is the sympy
+    symbol of the normalised name with the kind's assumption.  Two Parameter
+    objects are equal only when they are the same object.
+    """
+
+    __slots__ = ("name", "default", "kind", "symbol")
+
+    def __init__(self, name: str, default: Any = 0.0, kind: str = "real") -> None:
+        key = _normalize_symbol_name(name)
+        if kind not in ("real", "complex"):
             raise ValueError("kind must be 'real' or 'complex'")
-        object.__setattr__(self, "name", normalized)
-        assumptions = {"real": True} if self.kind == "real" else {"complex": True}
-        object.__setattr__(self, "symbol", smp.Symbol(normalized, **assumptions))
+        _freeze(self, name=key, default=default, kind=kind, symbol=smp.Symbol(key, **{kind: True}))
 
-    def __hash__(self) -> int:
-        return id(self)
+    __hash__ = object.__hash__
+
+    def __repr__(self) -> str:
+        return f"Parameter(name={self.name!r}, default={self.default!r}, kind={self.kind!r})"
 
     def as_expression(self) -> "RuntimeExpression":
         return RuntimeExpression(self.symbol, {self.name: self})
 
-    def _rt(self) -> "RuntimeExpression":
-        return self.as_expression()
 
-
-@dataclass(frozen=True)
-class RuntimeExpression(_Arith):
+@_with_arithmetic
+class RuntimeExpression(_Immutable):
     """A sympy expression plus the registered parameters it depends on."""
 
-    expr: smp.Expr
-    parameters: Mapping[str, Parameter] = field(default_factory=dict)
+    __slots__ = ("expr", "parameters")
 
-    def __post_init__(self) -> None:
-        object.__setattr__(self, "expr", smp.sympify(self.expr))
-        object.__setattr__(self, "parameters", dict(self.parameters))
+    def __init__(self, expr: Any, parameters: Mapping[str, Parameter] | None = None) -> None:
+        _freeze(self, expr=smp.sympify(expr), parameters=dict(parameters or {}))
 
-    def _rt(self) -> "RuntimeExpression":
-        return self
+    def __eq__(self, other: Any) -> bool:
+        if type(other) is not RuntimeExpression:
+            return NotImplemented
+        return self.expr == other.expr and self.parameters == other.parameters
 
-    def _substituted(self, variable: str | None, overrides: Mapping[str, Any]) -> smp.Expr:
-        by_name = {s.name: s for s in self.expr.free_symbols}
-        subs = []
-        remaining = dict(overrides)
-        for name, param in self.parameters.items():
-            if name == variable or name not in by_name:
-                continue
+    __hash__ = None  # the parameter map is mutable
+
+    def _bound(self, free: str | None, overrides: Mapping[str, Any]) -> smp.Expr:
+        """The expression with every symbol but ``free`` replaced: a tuple
+        parameter by its table (overrides do not apply to tables), a scalar
+        parameter by its override or default, any other symbol (``t``) by an
+        override of its name."""
+        values: dict[str, Any] = dict(overrides)
+        for key, param in self.parameters.items():
             if isinstance(param.default, tuple):
-                subs.append((by_name[name], smp.Tuple(*param.default)))
-            elif name in remaining:
-                subs.append((by_name[name], remaining.pop(name)))
+                values[key] = smp.Tuple(*param.default)
             else:
-                subs.append((by_name[name], param.default))
-        for key, val in remaining.items():
-            if key != variable and key in by_name:
-                subs.append((by_name[key], val))
-        return self.expr.subs(subs)
+                values.setdefault(key, param.default)
+        values.pop(free, None)
+        pairs = [(sym, values[sym.name]) for sym in self.expr.free_symbols if sym.name in values]
+        return self.expr.subs(pairs)
+
+    def _variable(self, name: str) -> smp.Symbol:
+        return next((sym for sym in self.expr.free_symbols if sym.name == name), smp.Symbol(name))
 
     def evaluate(self, **overrides: float | complex) -> float | complex:
-        """Evaluate with parameter defaults (or ``overrides``), including ``t``."""
-        expr = self._substituted(None, overrides)
-        if not expr.free_symbols:
-            result = smp.lambdify([smp.Symbol("_unused")], expr,
-                                  modules=[HELPER_FUNCTIONS, "numpy"])(0)
-            if isinstance(result, complex) or np.iscomplexobj(result):
-                return complex(result)
-            return float(result)
-        return complex(expr)
+        """Evaluate with parameter defaults (or ``overrides``), including ``t``.
+        An expression with symbols left unbound is returned as it stands."""
+        bound = self._bound(None, overrides)
+        if bound.free_symbols:
+            try:
+                return complex(bound)
+            except (TypeError, ValueError):
+                return bound
+        value = smp.lambdify([], bound, modules=[HELPER_FUNCTIONS, "numpy"])()
+        return complex(value) if np.iscomplexobj(value) else float(value)
+
+    def compile_callable(self, variable: str = "t", **overrides: float | complex):
+        """A numpy function of ``variable`` with every other symbol bound."""
+        return smp.lambdify(self._variable(variable), self._bound(variable, overrides),
+                            modules=[HELPER_FUNCTIONS, "numpy"])
 
     def evaluate_array(self, variable: str, values: np.ndarray, **overrides) -> np.ndarray:
-        expr = self._substituted(variable, overrides)
-        sym = {s.name: s for s in self.expr.free_symbols}.get(variable, smp.Symbol(variable))
-        func = smp.lambdify(sym, expr, modules=[HELPER_FUNCTIONS, "numpy"])
-        return np.asarray(func(np.asarray(values)), dtype=np.float64)
-
-    def compile_callable(self, variable: str = "t", **overrides):
-        expr = self._substituted(variable, overrides)
-        sym = {s.name: s for s in self.expr.free_symbols}.get(variable, smp.Symbol(variable))
-        return smp.lambdify(sym, expr, modules=[HELPER_FUNCTIONS, "numpy"])
+        return np.asarray(self.compile_callable(variable, **overrides)(np.asarray(values)), dtype=np.float64)
 
     def __repr__(self) -> str:
-        return f"RuntimeExpression({self.expr})"
+        scalars = ", ".join(f"{key}={param.default}" for key, param in self.parameters.items()
+                            if not isinstance(param.default, tuple))
+        return f"RuntimeExpression({self.expr}, {{{scalars}}})" if scalars else f"RuntimeExpression({self.expr})"
 
 
 ExpressionLike = RuntimeExpression | Parameter | smp.Expr | int | float | complex
 
 
 def _to_runtime_expression(value: Any) -> RuntimeExpression:
-    if isinstance(value, RuntimeExpression):
-        return value
-    if isinstance(value, Parameter):
-        return value.as_expression()
-    if isinstance(value, smp.Expr):
-        return RuntimeExpression(value, {})
-    if isinstance(value, np.generic):
-        return RuntimeExpression(smp.sympify(value.item()), {})
-    if isinstance(value, numbers.Number):
-        return RuntimeExpression(smp.sympify(value), {})
+    match value:
+        case RuntimeExpression():
+            return value
+        case Parameter():
+            return value.as_expression()
+        case smp.Expr():
+            return RuntimeExpression(value)
+        case np.generic():
+            return RuntimeExpression(smp.sympify(value.item()))
+        case numbers.Number():
+            return RuntimeExpression(smp.sympify(value))
     raise TypeError(f"unsupported runtime expression type {type(value).__name__}")
 
 
-def _binary(left: RuntimeExpression, right: Any, op) -> RuntimeExpression:
-    rhs = _to_runtime_expression(right)
-    return RuntimeExpression(op(left.expr, rhs.expr), _merge(left.parameters, rhs.parameters))
-
-
 def _call_function(name: str, *args: Any) -> RuntimeExpression:
-    coerced = [_to_runtime_expression(arg) for arg in args]
-    parameters: dict[str, Parameter] = {}
-    for arg in coerced:
-        parameters = _merge(parameters, arg.parameters)
-    return RuntimeExpression(smp.Function(name)(*(arg.expr for arg in coerced)), parameters)
+    """``name(args...)`` as an undefined sympy function (a helper call)."""
+    return _combine(smp.Function(name), *args)
 
 
 def _helper_or_value(name: str, *args: Any):
-    if any(isinstance(a, Parameter | RuntimeExpression | smp.Expr) for a in args):
-        return _call_function(name, *args)
-    return HELPER_FUNCTIONS[name](*args)
+    """A helper call: symbolic when any argument is an expression, else the
+    helper's value."""
+    symbolic = (Parameter, RuntimeExpression, smp.Expr)
+    return _call_function(name, *args) if any(isinstance(a, symbolic) for a in args) else HELPER_FUNCTIONS[name](*args)
 
 
 def Time() -> RuntimeExpression:
     """The integration time ``t`` (``parameters.py:437``)."""
-    return RuntimeExpression(smp.Symbol("t", real=True), {})
+    return RuntimeExpression(smp.Symbol("t", real=True))
 
 
 def gaussian_1d(x, center, sigma):
@@ -370,42 +367,53 @@ def pchip_interp(x, grid, values):
     return _helper_or_value("pchip_interp", x, grid, values)
 
 
+# Profiles built from sympy operations.  The operations run in the order the
+# formulas are written, so the stored expression trees (and the bytecode
+# lowered from them) do not depend on how a profile was assembled.
+
+
 def linear(x, offset=0.0, slope=1.0) -> RuntimeExpression:
-    return _to_runtime_expression(offset) + (_to_runtime_expression(slope) * _to_runtime_expression(x))
+    """``offset + slope * x``."""
+    return _combine(lambda off, k, xv: off + k * xv, offset, slope, x)
 
 
 def sine(x, offset=0.0, amplitude=1.0, angular_frequency=1.0, phase=0.0) -> RuntimeExpression:
-    arg = _to_runtime_expression(angular_frequency) * _to_runtime_expression(x) + phase
-    return _to_runtime_expression(offset) + _to_runtime_expression(amplitude) * RuntimeExpression(
-        smp.sin(arg.expr), arg.parameters
-    )
+    """``offset + amplitude * sin(angular_frequency * x + phase)``."""
+    return _combine(lambda off, amp, w, xv, ph: off + amp * smp.sin(w * xv + ph),
+                    offset, amplitude, angular_frequency, x, phase)
 
 
 def gaussian(x, center=0.0, sigma=1.0, amplitude=1.0, baseline=0.0) -> RuntimeExpression:
     """``baseline + amplitude * exp(-(x - center)^2 / (2 sigma^2))`` (``parameters.py:769-782``)."""
-    dx = _to_runtime_expression(x) - center
-    width = _to_runtime_expression(sigma)
-    exponent = -(dx**2) / (2.0 * width**2)
-    return _to_runtime_expression(baseline) + _to_runtime_expression(amplitude) * RuntimeExpression(
-        smp.exp(exponent.expr), exponent.parameters
-    )
+    return _combine(lambda base, amp, xv, mu, sd: base + amp * smp.exp(-((xv - mu) ** 2) / (2.0 * sd**2)),
+                    baseline, amplitude, x, center, sigma)
 
 
 def square_wave_profile(x, low=0.0, high=1.0, period=1.0, phase=0.0, duty=0.5) -> RuntimeExpression:
-    on = _call_function("variable_on_off_duty", x, duty, 1.0 / _to_runtime_expression(period), phase)
-    return _to_runtime_expression(low) + (_to_runtime_expression(high) - low) * on
+    """``low`` and ``high`` alternating with ``period``, ``high`` for the
+    fraction ``duty`` of each period (the ``variable_on_off_duty`` helper)."""
+    on = smp.Function("variable_on_off_duty")
+    return _combine(lambda lo, hi, xv, dv, per, ph: lo + (hi - lo) * on(xv, dv, 1.0 / per, ph),
+                    low, high, x, duty, period, phase)
+
+
+def _table_lookup(helper: str, caller: str, x, grid, values) -> RuntimeExpression:
+    if not (isinstance(grid, Parameter) and isinstance(values, Parameter)):
+        raise TypeError(f"{caller} requires grid and values to be registered tuple Parameters")
+    return _call_function(helper, x, grid, values)
 
 
 def tabulated(x, grid: Parameter, values: Parameter) -> RuntimeExpression:
-    if not isinstance(grid, Parameter) or not isinstance(values, Parameter):
-        raise TypeError("tabulated requires grid and values to be registered tuple Parameters")
-    return _call_function("linear_interp", x, grid, values)
+    """Linear interpolation in a table held by two tuple parameters."""
+    return _table_lookup("linear_interp", "tabulated", x, grid, values)
 
 
 def pchip_tabulated(x, grid: Parameter, values: Parameter) -> RuntimeExpression:
-    if not isinstance(grid, Parameter) or not isinstance(values, Parameter):
-        raise TypeError("pchip_tabulated requires grid and values to be registered tuple Parameters")
-    return _call_function("pchip_interp", x, grid, values)
+    """Monotone cubic (PCHIP) interpolation in a table held by two tuple parameters."""
+    return _table_lookup("pchip_interp", "pchip_tabulated", x, grid, values)
+
+
+# ---- the parameter set
 
 
 def _is_binding_mapping(mapping: Mapping[Any, Any]) -> bool:
@@ -434,6 +442,21 @@ def _greedy_dependency_order(names: Sequence[str], needs: Mapping[str, set[str]]
             raise ValueError("could not resolve compound parameter dependency order")
         pending = [name for name in pending if name not in resolved]
     return placed
+
+
+def _compound_form(expression: Any) -> smp.Expr:
+    """The sympy form of a compound's right-hand side: expression text is
+    parsed (helpers as function calls), numbers become constants."""
+    match expression:
+        case str():
+            return _parse_expr(expression)
+        case RuntimeExpression():
+            return expression.expr
+        case smp.Expr():
+            return expression
+        case numbers.Number():
+            return smp.sympify(expression)
+    raise TypeError(f"unsupported compound expression type {type(expression).__name__}")
 
 
 class LindbladParameters:
@@ -544,16 +567,7 @@ class LindbladParameters:
         self.base_parameters[_normalize_symbol_name(name)] = _slot_value(value)
 
     def _store_compound(self, name: str, expression: Any) -> None:
-        if isinstance(expression, RuntimeExpression):
-            expr = expression.expr
-        elif isinstance(expression, smp.Expr):
-            expr = expression
-        elif isinstance(expression, str):
-            expr = _parse_expr(expression)
-        elif isinstance(expression, numbers.Number):
-            expr = smp.sympify(expression)
-        else:
-            raise TypeError(f"unsupported compound expression type {type(expression).__name__}")
+        expr = _compound_form(expression)
         key = _normalize_symbol_name(name)
         self._exprs[key] = expr
         self.compound_parameters[key] = _normalize_symbol_name(str(expr))
@@ -581,41 +595,48 @@ class LindbladParameters:
 
 
 def adapt_lindblad_parameters(parameters: Any) -> LindbladParameters:
+    """A LindbladParameters from one, or from a mapping: names to numbers and
+    expression text (the keyword form), or symbols to expressions (bindings)."""
     if isinstance(parameters, LindbladParameters):
         return parameters
-    if isinstance(parameters, Mapping):
-        if any(not isinstance(k, str) or isinstance(v, RuntimeExpression | Parameter | smp.Expr)
-               for k, v in parameters.items()):
-            return LindbladParameters(parameters)
-        return LindbladParameters.from_kwargs(**dict(parameters))
-    raise TypeError("parameters must be a LindbladParameters instance or a mapping")
+    if not isinstance(parameters, Mapping):
+        raise TypeError("parameters must be a LindbladParameters instance or a mapping")
+    keyword_form = all(isinstance(key, str) for key in parameters) and not _is_binding_mapping(parameters)
+    return LindbladParameters.from_kwargs(**dict(parameters)) if keyword_form else LindbladParameters(parameters)
+
+
+def _transition_defaults(idx: int, selector: Any, gamma: float):
+    """(name, value) pairs of one transition's default parameters: its Rabi
+    rate (gamma, or gamma times a phase modulation), a zero detuning, and its
+    polarisation (a constant 1 for one symbol; two symbols switched by a
+    square wave of angular frequency gamma)."""
+    rabi = str(selector.Ω)
+    if getattr(selector, "phase_modulation", False):
+        yield rabi, f"Ωt{idx}*phase_modulation(t, β{idx}, ωphase{idx})"
+        yield f"Ωt{idx}", gamma
+        yield f"β{idx}", 3.8
+        yield f"ωphase{idx}", gamma
+    else:
+        yield rabi, gamma
+    yield str(selector.δ), 0.0
+    polarization = [str(sym) for sym in getattr(selector, "polarization_symbols", [])]
+    if len(polarization) > 2:
+        raise ValueError("polarization switching with more than two symbols is not supported")
+    if len(polarization) == 1:
+        yield polarization[0], 1.0
+    elif len(polarization) == 2:
+        first, second = polarization
+        yield f"ω{idx}", gamma
+        yield f"φ{idx}", 0.0
+        yield first, f"square_wave(t, ω{idx}, φ{idx})"
+        yield second, f"1 - {first}"
 
 
 def generate_lindblad_parameters(transition_selectors: Sequence[Any], gamma: float = GAMMA_DEFAULT,
                                  **kwargs: Any) -> LindbladParameters:
-    """Default parameter set for transition selectors (``parameters.py:1051-1089``)."""
-    entries: list[tuple[str, Any]] = []
-    for idx, selector in enumerate(transition_selectors):
-        if getattr(selector, "phase_modulation", False):
-            entries += [
-                (str(selector.Ω), f"Ωt{idx}*phase_modulation(t, β{idx}, ωphase{idx})"),
-                (f"Ωt{idx}", gamma), (f"β{idx}", 3.8), (f"ωphase{idx}", gamma),
-            ]
-        else:
-            entries.append((str(selector.Ω), gamma))
-        entries.append((str(selector.δ), 0.0))
-        symbols = list(getattr(selector, "polarization_symbols", []))
-        if len(symbols) == 1:
-            entries.append((str(symbols[0]), 1.0))
-        elif len(symbols) == 2:
-            entries += [
-                (f"ω{idx}", gamma), (f"φ{idx}", 0.0),
-                (str(symbols[0]), f"square_wave(t, ω{idx}, φ{idx})"),
-                (str(symbols[1]), f"1 - {symbols[0]}"),
-            ]
-        elif len(symbols) > 2:
-            raise ValueError("polarization switching with more than two symbols is not supported")
-    merged = OrderedDict(entries)
-    merged.update(kwargs)
-    return adapt_lindblad_parameters(merged)
-
+    """Default parameter set for transition selectors (``parameters.py:1051-1089``);
+    keyword arguments replace or add entries."""
+    entries = OrderedDict(pair for idx, selector in enumerate(transition_selectors)
+                          for pair in _transition_defaults(idx, selector, gamma))
+    entries.update(kwargs)
+    return adapt_lindblad_parameters(entries)
