@@ -266,6 +266,32 @@ def parameter_scan(
                                 parameter_batch=values, **kwargs)
 
 
+# grid_scan's keywords: the batch call's, with the same defaults; a conversion
+# (or None) applied to the given or default value.  use_split_input_rhs and
+# parallel are accepted and have no effect (one GPU kernel family serves every
+# system size, and trajectories always run in parallel).
+_GRID_OPTIONS: tuple[tuple[str, Any, Any], ...] = (
+    ("save_start", True, bool), ("saveat", None, None), ("solver", "dopri5", None),
+    ("execution_mode", "expanded_sparse", None), ("use_split_input_rhs", True, None),
+    ("abstol", 1e-7, float), ("reltol", 1e-4, float), ("dt", 1e-8, float), ("maxiters", 100_000, int),
+    ("collect_stats", False, bool), ("output", "populations", None), ("output_indices", None, None),
+    ("output_when", "final", None), ("integral_weights", None, None), ("dense_output", True, bool),
+    ("parallel", True, None), ("threads", None, None), ("metadata", None, None), ("stop_event", None, None),
+)
+
+
+def _grid_options(given: Mapping[str, Any]) -> dict[str, Any]:
+    known = {name for name, _, _ in _GRID_OPTIONS}
+    extra = sorted(set(given) - known)
+    if extra:
+        raise TypeError(f"unexpected grid_scan keyword argument(s): {', '.join(extra)}")
+    options = {}
+    for name, default, convert in _GRID_OPTIONS:
+        value = given.get(name, default)
+        options[name] = value if convert is None else convert(value)
+    return options
+
+
 def grid_scan(
     prepared: PreparedLindbladProblem,
     rho0: npt.NDArray[np.complex128] | npt.NDArray[np.float64],
@@ -286,28 +312,14 @@ def grid_scan(
     slot_indices = D.parameter_slot_indices(prepared, parameter_slots)
     packed = _pack_rho0(prepared, rho0)
     t_span_tuple = D.normalize_t_span(t_span)
-    save_start = bool(kwargs.pop("save_start", True))
-    saveat_values = D.normalize_saveat(kwargs.pop("saveat", None), t_span_tuple, save_start)
-    solver = kwargs.pop("solver", "dopri5")
-    execution_mode = kwargs.pop("execution_mode", "expanded_sparse")
-    kwargs.pop("use_split_input_rhs", True)
-    abstol = float(kwargs.pop("abstol", 1e-7))
-    reltol = float(kwargs.pop("reltol", 1e-4))
-    dt = float(kwargs.pop("dt", 1e-8))
-    maxiters = int(kwargs.pop("maxiters", 100_000))
-    collect_stats = bool(kwargs.pop("collect_stats", False))
-    output = kwargs.pop("output", "populations")
-    output_indices = kwargs.pop("output_indices", None)
-    output_when = kwargs.pop("output_when", "final")
-    integral_weights = kwargs.pop("integral_weights", None)
-    dense_output = bool(kwargs.pop("dense_output", True))
-    kwargs.pop("parallel", True)
-    threads = kwargs.pop("threads", None)
-    metadata = kwargs.pop("metadata", None)
-    stop_event = kwargs.pop("stop_event", None)
-    if kwargs:
-        unknown = ", ".join(sorted(kwargs))
-        raise TypeError(f"unexpected grid_scan keyword argument(s): {unknown}")
+    o = _grid_options(kwargs)
+    save_start = o["save_start"]
+    saveat_values = D.normalize_saveat(o["saveat"], t_span_tuple, save_start)
+    solver, execution_mode, output = o["solver"], o["execution_mode"], o["output"]
+    abstol, reltol, dt, maxiters = o["abstol"], o["reltol"], o["dt"], o["maxiters"]
+    output_indices, output_when, integral_weights = o["output_indices"], o["output_when"], o["integral_weights"]
+    collect_stats, dense_output, threads = o["collect_stats"], o["dense_output"], o["threads"]
+    metadata, stop_event = o["metadata"], o["stop_event"]
     mode = _common_checks(prepared, solver=solver, execution_mode=execution_mode, output=output,
                           output_indices=output_indices, output_when=output_when,
                           integral_weights=integral_weights, saveat_values=saveat_values,
