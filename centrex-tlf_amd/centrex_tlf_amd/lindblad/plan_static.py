@@ -96,24 +96,25 @@ def lower_structured_jumps(c_array: np.ndarray):
     Each ``C_k`` must have exactly one nonzero ``C_k[target, source]``; its rate
     ``|C|^2`` leaves ``source`` and arrives at ``target``.
     """
-    c_array = np.asarray(c_array, dtype=np.complex128)
-    n = int(c_array.shape[1])
-    jumps: list[dict[str, Any]] = []
-    source_decay = np.zeros(n, dtype=np.float64)
+    ops = np.asarray(c_array, dtype=np.complex128)
+    k, n = int(ops.shape[0]), int(ops.shape[1])
+    present = (np.abs(ops) > 0).reshape(k, n * n)
+    counts = present.sum(axis=1)
+    wrong = np.flatnonzero(counts != 1)
+    if wrong.size:
+        first = int(wrong[0])
+        raise AssertionError(
+            f"collapse operator {first} is not a single-jump operator; found {int(counts[first])} nonzero elements")
+    target, source = np.divmod(present.argmax(axis=1), n)
+    amp = ops[np.arange(k), target, source]
+    rate = (amp * amp.conj()).real
+    # bincount adds in operator order, as a running sum over the operators would
+    source_decay = np.bincount(source, weights=rate, minlength=n).astype(np.float64)
+    jumps = [{"target": int(t), "source": int(s_), "amp_re": float(a.real), "amp_im": float(a.imag),
+              "rate": float(r)} for t, s_, a, r in zip(target, source, amp, rate)]
     incoming: list[list[dict[str, Any]]] = [[] for _ in range(n)]
-    for idx, collapse in enumerate(c_array):
-        nz = np.argwhere(np.abs(collapse) > 0)
-        if nz.shape[0] != 1:
-            raise AssertionError(
-                f"collapse operator {idx} is not a single-jump operator; found {nz.shape[0]} nonzero elements"
-            )
-        target, source = int(nz[0, 0]), int(nz[0, 1])
-        amplitude = complex(collapse[target, source])
-        rate = float((amplitude * np.conjugate(amplitude)).real)
-        jumps.append({"target": target, "source": source, "amp_re": amplitude.real,
-                      "amp_im": amplitude.imag, "rate": rate})
-        incoming[target].append({"source": source, "rate": rate})
-        source_decay[source] += rate
+    for jump in jumps:
+        incoming[jump["target"]].append({"source": jump["source"], "rate": jump["rate"]})
     return jumps, source_decay, incoming
 
 
