@@ -52,6 +52,25 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+# LDS of a CU: 128 bytes per clock (the co-bound's peak: 256 CUs at 2.4 GHz)
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12
+
+
+def lds_bytes_per_rhs(prepared) -> int:
+    """Algorithmic LDS bytes of one RHS of the full plan: 8 bytes per term whose
+    input is a real (diagonal) element, 16 per complex one, plus 8*dim for
+    publishing the stage vector that the gathers read."""
+    import numpy as np
+
+    from centrex_tlf_amd.lindblad.state_layout import upper_offsets
+
+    plan = prepared.expanded_rhs_plan
+    n = int(plan["n"])
+    inputs = np.asarray(plan["input_indices"], dtype=np.int64)
+    diagonal = np.isin(inputs, upper_offsets(n))
+    return int(np.where(diagonal, 8, 16).sum()) + 8 * n * n
+
+
 def algorithmic_flops(stats, flops_per_rhs: int, dim: int, solver: str) -> float:
     """Sum over trajectories of rhs_calls*F_rhs + attempted_steps*dim*F_rk.
 
@@ -742,6 +761,8 @@ def main() -> int:
     executed = algorithmic_flops(stats_np, rinfo["flops_per_rhs"], rinfo["dim"], args.solver)
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
     achieved_tflops = flops / avg_kernel_s / 1e12
+    lds_rhs = lds_bytes_per_rhs(prepared)
+    lds_bytes = float(stats_np[:, 2].astype(np.float64).sum()) * lds_rhs
     traffic = None
     if TRAFFIC_FILE.exists():
         traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
@@ -791,6 +812,16 @@ def main() -> int:
                                 f"F_rk={F_RK[args.solver]}",
             "executed_flops_per_launch": executed,
             "executed_frac": executed / avg_kernel_s / 1e12 / FP64_PEAK_TFLOPS,
+            "lds_cobound": {
+                "achieved": lds_bytes / avg_kernel_s / 1e12,
+                "peak": LDS_PEAK_TBS,
+                "unit": "TB/s",
+                "frac": lds_bytes / avg_kernel_s / 1e12 / LDS_PEAK_TBS,
+                "bytes_per_launch": lds_bytes,
+                "definition": "sum over trajectories of rhs_calls * (sum over terms of 8 bytes per real input, 16 "
+                              f"per complex, + 8*dim), full plan: {lds_rhs} bytes per RHS; peak 256 CUs x 128 B/clk "
+                              "x 2.4 GHz",
+            },
         },
         "totals": {
             "accepted_steps": int(stats_np[:, 0].sum()),
