@@ -116,3 +116,23 @@ def test_entrywise_plan_rhs_matches_dense_tlf_n39():
     want = prepared.layout.pack(dense_rhs(prepared, rho, t))
     scale = np.abs(want).max()
     np.testing.assert_allclose(got, want, atol=1e-12 * scale, rtol=0)
+
+
+def test_structured_jumps_of_single_jump_operators():
+    """Rates |C|^2 of each operator's one element C[target, source]: per-jump
+    records, the decay rate summed per source and the transfers per target."""
+    from centrex_tlf_amd.lindblad.plan_static import lower_structured_jumps
+
+    c = np.zeros((3, 3, 3), dtype=np.complex128)
+    c[0, 0, 2] = 0.6 + 0.8j  # 2 -> 0, rate 1.0
+    c[1, 1, 2] = 2.0  # 2 -> 1, rate 4.0
+    c[2, 0, 1] = -0.5j  # 1 -> 0, rate 0.25
+    jumps, decay, incoming = lower_structured_jumps(c)
+    assert [(j["target"], j["source"]) for j in jumps] == [(0, 2), (1, 2), (0, 1)]
+    np.testing.assert_allclose([j["rate"] for j in jumps], [1.0, 4.0, 0.25], rtol=1e-15)
+    assert (jumps[0]["amp_re"], jumps[0]["amp_im"]) == (0.6, 0.8)
+    np.testing.assert_allclose(decay, [0.0, 0.25, 5.0], rtol=1e-15)
+    assert [[t["source"] for t in row] for row in incoming] == [[2, 1], [2], []]
+    c[1, 0, 0] = 1.0  # a second element in operator 1
+    with pytest.raises(AssertionError, match="collapse operator 1 is not a single-jump operator; found 2"):
+        lower_structured_jumps(c)
