@@ -1504,11 +1504,27 @@ int lb_adjoint_launch(const lb_plan* p, const lb_solve_args* a, void* stream, vo
   if (id < 0) return fail(LB_ERR_UNSUPPORTED, "plan is too large for the adjoint kernel variants");
   void* fn = lb_dev_adjoint_kernel(id, a->solver);
   if (!fn) return fail(LB_ERR_UNSUPPORTED, "adjoint kernel variant is not built");
-  // the multi-wave form evaluates the coefficient rows of E attempts per pass:
-  // the largest E of 4, 2 whose LDS leaves room for two workgroups per CU, else 1
+  // the multi-wave form evaluates the coefficient rows of E attempts per pass
+  // (7E lanes): the largest E of 8, 4, 2 whose LDS keeps the workgroups per CU
+  // that E = 1 has (the registers often allow one workgroup per CU anyway)
   int row_batch = 1;
-  for (int e : {4, 2})
-    if (lds_bytes_adj(im->dev, e) <= kMaxLdsBytes / 2) { row_batch = e; break; }
+  if (!wave) {
+    const int64_t lds1 = lds_bytes_adj(im->dev, 1);
+    if ((rc = set_lds_limit(fn, (size_t)lds1))) return rc;
+    int base_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&base_cu, fn, W, (size_t)lds1));
+    for (int e : {8, 4, 2}) {
+      const int64_t le = lds_bytes_adj(im->dev, e);
+      if (le > kMaxLdsBytes || 7 * e > W) continue;
+      if ((rc = set_lds_limit(fn, (size_t)le))) return rc;
+      int per_cu = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, W, (size_t)le));
+      if (per_cu >= base_cu && per_cu > 0) {
+        row_batch = e;
+        break;
+      }
+    }
+  }
   c.adj_row_batch = row_batch;
   const size_t lds = (size_t)(wave ? lds_wave : lds_bytes_adj(im->dev, row_batch));
   if ((rc = set_lds_limit(fn, lds))) return rc;
